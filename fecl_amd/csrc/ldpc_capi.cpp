@@ -1,0 +1,584 @@
+// C ABI of the decoder (include/fecl_ldpc.h): handles, batch engine, host/device entry points.
+//
+// Reference behaviour mirrored here:
+//   size validation and errors     src/Codec.h:140-143, 210-216, 249-270
+//   options / scaling table        src/detail/Ldpc.h:56-74, src/detail/Ldpc.cpp:57-112, 134-138
+//   encode / check                 src/detail/Ldpc.cpp:160-209
+//   decode / soDecode semantics    src/detail/BpDecoder/BpDecoderImpl.cpp:38-130
+// Frames are independent, so a call splits its frames into chunks that fit a cached device
+// workspace and decodes each chunk as one batch of tiles (no cross-frame arithmetic).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/fecl_ldpc.h"
+#include "ldpc_code.h"
+#include "ldpc_launch.h"
+
+namespace {
+
+using namespace fecl;
+
+thread_local std::string g_err;
+
+int fail(int status, const std::string& msg) {
+  g_err = msg;
+  return status;
+}
+
+#define HIP_TRY(expr)                                                                            \
+  do {                                                                                           \
+    hipError_t e_ = (expr);                                                                      \
+    if (e_ != hipSuccess)                                                                        \
+      return fail(e_ == hipErrorOutOfMemory ? FECL_OUT_OF_MEMORY : FECL_DEVICE_ERROR,            \
+                  std::string(#expr) + ": " + hipGetErrorString(e_));                            \
+  } while (0)
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  hipError_t ensure(size_t n) {
+    if (n <= bytes) return hipSuccess;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    hipError_t e = hipMalloc(&p, n);
+    if (e == hipSuccess) bytes = n;
+    return e;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <typename T> T* as() const { return static_cast<T*>(p); }
+};
+
+struct Workspace {
+  DevBuf msg, L, hard, phase, iter, fail, flags, active, out_hard, out_ext, out_app;
+  DevBuf stage_in, stage_syst, stage_state, stage_out;
+  int32_t* host_active = nullptr;
+  int host_active_cap = 0;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t done = nullptr;   // recorded after the last use, waited on by the next user's stream
+  std::vector<std::array<hipEvent_t, 4>> pev;   // per-step timing events (profiling only)
+  hipStream_t own_stream = nullptr;
+  ~Workspace() {
+    for (DevBuf* b : {&msg, &L, &hard, &phase, &iter, &fail, &flags, &active, &out_hard, &out_ext, &out_app,
+                      &stage_in, &stage_syst, &stage_state, &stage_out})
+      b->release();
+    if (host_active) (void)hipHostFree(host_active);
+    for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+    if (done) (void)hipEventDestroy(done);
+    for (auto& a : pev)
+      for (auto& e : a) (void)hipEventDestroy(e);
+    if (own_stream) (void)hipStreamDestroy(own_stream);
+  }
+};
+
+}  // namespace
+
+struct fecl_ldpc {
+  Code code;
+  int device = 0;
+  DevBuf d_row_ptr, d_col, d_var_ptr, d_var_edges, d_sf, d_sf32;
+  DevCode dc{};
+  // options
+  int32_t algorithm = FECL_APPROXIMATE;
+  int32_t iterations = 1;
+  int32_t msg_format = FECL_MSG_F32;
+  std::vector<double> sf;
+  int32_t sf_rows = 1, sf_cols = 1;
+  int32_t sf_mode = 0;
+  std::mutex mu;
+  std::vector<std::unique_ptr<Workspace>> pool;
+  // kernel timing (fecl_ldpc_profile): check, variable, syndrome+control
+  bool profiling = false;
+  double prof_ms[3] = {0, 0, 0};
+  int64_t prof_launches[3] = {0, 0, 0};
+  ~fecl_ldpc() {
+    pool.clear();
+    for (DevBuf* b : {&d_row_ptr, &d_col, &d_var_ptr, &d_var_edges, &d_sf, &d_sf32}) b->release();
+  }
+};
+
+namespace {
+
+// Frames decoded per batch: bounded so a workspace stays a few GiB at most.
+constexpr int64_t kMaxChunkFrames = 8192;
+
+int set_options(fecl_ldpc* h, const fecl_decoder_options* o) {
+  if (!o) return fail(FECL_INVALID_ARGUMENT, "null options");
+  if (o->algorithm < 0 || o->algorithm > 2) return fail(FECL_INVALID_ARGUMENT, "Invalid decoder algorithm");
+  if (o->iterations < 1) return fail(FECL_INVALID_ARGUMENT, "iterations must be >= 1");
+  if (o->msg_format < 0 || o->msg_format > 2) return fail(FECL_INVALID_ARGUMENT, "Invalid message format");
+  const int32_t cols = h->code.max_row_degree - 1;
+  std::vector<double> sf;
+  int32_t rows = 1;
+  if (o->scaling) {
+    if (o->scaling_cols != cols) return fail(FECL_INVALID_ARGUMENT, "Wrong size for scaling factor");
+    if (o->scaling_rows != 1 && o->scaling_rows != o->iterations)
+      return fail(FECL_INVALID_ARGUMENT, "Wrong size for scaling factor");
+    rows = o->scaling_rows;
+    sf.assign(o->scaling, o->scaling + (size_t)rows * cols);
+  } else {
+    sf.assign((size_t)cols, 1.0);
+  }
+  if (o->algorithm == FECL_APPROXIMATE && h->code.max_row_degree > 64)
+    return fail(FECL_NOT_SUPPORTED, "min-sum check rows are limited to 64 edges");
+  if (o->algorithm != FECL_APPROXIMATE && h->code.max_row_degree > 32)
+    return fail(FECL_NOT_SUPPORTED, "Exact/Linear check rows are limited to 32 edges");
+  int mode = 0;
+  std::vector<float> sf32(sf.size());
+  for (size_t i = 0; i < sf.size(); ++i) {
+    sf32[i] = (float)sf[i];
+    if (sf[i] != 1.0) mode = std::max(mode, 1);
+    if ((double)sf32[i] != sf[i]) mode = 2;
+  }
+  HIP_TRY(hipSetDevice(h->device));
+  HIP_TRY(h->d_sf.ensure(sf.size() * sizeof(double)));
+  HIP_TRY(h->d_sf32.ensure(sf.size() * sizeof(float)));
+  HIP_TRY(hipMemcpy(h->d_sf.p, sf.data(), sf.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(h->d_sf32.p, sf32.data(), sf32.size() * sizeof(float), hipMemcpyHostToDevice));
+  h->algorithm = o->algorithm;
+  h->iterations = o->iterations;
+  h->msg_format = o->msg_format;
+  h->sf = std::move(sf);
+  h->sf_rows = rows;
+  h->sf_cols = cols;
+  h->sf_mode = mode;
+  return FECL_OK;
+}
+
+DevOpts make_opts(const fecl_ldpc* h) {
+  DevOpts o{};
+  o.sf = h->d_sf.as<double>();
+  o.sf32 = h->d_sf32.as<float>();
+  o.sf_rows = h->sf_rows;
+  o.sf_cols = h->sf_cols;
+  o.sf_mode = h->sf_mode;
+  o.iterations = h->iterations;
+  o.algorithm = h->algorithm;
+  // FeCl's Linearlog1pexpm table in float: y[i] = log(1 + exp(-i/2)) evaluated in double
+  // (src/detail/LinearTable.h:40-44, 63), slopes y[i+1] - y[i] in float.
+  float y[8];
+  for (int i = 0; i < 8; ++i) y[i] = (float)std::log(1.0 + std::exp(-((double)(float)i / (double)2.0f)));
+  for (int i = 0; i < 8; ++i) {
+    o.lin_y[i] = y[i];
+    o.lin_s[i] = i < 7 ? y[i + 1] - y[i] : 0.0f;
+  }
+  return o;
+}
+
+int acquire(fecl_ldpc* h, std::unique_ptr<Workspace>* out) {
+  {
+    std::lock_guard<std::mutex> lk(h->mu);
+    if (!h->pool.empty()) {
+      *out = std::move(h->pool.back());
+      h->pool.pop_back();
+      return FECL_OK;
+    }
+  }
+  std::unique_ptr<Workspace> ws(new Workspace());
+  for (auto& e : ws->ev) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  HIP_TRY(hipEventCreateWithFlags(&ws->done, hipEventDisableTiming));
+  HIP_TRY(hipEventRecord(ws->done, nullptr));
+  *out = std::move(ws);
+  return FECL_OK;
+}
+
+void release(fecl_ldpc* h, std::unique_ptr<Workspace> ws) {
+  std::lock_guard<std::mutex> lk(h->mu);
+  h->pool.push_back(std::move(ws));
+}
+
+struct BatchIO {
+  const void* llr;        // [F][N]
+  const void* syst;       // [F][K] or null
+  const void* state;      // [F][E] or null
+  bool dbl_in;
+  BatchResults res;
+  bool dbl_out;
+};
+
+// Decodes frames [frame0, frame0 + frames) of io on stream st with workspace ws.
+int run_batch(fecl_ldpc* h, Workspace* ws, hipStream_t st, const BatchIO& io, int64_t frames, int64_t frame0) {
+  const int fmt = h->msg_format;
+  const int FPL = frames_per_lane(fmt), TW = tile_width(fmt);
+  const Code& c = h->code;
+  const int64_t tiles = (frames + TW - 1) / TW;
+  const bool soft = io.res.msg || io.res.syst || io.res.parity;
+  HIP_TRY(ws->msg.ensure((size_t)tiles * c.E * 64 * 4));
+  HIP_TRY(ws->L.ensure((size_t)tiles * c.N * TW * 4));
+  HIP_TRY(ws->hard.ensure((size_t)tiles * c.N * FPL * 8));
+  HIP_TRY(ws->phase.ensure((size_t)tiles * TW));
+  HIP_TRY(ws->iter.ensure((size_t)tiles * TW * 4));
+  HIP_TRY(ws->fail.ensure((size_t)tiles * FPL * 8));
+  HIP_TRY(ws->flags.ensure((size_t)tiles * 4));
+  HIP_TRY(ws->active.ensure((size_t)h->iterations * 4));
+  HIP_TRY(ws->out_hard.ensure((size_t)tiles * c.K * FPL * 8));
+  if (soft) {
+    HIP_TRY(ws->out_ext.ensure((size_t)tiles * c.N * TW * 4));
+    HIP_TRY(ws->out_app.ensure((size_t)tiles * c.K * TW * 4));
+  }
+  if (ws->host_active_cap < h->iterations) {
+    if (ws->host_active) (void)hipHostFree(ws->host_active);
+    ws->host_active = nullptr;
+    HIP_TRY(hipHostMalloc((void**)&ws->host_active, (size_t)h->iterations * 4, hipHostMallocDefault));
+    ws->host_active_cap = h->iterations;
+  }
+  DevState s{};
+  s.msg = ws->msg.as<uint32_t>();
+  s.L = ws->L.as<float>();
+  s.hard = ws->hard.as<uint64_t>();
+  s.phase = ws->phase.as<uint8_t>();
+  s.iter = ws->iter.as<int32_t>();
+  s.fail = ws->fail.as<uint64_t>();
+  s.tile_flags = ws->flags.as<uint32_t>();
+  s.active = ws->active.as<int32_t>();
+  s.tiles = (int32_t)tiles;
+  DevOut out{};
+  out.hard = ws->out_hard.as<uint64_t>();
+  out.ext = soft ? ws->out_ext.as<float>() : nullptr;
+  out.app = soft ? ws->out_app.as<float>() : nullptr;
+  const DevOpts o = make_opts(h);
+  const bool prof = h->profiling;
+  if (prof) {
+    while ((int)ws->pev.size() < h->iterations) {
+      std::array<hipEvent_t, 4> ev{};
+      for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
+      ws->pev.push_back(ev);
+    }
+  }
+  int steps_run = 0;
+
+  HIP_TRY(kernels_fill_i32(s.active, h->iterations, 0, st));
+  uint8_t init = (uint8_t)((h->iterations == 1 ? PH_FINAL : PH_RUN) | (io.state ? PH_VNINIT : PH_FRESH));
+  HIP_TRY(kernels_ingest(fmt, h->dc, s, io.llr, io.syst, io.dbl_in, frames, frame0, init, st));
+  if (io.state) {
+    HIP_TRY(kernels_state_in(fmt, h->dc, s, io.state, io.dbl_in, frames, frame0, st));
+    HIP_TRY(kernels_vn_init(fmt, h->dc, s, c.max_var_degree, st));
+  }
+  for (int step = 0; step < h->iterations; ++step) {
+    if (prof) HIP_TRY(hipEventRecord(ws->pev[step][0], st));
+    HIP_TRY(kernels_check(fmt, h->dc, s, o, c.max_row_degree, st));
+    if (prof) HIP_TRY(hipEventRecord(ws->pev[step][1], st));
+    HIP_TRY(kernels_variable(fmt, h->dc, s, c.max_var_degree, st));
+    if (prof) HIP_TRY(hipEventRecord(ws->pev[step][2], st));
+    HIP_TRY(kernels_syndrome_control(fmt, h->dc, s, h->iterations, step, st));
+    if (prof) HIP_TRY(hipEventRecord(ws->pev[step][3], st));
+    steps_run = step + 1;
+    HIP_TRY(hipMemcpyAsync(&ws->host_active[step], &s.active[step], 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipEventRecord(ws->ev[step % 4], st));
+    // Stop enqueuing once every frame of the batch has finished: the count of step-2 is read
+    // while the last two steps run, so the device queue never drains on the host.
+    if (step >= 2) {
+      HIP_TRY(hipEventSynchronize(ws->ev[(step - 2) % 4]));
+      if (ws->host_active[step - 2] == 0) break;
+    }
+  }
+  HIP_TRY(kernels_output(fmt, h->dc, s, out, c.max_var_degree, st));
+  HIP_TRY(kernels_results(fmt, h->dc, s, out, frames, frame0, io.res, io.dbl_out, st));
+  if (prof) {
+    // kernel durations from events on the launch stream: check, variable, syndrome+control
+    HIP_TRY(hipStreamSynchronize(st));
+    std::lock_guard<std::mutex> lk(h->mu);
+    for (int step = 0; step < steps_run; ++step) {
+      for (int k = 0; k < 3; ++k) {
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, ws->pev[step][k], ws->pev[step][k + 1]));
+        h->prof_ms[k] += ms;
+        h->prof_launches[k] += 1;
+      }
+    }
+  }
+  return FECL_OK;
+}
+
+int check_handle(const fecl_ldpc* h) {
+  if (!h) return fail(FECL_INVALID_ARGUMENT, "null handle");
+  return FECL_OK;
+}
+
+// Host-buffer decode/soDecode: stage each chunk through device memory.
+int host_decode(fecl_ldpc* h, const double* llr, const double* syst_in, const double* state_in, int64_t frames,
+                uint8_t* msg_bits, double* msg, double* syst, double* parity, double* state, int32_t* iterations) {
+  if (int rc = check_handle(h)) return rc;
+  if (frames < 0) return fail(FECL_INVALID_ARGUMENT, "Invalid size for parity");
+  if (frames == 0) return FECL_OK;
+  if (!llr) return fail(FECL_INVALID_ARGUMENT, "Input must contains parity");
+  const Code& c = h->code;
+  HIP_TRY(hipSetDevice(h->device));
+  std::unique_ptr<Workspace> ws;
+  if (int rc = acquire(h, &ws)) return rc;
+  struct Back { fecl_ldpc* h; std::unique_ptr<Workspace>* ws; ~Back() { release(h, std::move(*ws)); } } back{h, &ws};
+  if (!ws->own_stream) HIP_TRY(hipStreamCreateWithFlags(&ws->own_stream, hipStreamNonBlocking));
+  hipStream_t st = ws->own_stream;
+  const int64_t chunk = std::min<int64_t>(frames, kMaxChunkFrames);
+  // staging layout (device): inputs, then outputs
+  HIP_TRY(ws->stage_in.ensure((size_t)chunk * c.N * 8));
+  if (syst_in) HIP_TRY(ws->stage_syst.ensure((size_t)chunk * c.K * 8));
+  if (state_in) HIP_TRY(ws->stage_state.ensure((size_t)chunk * c.E * 8));
+  size_t out_bytes = 0;
+  auto take = [&](bool want, size_t n) { size_t off = out_bytes; if (want) out_bytes += (n + 255) / 256 * 256; return off; };
+  const size_t o_bits = take(msg_bits != nullptr, (size_t)chunk * c.K);
+  const size_t o_iter = take(iterations != nullptr, (size_t)chunk * 4);
+  const size_t o_msg = take(msg != nullptr, (size_t)chunk * c.K * 8);
+  const size_t o_syst = take(syst != nullptr, (size_t)chunk * c.K * 8);
+  const size_t o_par = take(parity != nullptr, (size_t)chunk * c.N * 8);
+  const size_t o_state = take(state != nullptr, (size_t)chunk * c.E * 8);
+  HIP_TRY(ws->stage_out.ensure(std::max<size_t>(out_bytes, 256)));
+  char* ob = ws->stage_out.as<char>();
+  for (int64_t f0 = 0; f0 < frames; f0 += chunk) {
+    const int64_t n = std::min(chunk, frames - f0);
+    HIP_TRY(hipMemcpyAsync(ws->stage_in.p, llr + f0 * c.N, (size_t)n * c.N * 8, hipMemcpyHostToDevice, st));
+    if (syst_in) HIP_TRY(hipMemcpyAsync(ws->stage_syst.p, syst_in + f0 * c.K, (size_t)n * c.K * 8, hipMemcpyHostToDevice, st));
+    if (state_in) HIP_TRY(hipMemcpyAsync(ws->stage_state.p, state_in + f0 * c.E, (size_t)n * c.E * 8, hipMemcpyHostToDevice, st));
+    BatchIO io{};
+    io.llr = ws->stage_in.p;
+    io.syst = syst_in ? ws->stage_syst.p : nullptr;
+    io.state = state_in ? ws->stage_state.p : nullptr;
+    io.dbl_in = true;
+    io.dbl_out = true;
+    io.res.msg_bits = msg_bits ? (uint8_t*)(ob + o_bits) : nullptr;
+    io.res.iterations = iterations ? (int32_t*)(ob + o_iter) : nullptr;
+    io.res.msg = msg ? (void*)(ob + o_msg) : nullptr;
+    io.res.syst = syst ? (void*)(ob + o_syst) : nullptr;
+    io.res.parity = parity ? (void*)(ob + o_par) : nullptr;
+    io.res.state = state ? (void*)(ob + o_state) : nullptr;
+    if (int rc = run_batch(h, ws.get(), st, io, n, 0)) return rc;
+    if (msg_bits) HIP_TRY(hipMemcpyAsync(msg_bits + f0 * c.K, ob + o_bits, (size_t)n * c.K, hipMemcpyDeviceToHost, st));
+    if (iterations) HIP_TRY(hipMemcpyAsync(iterations + f0, ob + o_iter, (size_t)n * 4, hipMemcpyDeviceToHost, st));
+    if (msg) HIP_TRY(hipMemcpyAsync(msg + f0 * c.K, ob + o_msg, (size_t)n * c.K * 8, hipMemcpyDeviceToHost, st));
+    if (syst) HIP_TRY(hipMemcpyAsync(syst + f0 * c.K, ob + o_syst, (size_t)n * c.K * 8, hipMemcpyDeviceToHost, st));
+    if (parity) HIP_TRY(hipMemcpyAsync(parity + f0 * c.N, ob + o_par, (size_t)n * c.N * 8, hipMemcpyDeviceToHost, st));
+    if (state) HIP_TRY(hipMemcpyAsync(state + f0 * c.E, ob + o_state, (size_t)n * c.E * 8, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+  }
+  return FECL_OK;
+}
+
+int device_decode(fecl_ldpc* h, const float* llr, const float* syst_in, const float* state_in, int64_t frames,
+                  uint8_t* msg_bits, float* msg, float* syst, float* parity, float* state, int32_t* iterations,
+                  void* stream) {
+  if (int rc = check_handle(h)) return rc;
+  if (frames < 0) return fail(FECL_INVALID_ARGUMENT, "Invalid size for parity");
+  if (frames == 0) return FECL_OK;
+  if (!llr) return fail(FECL_INVALID_ARGUMENT, "Input must contains parity");
+  HIP_TRY(hipSetDevice(h->device));
+  std::unique_ptr<Workspace> ws;
+  if (int rc = acquire(h, &ws)) return rc;
+  struct Back { fecl_ldpc* h; std::unique_ptr<Workspace>* ws; ~Back() { release(h, std::move(*ws)); } } back{h, &ws};
+  hipStream_t st = (hipStream_t)stream;
+  HIP_TRY(hipStreamWaitEvent(st, ws->done, 0));
+  const int64_t chunk = std::min<int64_t>(frames, kMaxChunkFrames);
+  for (int64_t f0 = 0; f0 < frames; f0 += chunk) {
+    const int64_t n = std::min(chunk, frames - f0);
+    BatchIO io{};
+    io.llr = llr;
+    io.syst = syst_in;
+    io.state = state_in;
+    io.dbl_in = false;
+    io.dbl_out = false;
+    io.res = BatchResults{msg_bits, iterations, msg, syst, parity, state};
+    if (int rc = run_batch(h, ws.get(), st, io, n, f0)) return rc;
+  }
+  HIP_TRY(hipEventRecord(ws->done, st));
+  return FECL_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* fecl_last_error(void) { return g_err.c_str(); }
+
+int fecl_dvbs2_matrix(int32_t n, int32_t rate_num, int32_t rate_den, int64_t* dims, int32_t* row_ptr, int32_t* col) {
+  if (!dims) return fail(FECL_INVALID_ARGUMENT, "null dims");
+  if (!dvbs2_matrix(n, rate_num, rate_den, dims, row_ptr, col)) {
+    if (n != 64800 && n != 16200) return fail(FECL_INVALID_ARGUMENT, "Invalid length");
+    return fail(FECL_INVALID_ARGUMENT, "Invalid rate");
+  }
+  return FECL_OK;
+}
+
+int fecl_ldpc_scaling_table(const int32_t* row_ptr, int64_t M, int64_t iterations, const int64_t* keys,
+                            const int64_t* lengths, const int64_t* offsets, const double* values, int64_t entries,
+                            double* table, int64_t table_capacity, int32_t* rows, int32_t* cols) {
+  if (!row_ptr || M < 1 || (entries > 0 && (!keys || !lengths || !offsets || !values)) || !rows || !cols)
+    return fail(FECL_INVALID_ARGUMENT, "invalid arguments");
+  Code c;
+  c.M = M;
+  c.row_ptr.assign(row_ptr, row_ptr + M + 1);
+  for (int64_t r = 0; r < M; ++r) c.max_row_degree = std::max(c.max_row_degree, row_ptr[r + 1] - row_ptr[r]);
+  if (c.max_row_degree < 2) return fail(FECL_INVALID_ARGUMENT, "every check row needs at least two edges");
+  std::unordered_map<size_t, std::vector<double>> map;
+  for (int64_t i = 0; i < entries; ++i)
+    map[(size_t)keys[i]] = std::vector<double>(values + offsets[i], values + offsets[i] + lengths[i]);
+  std::vector<double> t;
+  std::string err = scaling_map_to_table(map, c, iterations, &t, rows, cols);
+  if (!err.empty()) return fail(FECL_INVALID_ARGUMENT, err);
+  if (table) {
+    if ((int64_t)t.size() > table_capacity) return fail(FECL_INVALID_ARGUMENT, "scaling table buffer too small");
+    std::copy(t.begin(), t.end(), table);
+  }
+  return FECL_OK;
+}
+
+int fecl_ldpc_create(const int32_t* row_ptr, const int32_t* col, int64_t M, int64_t N,
+                     const fecl_decoder_options* options, fecl_ldpc** out) {
+  if (!out) return fail(FECL_INVALID_ARGUMENT, "null output handle");
+  *out = nullptr;
+  std::unique_ptr<fecl_ldpc> h(new fecl_ldpc());
+  std::string err = build_code(row_ptr, col, M, N, &h->code);
+  if (!err.empty()) return fail(FECL_INVALID_ARGUMENT, err);
+  const Code& c = h->code;
+  if (c.max_var_degree > 32) return fail(FECL_NOT_SUPPORTED, "variable degree above 32 is not supported");
+  HIP_TRY(hipGetDevice(&h->device));
+  HIP_TRY(h->d_row_ptr.ensure(c.row_ptr.size() * 4));
+  HIP_TRY(h->d_col.ensure(c.col.size() * 4));
+  HIP_TRY(h->d_var_ptr.ensure(c.var_ptr.size() * 4));
+  HIP_TRY(h->d_var_edges.ensure(c.var_edges.size() * 4));
+  HIP_TRY(hipMemcpy(h->d_row_ptr.p, c.row_ptr.data(), c.row_ptr.size() * 4, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(h->d_col.p, c.col.data(), c.col.size() * 4, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(h->d_var_ptr.p, c.var_ptr.data(), c.var_ptr.size() * 4, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(h->d_var_edges.p, c.var_edges.data(), c.var_edges.size() * 4, hipMemcpyHostToDevice));
+  h->dc.row_ptr = h->d_row_ptr.as<int32_t>();
+  h->dc.col = h->d_col.as<int32_t>();
+  h->dc.var_ptr = h->d_var_ptr.as<int32_t>();
+  h->dc.var_edges = h->d_var_edges.as<int32_t>();
+  h->dc.M = (int32_t)c.M;
+  h->dc.N = (int32_t)c.N;
+  h->dc.K = (int32_t)c.K;
+  h->dc.E = (int32_t)c.E;
+  fecl_decoder_options def{FECL_APPROXIMATE, 1, nullptr, 1, c.max_row_degree - 1, FECL_MSG_F32};
+  if (int rc = set_options(h.get(), options ? options : &def)) return rc;
+  *out = h.release();
+  return FECL_OK;
+}
+
+int fecl_ldpc_destroy(fecl_ldpc* h) {
+  if (h) {
+    (void)hipSetDevice(h->device);
+    delete h;
+  }
+  return FECL_OK;
+}
+
+int fecl_ldpc_set_options(fecl_ldpc* h, const fecl_decoder_options* options) {
+  if (int rc = check_handle(h)) return rc;
+  return set_options(h, options);
+}
+
+int fecl_ldpc_get_options(const fecl_ldpc* h, fecl_decoder_options* options, double* scaling, int64_t capacity) {
+  if (int rc = check_handle(h)) return rc;
+  if (!options) return fail(FECL_INVALID_ARGUMENT, "null options");
+  options->algorithm = h->algorithm;
+  options->iterations = h->iterations;
+  options->scaling_rows = h->sf_rows;
+  options->scaling_cols = h->sf_cols;
+  options->msg_format = h->msg_format;
+  options->scaling = nullptr;
+  if (scaling) {
+    if ((int64_t)h->sf.size() > capacity) return fail(FECL_INVALID_ARGUMENT, "scaling buffer too small");
+    std::copy(h->sf.begin(), h->sf.end(), scaling);
+    options->scaling = scaling;
+  }
+  return FECL_OK;
+}
+
+int fecl_ldpc_sizes(const fecl_ldpc* h, int64_t* msg, int64_t* syst, int64_t* parity, int64_t* state) {
+  if (int rc = check_handle(h)) return rc;
+  if (msg) *msg = h->code.K;
+  if (syst) *syst = h->code.K;
+  if (parity) *parity = h->code.N;
+  if (state) *state = h->code.E;
+  return FECL_OK;
+}
+
+int fecl_ldpc_encode(const fecl_ldpc* h, const uint8_t* msg, int64_t frames, uint8_t* parity) {
+  if (int rc = check_handle(h)) return rc;
+  if (frames < 0 || (frames > 0 && (!msg || !parity))) return fail(FECL_INVALID_ARGUMENT, "Invalid size for message");
+  if (!h->code.staircase) return fail(FECL_NOT_SUPPORTED, "encoder supports staircase (DVB-S2 type) codes only");
+  for (int64_t f = 0; f < frames; ++f) encode_staircase(h->code, msg + f * h->code.K, parity + f * h->code.N);
+  return FECL_OK;
+}
+
+int fecl_ldpc_check(const fecl_ldpc* h, const uint8_t* parity, int64_t frames, int32_t* ok) {
+  if (int rc = check_handle(h)) return rc;
+  if (!ok || frames < 0 || (frames > 0 && !parity)) return fail(FECL_INVALID_ARGUMENT, "Invalid size for parity");
+  const Code& c = h->code;
+  *ok = 1;
+  for (int64_t f = 0; f < frames && *ok; ++f) {
+    const uint8_t* x = parity + f * c.N;
+    for (int64_t r = 0; r < c.M; ++r) {
+      uint8_t s = 0;
+      for (int32_t e = c.row_ptr[r]; e < c.row_ptr[r + 1]; ++e) s ^= x[c.col[e]] & 1;
+      if (s) { *ok = 0; break; }
+    }
+  }
+  return FECL_OK;
+}
+
+int fecl_ldpc_decode(fecl_ldpc* h, const double* llr, int64_t frames, uint8_t* msg, int32_t* iterations) {
+  return host_decode(h, llr, nullptr, nullptr, frames, msg, nullptr, nullptr, nullptr, nullptr, iterations);
+}
+
+int fecl_ldpc_sodecode(fecl_ldpc* h, const double* parity_in, const double* syst_in, const double* state_in,
+                       int64_t frames, double* msg_out, double* syst_out, double* parity_out, double* state_out,
+                       int32_t* iterations) {
+  return host_decode(h, parity_in, syst_in, state_in, frames, nullptr, msg_out, syst_out, parity_out, state_out,
+                     iterations);
+}
+
+int fecl_ldpc_decode_device(fecl_ldpc* h, const float* llr, int64_t frames, uint8_t* msg, int32_t* iterations,
+                            void* stream) {
+  return device_decode(h, llr, nullptr, nullptr, frames, msg, nullptr, nullptr, nullptr, nullptr, iterations, stream);
+}
+
+int fecl_ldpc_sodecode_device(fecl_ldpc* h, const float* parity_in, const float* syst_in, const float* state_in,
+                              int64_t frames, float* msg_out, float* syst_out, float* parity_out, float* state_out,
+                              int32_t* iterations, void* stream) {
+  return device_decode(h, parity_in, syst_in, state_in, frames, nullptr, msg_out, syst_out, parity_out, state_out,
+                       iterations, stream);
+}
+
+int fecl_ldpc_profile(fecl_ldpc* h, int32_t enable) {
+  if (int rc = check_handle(h)) return rc;
+  std::lock_guard<std::mutex> lk(h->mu);
+  h->profiling = enable != 0;
+  for (int k = 0; k < 3; ++k) {
+    h->prof_ms[k] = 0;
+    h->prof_launches[k] = 0;
+  }
+  return FECL_OK;
+}
+
+int fecl_ldpc_profile_read(fecl_ldpc* h, double* ms, int64_t* launches) {
+  if (int rc = check_handle(h)) return rc;
+  if (!ms || !launches) return fail(FECL_INVALID_ARGUMENT, "null output");
+  std::lock_guard<std::mutex> lk(h->mu);
+  for (int k = 0; k < 3; ++k) {
+    ms[k] = h->prof_ms[k];
+    launches[k] = h->prof_launches[k];
+  }
+  return FECL_OK;
+}
+
+int fecl_ldpc_trim(fecl_ldpc* h) {
+  if (int rc = check_handle(h)) return rc;
+  std::lock_guard<std::mutex> lk(h->mu);
+  h->pool.clear();
+  return FECL_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,70 @@
+// Device-side building blocks of the flooding BP decoder (gfx950, wave64).
+//
+// Layout in HBM.  Frames are processed in tiles of TW = 64*FPL frames (FPL = frames per lane,
+// 1 for FP32 messages, 2 for FP16/BF16) so that every lane moves one 32-bit word per edge and
+// every wavefront access is 256 contiguous bytes, whatever the Tanner graph looks like:
+//   msg   [tile][edge][64] 32-bit words   edge messages, in place (v2c before a check update,
+//                                         c2v after it), edges in H's row-major order
+//   L     [tile][var][TW]  float          channel LLR (+ a-priori)
+//   hard  [tile][var][FPL] uint64         hard decisions; bit `lane` of word k is frame
+//                                         lane*FPL + k of the tile
+//   phase / iter [tile*TW]                per-frame decoder state
+// Graph indices (row_ptr, col, var_ptr, var_edges) are shared by all frames of a tile and are
+// read with scalar loads.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fecl {
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;   // 4 wavefronts per workgroup
+constexpr int kWavesPerBlock = kBlock / kWave;
+
+// Per-frame phase (low two bits) and flags.
+enum : uint8_t {
+  PH_FREE = 0, PH_RUN = 1, PH_FINAL = 2, PH_DONE = 3, PH_MASK = 3,
+  PH_FRESH = 4,    // v2c not yet materialised: the next check update reads L[col(e)]
+  PH_VNINIT = 8,   // state supplied: a variable-node pass must build v2c from it first
+};
+enum : uint32_t { TF_CN = 1, TF_VN = 2, TF_VNINIT = 4 };
+
+struct DevCode {
+  const int32_t* row_ptr;
+  const int32_t* col;
+  const int32_t* var_ptr;
+  const int32_t* var_edges;
+  int32_t M, N, K, E;
+};
+
+struct DevState {
+  uint32_t* msg;
+  float* L;
+  uint64_t* hard;
+  uint8_t* phase;
+  int32_t* iter;
+  uint64_t* fail;          // [tile][FPL]
+  uint32_t* tile_flags;    // [tile]
+  int32_t* active;         // [iterations] frames still iterating after each step
+  int32_t tiles;
+};
+
+struct DevOpts {
+  const double* sf;        // [sf_rows][sf_cols]
+  const float* sf32;       // same table rounded to float
+  int32_t sf_rows, sf_cols;
+  int32_t sf_mode;         // 0: all factors 1.0; 1: all exactly representable in float; 2: general
+  int32_t iterations;
+  int32_t algorithm;
+  float lin_y[8];          // Linear table knots y[i] = log(1 + exp(-i/2)) (float)
+  float lin_s[8];          // slopes y[i+1] - y[i] (float)
+};
+
+// Outputs of a finished batch, in tile layout (device buffers owned by the engine).
+struct DevOut {
+  uint64_t* hard;          // [tile][K][FPL]  decode hard decisions (L + sum ascending >= 0)
+  float* ext;              // [tile][N][TW]   extrinsic = sum of c2v ascending (nullable)
+  float* app;              // [tile][K][TW]   L + ext (nullable)
+};
+
+}  // namespace fecl

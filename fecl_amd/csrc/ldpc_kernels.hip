@@ -1,0 +1,835 @@
+// HIP kernels of the flooding BP decoder for gfx950 (wave64).  Compiled with
+// -ffp-contract=off: every float operation rounds exactly as FeCl's CPU decoder instantiated
+// with a float LLR type does, which is what makes the FP32 min-sum and Linear paths
+// bit-identical to it.
+//
+// Reference arithmetic (FeCl):
+//   check update     src/detail/BpDecoder/BpDecoderImpl.cpp:133-159
+//   bit update       src/detail/BpDecoder/BpDecoderImpl.cpp:162-179
+//   schedule/output  src/detail/BpDecoder/BpDecoderImpl.cpp:38-130
+//   MinBoxSum / LinearBoxSum / BoxSum   src/detail/LlrMetrics.h:126-196
+//   linear table     src/detail/LinearTable.h:57-81
+//   syndrome test    src/detail/Ldpc.cpp:160-172
+#include "ldpc_kernels.h"
+#include "ldpc_launch.h"
+
+namespace fecl {
+
+// ------------------------------------------------------------------------------------------
+// Message formats.  A lane holds FPL frames packed in one 32-bit word.
+struct FmtF32 {
+  static constexpr int FPL = 1;
+  __device__ static inline void unpack(uint32_t w, float (&x)[1]) { x[0] = __uint_as_float(w); }
+  __device__ static inline uint32_t pack(const float (&x)[1]) { return __float_as_uint(x[0]); }
+  __device__ static inline float quantize(float x) { return x; }
+};
+
+__device__ static inline uint32_t f32_to_f16_bits(float x) {
+  // saturate to +-65504, then round to nearest even (v_cvt_f16_f32 in the default mode)
+  if (x > 65504.0f) x = 65504.0f;
+  if (x < -65504.0f) x = -65504.0f;
+  _Float16 h = (_Float16)x;
+  return (uint32_t)__builtin_bit_cast(uint16_t, h);
+}
+__device__ static inline float f16_bits_to_f32(uint32_t b) {
+  return (float)__builtin_bit_cast(_Float16, (uint16_t)b);
+}
+struct FmtF16 {
+  static constexpr int FPL = 2;
+  __device__ static inline void unpack(uint32_t w, float (&x)[2]) {
+    x[0] = f16_bits_to_f32(w & 0xffffu);
+    x[1] = f16_bits_to_f32(w >> 16);
+  }
+  __device__ static inline uint32_t pack(const float (&x)[2]) {
+    return f32_to_f16_bits(x[0]) | (f32_to_f16_bits(x[1]) << 16);
+  }
+  __device__ static inline float quantize(float x) { return f16_bits_to_f32(f32_to_f16_bits(x)); }
+};
+
+__device__ static inline uint32_t f32_to_bf16_bits(float x) {
+  // saturate to +-0x1.fep127, then round to nearest even
+  const float bmax = 3.38953138925153547590e38f;
+  if (x > bmax) x = bmax;
+  if (x < -bmax) x = -bmax;
+  uint32_t u = __float_as_uint(x);
+  if (x != x) return 0x7fc0u;
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return u >> 16;
+}
+struct FmtBF16 {
+  static constexpr int FPL = 2;
+  __device__ static inline void unpack(uint32_t w, float (&x)[2]) {
+    x[0] = __uint_as_float(w << 16);
+    x[1] = __uint_as_float(w & 0xffff0000u);
+  }
+  __device__ static inline uint32_t pack(const float (&x)[2]) {
+    return f32_to_bf16_bits(x[0]) | (f32_to_bf16_bits(x[1]) << 16);
+  }
+  __device__ static inline float quantize(float x) { return __uint_as_float(f32_to_bf16_bits(x) << 16); }
+};
+
+__device__ static inline int uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
+
+__device__ static inline uint64_t wave_or(uint64_t x) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) x |= __shfl_xor(x, o, 64);
+  return x;
+}
+
+// Scaling factor of check update t for a lane (FeCl: sf * post(...) with a double sf,
+// rounded to the message type).  mode 0: 1.0; 1: float product equals the rounded double one.
+__device__ static inline float apply_sf(const DevOpts& o, int row, int dcol, float v) {
+  if (o.sf_mode == 0) return v;
+  if (o.sf_mode == 1) return o.sf32[row * o.sf_cols + dcol] * v;
+  return (float)(o.sf[row * o.sf_cols + dcol] * (double)v);
+}
+
+// ------------------------------------------------------------------------------------------
+// Check-node update, min-sum (DecoderAlgorithm::Approximate).  MinBoxSum is sign * min and is
+// associative, so FeCl's forward/backward pass equals "min of the others, xor of the others'
+// signs" exactly: output j has magnitude min_{i != j} |x_i| and sign bit
+// ((d-2) & 1) ^ xor_{i != j} signbit(x_i)  (a box-sum of two operands sets the sign bit
+// 1 ^ s_a ^ s_b).  One wavefront per (check row, tile); two passes, no per-edge registers.
+template <class Fmt>
+__global__ __launch_bounds__(kBlock) void cn_minsum_kernel(DevCode c, DevState s, DevOpts o) {
+  constexpr int FPL = Fmt::FPL, TW = kWave * FPL;
+  const int t = blockIdx.y;
+  if (!(s.tile_flags[t] & TF_CN)) return;
+  const int r = uniform(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6));
+  if (r >= c.M) return;
+  const int lane = threadIdx.x & 63;
+
+  bool act[FPL], fresh[FPL];
+  int sfrow[FPL];
+  bool lane_any = false, lane_all = true, any_fresh = false;
+#pragma unroll
+  for (int k = 0; k < FPL; ++k) {
+    const int64_t p = (int64_t)t * TW + lane * FPL + k;
+    const uint8_t ph = s.phase[p];
+    const int base = ph & PH_MASK;
+    act[k] = base == PH_RUN || base == PH_FINAL;
+    fresh[k] = act[k] && (ph & PH_FRESH);
+    const int tt = base == PH_FINAL ? o.iterations - 1 : s.iter[p];
+    sfrow[k] = tt % o.sf_rows;
+    lane_any |= act[k];
+    lane_all &= act[k];
+    any_fresh |= fresh[k];
+  }
+  if (__ballot(lane_any) == 0) return;
+  const bool wave_fresh = __ballot(any_fresh) != 0;
+
+  const int e0 = uniform(c.row_ptr[r]);
+  const int d = uniform(c.row_ptr[r + 1]) - e0;
+  uint32_t* mrow = s.msg + ((int64_t)t * c.E + e0) * kWave + lane;
+  const float* Lt = s.L + (int64_t)t * c.N * TW + lane * FPL;
+
+  float m1[FPL], m2[FPL];
+  int i1[FPL];
+  uint64_t sg[FPL];
+#pragma unroll
+  for (int k = 0; k < FPL; ++k) { m1[k] = __builtin_inff(); m2[k] = __builtin_inff(); i1[k] = 0; sg[k] = 0; }
+
+  for (int j = 0; j < d; ++j) {
+    float x[FPL];
+    if (lane_any) Fmt::unpack(mrow[(int64_t)j * kWave], x);
+    else {
+#pragma unroll
+      for (int k = 0; k < FPL; ++k) x[k] = 0.0f;
+    }
+    if (wave_fresh) {
+      const int v = uniform(c.col[e0 + j]);
+#pragma unroll
+      for (int k = 0; k < FPL; ++k)
+        if (fresh[k]) x[k] = Fmt::quantize(Lt[(int64_t)v * TW + k]);
+    }
+#pragma unroll
+    for (int k = 0; k < FPL; ++k) {
+      const float a = fabsf(x[k]);
+      sg[k] |= (uint64_t)(__float_as_uint(x[k]) >> 31) << j;
+      if (a < m1[k]) { m2[k] = m1[k]; m1[k] = a; i1[k] = j; }
+      else if (a < m2[k]) { m2[k] = a; }
+    }
+  }
+  float m1s[FPL], m2s[FPL];
+  uint32_t par[FPL];
+#pragma unroll
+  for (int k = 0; k < FPL; ++k) {
+    par[k] = (uint32_t)(((d - 2) & 1) ^ (__popcll(sg[k]) & 1));
+    m1s[k] = apply_sf(o, sfrow[k], d - 2, m1[k]);
+    m2s[k] = apply_sf(o, sfrow[k], d - 2, m2[k]);
+  }
+  if (!lane_any) return;
+  for (int j = 0; j < d; ++j) {
+    float y[FPL];
+#pragma unroll
+    for (int k = 0; k < FPL; ++k) {
+      const float mag = (j == i1[k]) ? m2s[k] : m1s[k];
+      const uint32_t sb = par[k] ^ (uint32_t)((sg[k] >> j) & 1);
+      y[k] = sb ? -mag : mag;
+    }
+    uint32_t* dst = mrow + (int64_t)j * kWave;
+    if (lane_all) {
+      *dst = Fmt::pack(y);
+    } else {
+      float old[FPL];
+      Fmt::unpack(*dst, old);
+#pragma unroll
+      for (int k = 0; k < FPL; ++k) if (!act[k]) y[k] = old[k];
+      *dst = Fmt::pack(y);
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// Pairwise box-sums for the forward/backward check update (FeCl sign convention: the result
+// has magnitude ~min(|a|,|b|) and sign -sign(a)*sign(b)).
+//   Linear: LinearBoxSum, min - f(|a+b|) + f(|a-b|) with FeCl's 8-knot table f.
+//   Exact : the same form with f(z) = log(1 + e^-z), i.e. the exact box-plus
+//           -2 atanh(tanh(-a/2) tanh(-b/2)) in a form that stays finite in FP32.
+__device__ static inline float lin_f(const DevOpts& o, float z) {
+  float x = z * 2.0f;
+  if (x >= 7.0f) return 0.0f;
+  const int i = (int)x;
+  float y = o.lin_y[0], sl = o.lin_s[0];
+#pragma unroll
+  for (int q = 1; q < 7; ++q) { if (i >= q) { y = o.lin_y[q]; sl = o.lin_s[q]; } }
+  const float dx = x - (float)i;
+  const float prod = sl * dx;
+  return prod + y;
+}
+__device__ static inline float exact_f(float z) { return __logf(1.0f + __expf(-z)); }
+
+template <int ALG>
+__device__ static inline float box_sum(const DevOpts& o, float a, float b) {
+  const float fa = fabsf(a), fb = fabsf(b);
+  const float m = (fb < fa) ? fb : fa;
+  const float s1 = fabsf(a + b), s2 = fabsf(a - b);
+  float f1, f2;
+  if (ALG == 1) { f1 = lin_f(o, s1); f2 = lin_f(o, s2); }
+  else { f1 = exact_f(s1); f2 = exact_f(s2); }
+  const bool diff = (__float_as_uint(a) ^ __float_as_uint(b)) >> 31;
+  const float t = (diff ? m : -m) - f1;
+  return t + f2;
+}
+
+// Check-node update for Linear (ALG 1) and Exact (ALG 2 here) in FeCl's forward/backward
+// order: prefix P_j = ((x_0 # x_1) # ...) # x_{j-1}; suffix from x_{d-1} downwards;
+// out_{d-1} = P_{d-1}, out_j = P_j # S_j, out_0 = S_0, each times the scaling factor.
+template <class Fmt, int DMAX, int ALG>
+__global__ __launch_bounds__(kBlock) void cn_box_kernel(DevCode c, DevState s, DevOpts o) {
+  constexpr int FPL = Fmt::FPL, TW = kWave * FPL;
+  const int t = blockIdx.y;
+  if (!(s.tile_flags[t] & TF_CN)) return;
+  const int r = uniform(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6));
+  if (r >= c.M) return;
+  const int lane = threadIdx.x & 63;
+
+  bool act[FPL], fresh[FPL];
+  int sfrow[FPL];
+  bool lane_any = false, lane_all = true, any_fresh = false;
+#pragma unroll
+  for (int k = 0; k < FPL; ++k) {
+    const int64_t p = (int64_t)t * TW + lane * FPL + k;
+    const uint8_t ph = s.phase[p];
+    const int base = ph & PH_MASK;
+    act[k] = base == PH_RUN || base == PH_FINAL;
+    fresh[k] = act[k] && (ph & PH_FRESH);
+    const int tt = base == PH_FINAL ? o.iterations - 1 : s.iter[p];
+    sfrow[k] = tt % o.sf_rows;
+    lane_any |= act[k];
+    lane_all &= act[k];
+    any_fresh |= fresh[k];
+  }
+  if (__ballot(lane_any) == 0) return;
+  const bool wave_fresh = __ballot(any_fresh) != 0;
+
+  const int e0 = uniform(c.row_ptr[r]);
+  const int d = uniform(c.row_ptr[r + 1]) - e0;
+  uint32_t* mrow = s.msg + ((int64_t)t * c.E + e0) * kWave + lane;
+  const float* Lt = s.L + (int64_t)t * c.N * TW + lane * FPL;
+
+  float x[DMAX][FPL];
+  uint32_t wold[DMAX];
+#pragma unroll
+  for (int j = 0; j < DMAX; ++j) {
+    if (j < d) {
+      wold[j] = lane_any ? mrow[(int64_t)j * kWave] : 0u;
+      Fmt::unpack(wold[j], x[j]);
+      if (wave_fresh) {
+        const int v = uniform(c.col[e0 + j]);
+#pragma unroll
+        for (int k = 0; k < FPL; ++k)
+          if (fresh[k]) x[j][k] = Fmt::quantize(Lt[(int64_t)v * TW + k]);
+      }
+    }
+  }
+  if (!lane_any) return;
+  // forward prefixes: pre[j] for j = 1 .. d-2, P = prefix over 0 .. d-2
+  float pre[DMAX][FPL];
+  float P[FPL];
+#pragma unroll
+  for (int k = 0; k < FPL; ++k) P[k] = x[0][k];
+#pragma unroll
+  for (int j = 1; j < DMAX - 1; ++j) {
+    if (j < d - 1) {
+#pragma unroll
+      for (int k = 0; k < FPL; ++k) { pre[j][k] = P[k]; P[k] = box_sum<ALG>(o, P[k], x[j][k]); }
+    }
+  }
+  auto store = [&](int j, float (&y)[FPL]) {
+#pragma unroll
+    for (int k = 0; k < FPL; ++k) {
+      y[k] = apply_sf(o, sfrow[k], d - 2, y[k]);
+      y[k] = Fmt::quantize(y[k]);
+    }
+    if (!lane_all) {
+      float old[FPL];
+      Fmt::unpack(wold[j], old);
+#pragma unroll
+      for (int k = 0; k < FPL; ++k) if (!act[k]) y[k] = old[k];
+    }
+    mrow[(int64_t)j * kWave] = Fmt::pack(y);
+  };
+  // out_{d-1} = P
+  {
+    float y[FPL];
+#pragma unroll
+    for (int k = 0; k < FPL; ++k) y[k] = P[k];
+#pragma unroll
+    for (int j = DMAX - 1; j >= 1; --j) if (j == d - 1) store(j, y);
+  }
+  float S[FPL];
+#pragma unroll
+  for (int j = DMAX - 1; j >= 1; --j) {
+    if (j == d - 1) {
+#pragma unroll
+      for (int k = 0; k < FPL; ++k) S[k] = x[j][k];
+    }
+  }
+#pragma unroll
+  for (int j = DMAX - 2; j >= 1; --j) {
+    if (j <= d - 2) {
+      float y[FPL];
+#pragma unroll
+      for (int k = 0; k < FPL; ++k) {
+        y[k] = box_sum<ALG>(o, pre[j][k], S[k]);
+        S[k] = box_sum<ALG>(o, S[k], x[j][k]);
+      }
+      store(j, y);
+    }
+  }
+  {
+    float y[FPL];
+#pragma unroll
+    for (int k = 0; k < FPL; ++k) y[k] = S[k];
+    store(0, y);
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// Variable-node update in FeCl's order (BpDecoderImpl.cpp:162-179):
+//   v2c_j = (0 + c_1 + ... + c_{j-1}) + (L + c_dv + ... + c_{j+1}),  APP = L + c_dv + ... + c_1
+// with edges of the variable in ascending edge order, then hard = APP >= 0 per frame.
+// One wavefront per (variable, tile).  init_pass builds v2c from a supplied state
+// (BpDecoderImpl.cpp:81-88) for frames flagged PH_VNINIT.
+template <class Fmt, int VMAX>
+__global__ __launch_bounds__(kBlock) void vn_kernel(DevCode c, DevState s, int init_pass) {
+  constexpr int FPL = Fmt::FPL, TW = kWave * FPL;
+  const int t = blockIdx.y;
+  if (!(s.tile_flags[t] & (init_pass ? TF_VNINIT : TF_VN))) return;
+  const int v = uniform(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6));
+  if (v >= c.N) return;
+  const int lane = threadIdx.x & 63;
+  bool run[FPL];
+  bool lane_any = false;
+#pragma unroll
+  for (int k = 0; k < FPL; ++k) {
+    const uint8_t ph = s.phase[(int64_t)t * TW + lane * FPL + k];
+    run[k] = init_pass ? ((ph & PH_VNINIT) != 0) : ((ph & PH_MASK) == PH_RUN);
+    lane_any |= run[k];
+  }
+  if (__ballot(lane_any) == 0) return;
+  const int s0 = uniform(c.var_ptr[v]);
+  const int dv = uniform(c.var_ptr[v + 1]) - s0;
+  float Lv[FPL];
+  const float* Lp = s.L + ((int64_t)t * c.N + v) * TW + lane * FPL;
+#pragma unroll
+  for (int k = 0; k < FPL; ++k) Lv[k] = Lp[k];
+  uint32_t* mt = s.msg + (int64_t)t * c.E * kWave + lane;
+  float cv[VMAX][FPL];
+  uint32_t wold[VMAX];
+  int eid[VMAX];
+#pragma unroll
+  for (int j = 0; j < VMAX; ++j) {
+    if (j < dv) {
+      eid[j] = uniform(c.var_edges[s0 + j]);
+      wold[j] = mt[(int64_t)eid[j] * kWave];
+      Fmt::unpack(wold[j], cv[j]);
+    }
+  }
+  float pre[VMAX][FPL];
+  float acc[FPL];
+#pragma unroll
+  for (int k = 0; k < FPL; ++k) acc[k] = 0.0f;
+#pragma unroll
+  for (int j = 0; j < VMAX; ++j) {
+    if (j < dv) {
+#pragma unroll
+      for (int k = 0; k < FPL; ++k) { pre[j][k] = acc[k]; acc[k] = acc[k] + cv[j][k]; }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < FPL; ++k) acc[k] = Lv[k];
+#pragma unroll
+  for (int j = VMAX - 1; j >= 0; --j) {
+    if (j < dv) {
+      float y[FPL];
+#pragma unroll
+      for (int k = 0; k < FPL; ++k) {
+        y[k] = pre[j][k] + acc[k];
+        acc[k] = acc[k] + cv[j][k];
+        if (!run[k]) y[k] = cv[j][k];
+      }
+      if (lane_any) mt[(int64_t)eid[j] * kWave] = Fmt::pack(y);
+    }
+  }
+  if (init_pass) return;
+#pragma unroll
+  for (int k = 0; k < FPL; ++k) {
+    const uint64_t m = __ballot(acc[k] >= 0.0f);
+    if (lane == 0) s.hard[((int64_t)t * c.N + v) * FPL + k] = m;
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// Syndrome: one thread per (check row, tile) xors the hard words of its columns; the failing
+// frames of the tile are or-ed into fail[tile].
+template <int FPL>
+__global__ __launch_bounds__(kBlock) void syndrome_kernel(DevCode c, DevState s) {
+  const int t = blockIdx.y;
+  if (!(s.tile_flags[t] & TF_VN)) return;
+  const int r = blockIdx.x * kBlock + threadIdx.x;
+  uint64_t f[FPL];
+#pragma unroll
+  for (int k = 0; k < FPL; ++k) f[k] = 0;
+  if (r < c.M) {
+    const uint64_t* ht = s.hard + (int64_t)t * c.N * FPL;
+    for (int e = c.row_ptr[r]; e < c.row_ptr[r + 1]; ++e) {
+      const int v = c.col[e];
+#pragma unroll
+      for (int k = 0; k < FPL; ++k) f[k] ^= ht[(int64_t)v * FPL + k];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < FPL; ++k) {
+    const uint64_t w = wave_or(f[k]);
+    if ((threadIdx.x & 63) == 0 && w) atomicOr((unsigned long long*)&s.fail[t * FPL + k], (unsigned long long)w);
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// Per-frame state machine after a step (one wavefront per tile).  FeCl's loop
+//   for t in 0 .. I-2: { CN(t); VN; if syndrome ok: break }  CN(I-1)
+// becomes: RUN frames count the update; they turn FINAL when the syndrome passes or the
+// I-1 regular updates are used up; FINAL frames (their last update used factor row I-1)
+// turn DONE.  The iteration count is the number of check-node updates executed.
+template <int FPL>
+__global__ __launch_bounds__(kWave) void control_kernel(DevState s, int iterations, int step) {
+  constexpr int TW = kWave * FPL;
+  const int t = blockIdx.x;
+  const int lane = threadIdx.x;
+  bool any_cn = false, any_vn = false;
+  int alive = 0;
+#pragma unroll
+  for (int k = 0; k < FPL; ++k) {
+    const uint64_t fw = s.fail[t * FPL + k];
+    const int64_t p = (int64_t)t * TW + lane * FPL + k;
+    uint8_t ph = s.phase[p];
+    const int base = ph & PH_MASK;
+    if (base == PH_RUN) {
+      const int it = s.iter[p] + 1;
+      s.iter[p] = it;
+      const bool ok = ((fw >> lane) & 1ull) == 0;
+      ph = (ok || it >= iterations - 1) ? PH_FINAL : PH_RUN;
+    } else if (base == PH_FINAL) {
+      s.iter[p] = s.iter[p] + 1;
+      ph = PH_DONE;
+    } else {
+      ph = base;
+    }
+    s.phase[p] = ph;
+    any_cn |= ph == PH_RUN || ph == PH_FINAL;
+    any_vn |= ph == PH_RUN;
+    alive += (ph == PH_RUN || ph == PH_FINAL) ? 1 : 0;
+  }
+  const uint64_t bcn = __ballot(any_cn), bvn = __ballot(any_vn);
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) alive += __shfl_xor(alive, o, 64);
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < FPL; ++k) s.fail[t * FPL + k] = 0;
+    s.tile_flags[t] = (bcn ? TF_CN : 0u) | (bvn ? TF_VN : 0u);
+    if (alive) atomicAdd(&s.active[step], alive);
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// Batch set-up: LLR [F][N] (row stride ld) -> L tiles, optional a-priori on the first K bits
+// (FeCl: parity_[i] += syst[i] in the LLR type), per-frame phase/iteration.
+template <typename In, int FPL>
+__global__ __launch_bounds__(kBlock) void ingest_kernel(DevCode c, DevState s, const In* llr,
+                                                        const In* syst, int64_t frames,
+                                                        int64_t frame0, uint8_t init_phase) {
+  constexpr int TW = kWave * FPL;
+  __shared__ float tile[64][65];
+  const int t = blockIdx.y;
+  const int v0 = blockIdx.x * 64;
+  // 64 variables x TW frames, in chunks of 64 frames
+  for (int fc = 0; fc < TW; fc += 64) {
+    for (int i = threadIdx.x; i < 64 * 64; i += kBlock) {
+      const int fl = i / 64, vl = i % 64;                 // read: frame-major, coalesced in v
+      const int64_t f = (int64_t)t * TW + fc + fl;
+      const int v = v0 + vl;
+      float x = 0.0f;
+      if (f < frames && v < c.N) {
+        x = (float)llr[(frame0 + f) * c.N + v];
+        if (syst && v < c.K) x = (float)((double)x + (double)syst[(frame0 + f) * c.K + v]);
+      }
+      tile[fl][vl] = x;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < 64 * 64; i += kBlock) {
+      const int vl = i / 64, fl = i % 64;                 // write: variable-major, coalesced in f
+      const int v = v0 + vl;
+      if (v < c.N) s.L[((int64_t)t * c.N + v) * TW + fc + fl] = tile[fl][vl];
+    }
+    __syncthreads();
+  }
+  if (blockIdx.x == 0) {
+    for (int i = threadIdx.x; i < TW; i += kBlock) {
+      const int64_t f = (int64_t)t * TW + i;
+      s.phase[f] = f < frames ? init_phase : (uint8_t)PH_FREE;
+      s.iter[f] = 0;
+    }
+    if (threadIdx.x < FPL) s.fail[t * FPL + threadIdx.x] = 0;
+    if (threadIdx.x == 0) {
+      const int64_t live = frames - (int64_t)t * TW;
+      uint32_t fl = 0;
+      if (live > 0) {
+        const uint8_t base = init_phase & PH_MASK;
+        fl = (base == PH_RUN || base == PH_FINAL ? TF_CN : 0u) | (base == PH_RUN ? TF_VN : 0u) |
+             ((init_phase & PH_VNINIT) ? TF_VNINIT : 0u);
+      }
+      s.tile_flags[t] = fl;
+    }
+  }
+}
+
+// State [F][E] (check-to-variable, FeCl's edge order) -> msg tiles, rounded to the format.
+template <typename In, class Fmt>
+__global__ __launch_bounds__(kBlock) void state_in_kernel(DevCode c, DevState s, const In* state,
+                                                          int64_t frames, int64_t frame0) {
+  constexpr int FPL = Fmt::FPL, TW = kWave * FPL;
+  __shared__ float tile[64][65];
+  const int t = blockIdx.y;
+  const int e0 = blockIdx.x * 64;
+  for (int fc = 0; fc < TW; fc += 64) {
+    for (int i = threadIdx.x; i < 64 * 64; i += kBlock) {
+      const int fl = i / 64, el = i % 64;
+      const int64_t f = (int64_t)t * TW + fc + fl;
+      const int e = e0 + el;
+      tile[fl][el] = (f < frames && e < c.E) ? (float)state[(frame0 + f) * c.E + e] : 0.0f;
+    }
+    __syncthreads();
+    // each thread packs one word: edge el, lane (fc + fl) / FPL
+    for (int i = threadIdx.x; i < 64 * (64 / FPL); i += kBlock) {
+      const int el = i / (64 / FPL), ll = i % (64 / FPL);
+      const int e = e0 + el;
+      float x[FPL];
+#pragma unroll
+      for (int k = 0; k < FPL; ++k) x[k] = tile[ll * FPL + k][el];
+      if (e < c.E) s.msg[((int64_t)t * c.E + e) * kWave + fc / FPL + ll] = Fmt::pack(x);
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// Outputs of finished frames (BpDecoderImpl.cpp:61-69 and :109-129): with c the final c2v,
+//   decode   : app = L + c_1 + ... + c_dv (ascending), hard = app >= 0 for v < K
+//   soDecode : ext = 0 + c_1 + ... + c_dv, app_msg = L + ext for v < K
+template <class Fmt>
+__global__ __launch_bounds__(kBlock) void output_kernel(DevCode c, DevState s, DevOut out) {
+  constexpr int FPL = Fmt::FPL, TW = kWave * FPL;
+  const int t = blockIdx.y;
+  const int v = uniform(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6));
+  if (v >= c.N) return;
+  const int lane = threadIdx.x & 63;
+  const int s0 = uniform(c.var_ptr[v]);
+  const int dv = uniform(c.var_ptr[v + 1]) - s0;
+  const float* Lp = s.L + ((int64_t)t * c.N + v) * TW + lane * FPL;
+  const uint32_t* mt = s.msg + (int64_t)t * c.E * kWave + lane;
+  float app[FPL], ext[FPL];
+#pragma unroll
+  for (int k = 0; k < FPL; ++k) { app[k] = Lp[k]; ext[k] = 0.0f; }
+  for (int j = 0; j < dv; ++j) {
+    const int e = uniform(c.var_edges[s0 + j]);
+    float cv[FPL];
+    Fmt::unpack(mt[(int64_t)e * kWave], cv);
+#pragma unroll
+    for (int k = 0; k < FPL; ++k) { app[k] = app[k] + cv[k]; ext[k] = ext[k] + cv[k]; }
+  }
+  if (v < c.K && out.hard) {
+#pragma unroll
+    for (int k = 0; k < FPL; ++k) {
+      const uint64_t m = __ballot(app[k] >= 0.0f);
+      if (lane == 0) out.hard[((int64_t)t * c.K + v) * FPL + k] = m;
+    }
+  }
+  if (out.ext) {
+#pragma unroll
+    for (int k = 0; k < FPL; ++k) out.ext[((int64_t)t * c.N + v) * TW + lane * FPL + k] = ext[k];
+  }
+  if (out.app && v < c.K) {
+#pragma unroll
+    for (int k = 0; k < FPL; ++k)
+      out.app[((int64_t)t * c.K + v) * TW + lane * FPL + k] = Lp[k] + ext[k];
+  }
+}
+
+// Hard decisions [tile][K][FPL] -> msg [F][K] bytes (0/1).
+template <int FPL>
+__global__ __launch_bounds__(kBlock) void unpack_hard_kernel(const uint64_t* hard, int K, int64_t frames,
+                                                             uint8_t* msg, int64_t frame0) {
+  constexpr int TW = kWave * FPL;
+  const int64_t f = blockIdx.y;
+  const int v = blockIdx.x * kBlock + threadIdx.x;
+  if (v >= K || f >= frames) return;
+  const int64_t t = f / TW;
+  const int p = (int)(f % TW);
+  const uint64_t w = hard[(t * K + v) * FPL + (p % FPL)];
+  msg[(frame0 + f) * K + v] = (uint8_t)((w >> (p / FPL)) & 1ull);
+}
+
+// [tile][X][TW] float -> [F][X] (Out = float or double), through LDS.
+// Copies x < count of each frame; the output row stride is `count`.
+template <typename Out, int FPL>
+__global__ __launch_bounds__(kBlock) void tiles_to_frames_kernel(const float* in, int64_t X, int64_t count,
+                                                                 int64_t frames, Out* out, int64_t frame0) {
+  constexpr int TW = kWave * FPL;
+  __shared__ float tile[64][65];
+  const int t = blockIdx.y;
+  const int64_t x0 = (int64_t)blockIdx.x * 64;
+  for (int fc = 0; fc < TW; fc += 64) {
+    for (int i = threadIdx.x; i < 64 * 64; i += kBlock) {
+      const int xl = i / 64, fl = i % 64;
+      const int64_t x = x0 + xl;
+      tile[xl][fl] = x < X ? in[((int64_t)t * X + x) * TW + fc + fl] : 0.0f;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < 64 * 64; i += kBlock) {
+      const int fl = i / 64, xl = i % 64;
+      const int64_t f = (int64_t)t * TW + fc + fl;
+      const int64_t x = x0 + xl;
+      if (f < frames && x < count) out[(frame0 + f) * count + x] = (Out)tile[xl][fl];
+    }
+    __syncthreads();
+  }
+}
+
+// msg tiles (formatted words) -> state [F][E] (Out = float or double).
+template <typename Out, class Fmt>
+__global__ __launch_bounds__(kBlock) void state_out_kernel(DevCode c, DevState s, int64_t frames,
+                                                           Out* out, int64_t frame0) {
+  constexpr int FPL = Fmt::FPL, TW = kWave * FPL;
+  __shared__ float tile[64][65];
+  const int t = blockIdx.y;
+  const int e0 = blockIdx.x * 64;
+  for (int fc = 0; fc < TW; fc += 64) {
+    for (int i = threadIdx.x; i < 64 * (64 / FPL); i += kBlock) {
+      const int el = i / (64 / FPL), ll = i % (64 / FPL);
+      const int e = e0 + el;
+      float x[FPL];
+      if (e < c.E) Fmt::unpack(s.msg[((int64_t)t * c.E + e) * kWave + fc / FPL + ll], x);
+      else {
+#pragma unroll
+        for (int k = 0; k < FPL; ++k) x[k] = 0.0f;
+      }
+#pragma unroll
+      for (int k = 0; k < FPL; ++k) tile[ll * FPL + k][el] = x[k];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < 64 * 64; i += kBlock) {
+      const int fl = i / 64, el = i % 64;
+      const int64_t f = (int64_t)t * TW + fc + fl;
+      const int e = e0 + el;
+      if (f < frames && e < c.E) out[(frame0 + f) * c.E + e] = (Out)tile[fl][el];
+    }
+    __syncthreads();
+  }
+}
+
+// Per-frame iteration counts -> iterations [F].
+__global__ void iters_out_kernel(const int32_t* iter, int64_t frames, int32_t* out, int64_t frame0) {
+  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f < frames) out[frame0 + f] = iter[f];
+}
+
+__global__ void fill_i32_kernel(int32_t* p, int64_t n, int32_t v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// ------------------------------------------------------------------------------------------
+// Launchers (host side).
+
+template <class Fmt>
+static hipError_t launch_vn(const DevCode& c, const DevState& s, int max_dv, int init, hipStream_t st) {
+  dim3 grid((c.N + kWavesPerBlock - 1) / kWavesPerBlock, s.tiles);
+  if (max_dv <= 4) hipLaunchKernelGGL((vn_kernel<Fmt, 4>), grid, dim3(kBlock), 0, st, c, s, init);
+  else if (max_dv <= 8) hipLaunchKernelGGL((vn_kernel<Fmt, 8>), grid, dim3(kBlock), 0, st, c, s, init);
+  else if (max_dv <= 16) hipLaunchKernelGGL((vn_kernel<Fmt, 16>), grid, dim3(kBlock), 0, st, c, s, init);
+  else if (max_dv <= 32) hipLaunchKernelGGL((vn_kernel<Fmt, 32>), grid, dim3(kBlock), 0, st, c, s, init);
+  else return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+template <class Fmt, int ALG>
+static hipError_t launch_cn_box(const DevCode& c, const DevState& s, const DevOpts& o, int max_dc, hipStream_t st) {
+  dim3 grid((c.M + kWavesPerBlock - 1) / kWavesPerBlock, s.tiles);
+  if (max_dc <= 8) hipLaunchKernelGGL((cn_box_kernel<Fmt, 8, ALG>), grid, dim3(kBlock), 0, st, c, s, o);
+  else if (max_dc <= 16) hipLaunchKernelGGL((cn_box_kernel<Fmt, 16, ALG>), grid, dim3(kBlock), 0, st, c, s, o);
+  else if (max_dc <= 32) hipLaunchKernelGGL((cn_box_kernel<Fmt, 32, ALG>), grid, dim3(kBlock), 0, st, c, s, o);
+  else return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+template <class Fmt>
+static hipError_t launch_cn(const DevCode& c, const DevState& s, const DevOpts& o, int max_dc, hipStream_t st) {
+  if (o.algorithm == 2) {
+    if (max_dc > 64) return hipErrorInvalidValue;
+    dim3 grid((c.M + kWavesPerBlock - 1) / kWavesPerBlock, s.tiles);
+    hipLaunchKernelGGL((cn_minsum_kernel<Fmt>), grid, dim3(kBlock), 0, st, c, s, o);
+    return hipGetLastError();
+  }
+  if (o.algorithm == 1) return launch_cn_box<Fmt, 1>(c, s, o, max_dc, st);
+  return launch_cn_box<Fmt, 2>(c, s, o, max_dc, st);
+}
+
+template <int FPL>
+static hipError_t launch_syn_ctrl(const DevCode& c, const DevState& s, int iterations, int step, hipStream_t st) {
+  hipLaunchKernelGGL((syndrome_kernel<FPL>), dim3((c.M + kBlock - 1) / kBlock, s.tiles), dim3(kBlock), 0, st, c, s);
+  hipLaunchKernelGGL((control_kernel<FPL>), dim3(s.tiles), dim3(kWave), 0, st, s, iterations, step);
+  return hipGetLastError();
+}
+
+template <class Fmt>
+static hipError_t launch_output(const DevCode& c, const DevState& s, const DevOut& out, int max_dv, hipStream_t st) {
+  dim3 grid((c.N + kWavesPerBlock - 1) / kWavesPerBlock, s.tiles);
+  (void)max_dv;
+  hipLaunchKernelGGL((output_kernel<Fmt>), grid, dim3(kBlock), 0, st, c, s, out);
+  return hipGetLastError();
+}
+
+template <class Fmt>
+static hipError_t launch_ingest(const DevCode& c, const DevState& s, const void* llr, const void* syst,
+                                bool dbl, int64_t frames, int64_t frame0, uint8_t init_phase, hipStream_t st) {
+  constexpr int FPL = Fmt::FPL;
+  dim3 grid((c.N + 63) / 64, s.tiles);
+  if (dbl)
+    hipLaunchKernelGGL((ingest_kernel<double, FPL>), grid, dim3(kBlock), 0, st, c, s, (const double*)llr,
+                       (const double*)syst, frames, frame0, init_phase);
+  else
+    hipLaunchKernelGGL((ingest_kernel<float, FPL>), grid, dim3(kBlock), 0, st, c, s, (const float*)llr,
+                       (const float*)syst, frames, frame0, init_phase);
+  return hipGetLastError();
+}
+
+template <class Fmt>
+static hipError_t launch_state_in(const DevCode& c, const DevState& s, const void* state, bool dbl,
+                                  int64_t frames, int64_t frame0, hipStream_t st) {
+  dim3 grid((c.E + 63) / 64, s.tiles);
+  if (dbl)
+    hipLaunchKernelGGL((state_in_kernel<double, Fmt>), grid, dim3(kBlock), 0, st, c, s, (const double*)state, frames, frame0);
+  else
+    hipLaunchKernelGGL((state_in_kernel<float, Fmt>), grid, dim3(kBlock), 0, st, c, s, (const float*)state, frames, frame0);
+  return hipGetLastError();
+}
+
+template <class Fmt>
+static hipError_t launch_results(const DevCode& c, const DevState& s, const DevOut& out, int64_t frames,
+                                 int64_t frame0, const BatchResults& r, bool dbl, hipStream_t st) {
+  constexpr int FPL = Fmt::FPL;
+  if (r.msg_bits)
+    hipLaunchKernelGGL((unpack_hard_kernel<FPL>), dim3((c.K + kBlock - 1) / kBlock, (unsigned)frames), dim3(kBlock), 0, st,
+                       out.hard, c.K, frames, r.msg_bits, frame0);
+  if (r.iterations)
+    hipLaunchKernelGGL(iters_out_kernel, dim3((unsigned)((frames + 255) / 256)), dim3(256), 0, st, s.iter, frames,
+                       r.iterations, frame0);
+  auto to_frames = [&](const float* in, int64_t X, int64_t count, void* dst) {
+    dim3 grid((unsigned)((count + 63) / 64), s.tiles);
+    if (dbl) hipLaunchKernelGGL((tiles_to_frames_kernel<double, FPL>), grid, dim3(kBlock), 0, st, in, X, count,
+                                frames, (double*)dst, frame0);
+    else hipLaunchKernelGGL((tiles_to_frames_kernel<float, FPL>), grid, dim3(kBlock), 0, st, in, X, count,
+                            frames, (float*)dst, frame0);
+  };
+  if (r.msg) to_frames(out.app, c.K, c.K, r.msg);
+  if (r.parity) to_frames(out.ext, c.N, c.N, r.parity);
+  if (r.syst) to_frames(out.ext, c.N, c.K, r.syst);
+  if (r.state) {
+    if (dbl) hipLaunchKernelGGL((state_out_kernel<double, Fmt>), dim3((c.E + 63) / 64, s.tiles), dim3(kBlock), 0, st,
+                                c, s, frames, (double*)r.state, frame0);
+    else hipLaunchKernelGGL((state_out_kernel<float, Fmt>), dim3((c.E + 63) / 64, s.tiles), dim3(kBlock), 0, st,
+                            c, s, frames, (float*)r.state, frame0);
+  }
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------------------
+// Format dispatch.
+hipError_t kernels_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(fill_i32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, v);
+  return hipGetLastError();
+}
+
+#define FECL_DISPATCH(fmt, CALL)                      \
+  switch (fmt) {                                      \
+    case 0: { using F = FmtF32; return CALL; }        \
+    case 1: { using F = FmtF16; return CALL; }        \
+    case 2: { using F = FmtBF16; return CALL; }       \
+    default: return hipErrorInvalidValue;             \
+  }
+
+hipError_t kernels_ingest(int fmt, const DevCode& c, const DevState& s, const void* llr, const void* syst, bool dbl,
+                          int64_t frames, int64_t frame0, uint8_t init_phase, hipStream_t st) {
+  FECL_DISPATCH(fmt, launch_ingest<F>(c, s, llr, syst, dbl, frames, frame0, init_phase, st));
+}
+hipError_t kernels_state_in(int fmt, const DevCode& c, const DevState& s, const void* state, bool dbl,
+                            int64_t frames, int64_t frame0, hipStream_t st) {
+  FECL_DISPATCH(fmt, launch_state_in<F>(c, s, state, dbl, frames, frame0, st));
+}
+hipError_t kernels_vn_init(int fmt, const DevCode& c, const DevState& s, int max_dv, hipStream_t st) {
+  FECL_DISPATCH(fmt, launch_vn<F>(c, s, max_dv, 1, st));
+}
+hipError_t kernels_check(int fmt, const DevCode& c, const DevState& s, const DevOpts& o, int max_dc, hipStream_t st) {
+  FECL_DISPATCH(fmt, launch_cn<F>(c, s, o, max_dc, st));
+}
+hipError_t kernels_variable(int fmt, const DevCode& c, const DevState& s, int max_dv, hipStream_t st) {
+  FECL_DISPATCH(fmt, launch_vn<F>(c, s, max_dv, 0, st));
+}
+hipError_t kernels_syndrome_control(int fmt, const DevCode& c, const DevState& s, int iterations, int step,
+                                    hipStream_t st) {
+  if (fmt == 0) return launch_syn_ctrl<1>(c, s, iterations, step, st);
+  if (fmt == 1 || fmt == 2) return launch_syn_ctrl<2>(c, s, iterations, step, st);
+  return hipErrorInvalidValue;
+}
+hipError_t kernels_output(int fmt, const DevCode& c, const DevState& s, const DevOut& out, int max_dv, hipStream_t st) {
+  FECL_DISPATCH(fmt, launch_output<F>(c, s, out, max_dv, st));
+}
+hipError_t kernels_results(int fmt, const DevCode& c, const DevState& s, const DevOut& out, int64_t frames,
+                           int64_t frame0, const BatchResults& r, bool dbl, hipStream_t st) {
+  FECL_DISPATCH(fmt, launch_results<F>(c, s, out, frames, frame0, r, dbl, st));
+}
+
+}  // namespace fecl

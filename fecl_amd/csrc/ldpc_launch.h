@@ -1,0 +1,39 @@
+// Host entry points into the kernels (ldpc_kernels.hip).  fmt = fecl_msg_format.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "ldpc_kernels.h"
+
+namespace fecl {
+
+// Where the results of a batch go (device pointers, frame-major; null = not wanted).
+// msg/syst/parity/state are double when `dbl`, float otherwise.
+struct BatchResults {
+  uint8_t* msg_bits;     // [F][K] 0/1
+  int32_t* iterations;   // [F]
+  void* msg;             // [F][K]  a-posteriori (soDecode)
+  void* syst;            // [F][K]  extrinsic
+  void* parity;          // [F][N]  extrinsic
+  void* state;           // [F][E]  check-to-variable
+};
+
+inline int frames_per_lane(int fmt) { return fmt == 0 ? 1 : 2; }
+inline int tile_width(int fmt) { return 64 * frames_per_lane(fmt); }
+
+hipError_t kernels_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t st);
+hipError_t kernels_ingest(int fmt, const DevCode& c, const DevState& s, const void* llr, const void* syst,
+                          bool dbl, int64_t frames, int64_t frame0, uint8_t init_phase, hipStream_t st);
+hipError_t kernels_state_in(int fmt, const DevCode& c, const DevState& s, const void* state, bool dbl,
+                            int64_t frames, int64_t frame0, hipStream_t st);
+hipError_t kernels_vn_init(int fmt, const DevCode& c, const DevState& s, int max_dv, hipStream_t st);
+hipError_t kernels_check(int fmt, const DevCode& c, const DevState& s, const DevOpts& o, int max_dc,
+                         hipStream_t st);
+hipError_t kernels_variable(int fmt, const DevCode& c, const DevState& s, int max_dv, hipStream_t st);
+hipError_t kernels_syndrome_control(int fmt, const DevCode& c, const DevState& s, int iterations, int step,
+                                    hipStream_t st);
+hipError_t kernels_output(int fmt, const DevCode& c, const DevState& s, const DevOut& out, int max_dv,
+                          hipStream_t st);
+hipError_t kernels_results(int fmt, const DevCode& c, const DevState& s, const DevOut& out, int64_t frames,
+                           int64_t frame0, const BatchResults& r, bool dbl, hipStream_t st);
+
+}  // namespace fecl

@@ -1,0 +1,343 @@
+"""Python mirror of FeCl's LDPC API over the HIP decoder's C ABI (include/fecl_ldpc.h).
+
+Names, argument meaning and errors follow the reference:
+    fec::DecoderAlgorithm            src/DecoderAlgorithm.h:30-34
+    fec::Ldpc::DecoderOptions        src/detail/Ldpc.h:56-74   (builder methods)
+    fec::Ldpc::DvbS2::matrix         src/Ldpc.h:66-75, src/DvbS2.cpp:1595-1645
+    fec::Ldpc / Codec decode, soDecode, encode, check, sizes
+                                     src/Ldpc.h:59-108, src/Codec.h:60-82, 210-307
+Errors raise ValueError where FeCl throws std::invalid_argument.
+
+Host arrays (numpy float64, FeCl's type) go through the host entry points; torch CUDA float32
+tensors stay on the device and run on the current stream.
+"""
+import ctypes
+import enum
+from fractions import Fraction
+
+import numpy as np
+
+from ._lib import lib, check
+
+
+class DecoderAlgorithm(enum.IntEnum):
+    Exact = 0
+    Linear = 1
+    Approximate = 2
+
+
+Exact = DecoderAlgorithm.Exact
+Linear = DecoderAlgorithm.Linear
+Approximate = DecoderAlgorithm.Approximate
+
+
+class MsgFormat(enum.IntEnum):
+    """Device storage of edge messages (arithmetic is FP32 in all formats)."""
+    F32 = 0
+    F16 = 1
+    BF16 = 2
+
+
+class CheckMatrix:
+    """Parity-check matrix in CSR form (rows are checks, row-major edge order), the role of
+    FeCl's SparseBitMatrix (src/BitMatrix.h:42-625)."""
+
+    def __init__(self, row_ptr, col, cols):
+        self.row_ptr = np.ascontiguousarray(row_ptr, dtype=np.int32)
+        self.col = np.ascontiguousarray(col, dtype=np.int32)
+        self._cols = int(cols)
+
+    def rows(self):
+        return len(self.row_ptr) - 1
+
+    def cols(self):
+        return self._cols
+
+    def size(self):
+        return int(self.row_ptr[-1])
+
+    def rowSizes(self):
+        return np.diff(self.row_ptr)
+
+    def __eq__(self, other):
+        return (isinstance(other, CheckMatrix) and self._cols == other._cols
+                and np.array_equal(self.row_ptr, other.row_ptr) and np.array_equal(self.col, other.col))
+
+
+_DVBS2_RATES = {64800: [(1, 4), (1, 3), (2, 5), (1, 2), (3, 5), (2, 3), (3, 4), (4, 5), (5, 6), (8, 9), (9, 10)],
+                16200: [(1, 4), (1, 3), (2, 5), (1, 2), (3, 5), (2, 3), (3, 4), (4, 5), (5, 6), (8, 9)]}
+
+
+class DvbS2:
+    @staticmethod
+    def matrix(n, rate):
+        """DVB-S2 parity-check matrix for frame length n (64800 or 16200) and nominal rate
+        (a float such as 0.5 as in FeCl, or a (num, den) pair / Fraction)."""
+        if n not in _DVBS2_RATES:
+            raise ValueError("Invalid length")
+        if isinstance(rate, tuple):
+            num, den = rate
+        elif isinstance(rate, Fraction):
+            num, den = rate.numerator, rate.denominator
+        else:
+            match = [(a, b) for (a, b) in _DVBS2_RATES[n] if float(rate) == a / b]
+            if not match:
+                raise ValueError("Invalid rate")
+            num, den = match[0]
+        dims = np.zeros(4, np.int64)
+        check(lib().fecl_dvbs2_matrix(n, num, den, _p(dims), None, None))
+        M, N, K, E = (int(x) for x in dims)
+        rp = np.zeros(M + 1, np.int32)
+        col = np.zeros(E, np.int32)
+        check(lib().fecl_dvbs2_matrix(n, num, den, _p(dims), _p(rp), _p(col)))
+        return CheckMatrix(rp, col, N)
+
+
+class DecoderOptions:
+    """Ldpc::DecoderOptions builder.  Defaults as FeCl's C++ API: Approximate, scaling 1.0;
+    iterations has no default and must be set before use."""
+
+    def __init__(self):
+        self._algorithm = DecoderAlgorithm.Approximate
+        self._iterations = None
+        self._scaling = {0: [1.0]}
+        self._msg_format = MsgFormat.F32
+
+    def algorithm(self, a=None):
+        if a is None:
+            return self._algorithm
+        self._algorithm = DecoderAlgorithm(a)
+        return self
+
+    def iterations(self, n=None):
+        if n is None:
+            return self._iterations
+        self._iterations = int(n)
+        return self
+
+    def scalingFactor(self, factor=None):
+        if factor is None:
+            return {k: list(v) for k, v in self._scaling.items()}
+        if isinstance(factor, dict):
+            self._scaling = {int(k): [float(x) for x in np.atleast_1d(v)] for k, v in factor.items()}
+        else:
+            self._scaling = {0: [float(factor)]}
+        return self
+
+    def msgFormat(self, fmt=None):
+        if fmt is None:
+            return self._msg_format
+        self._msg_format = MsgFormat(fmt)
+        return self
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def scaling_table(H, options):
+    """Structure::scalingMapToVector: {degree: factors} -> [rows][max degree - 1]."""
+    keys = np.array(sorted(options._scaling), np.int64)
+    vals = [np.asarray(options._scaling[k], np.float64) for k in keys]
+    lengths = np.array([len(v) for v in vals], np.int64)
+    offsets = np.concatenate([[0], np.cumsum(lengths)[:-1]]).astype(np.int64)
+    values = np.concatenate(vals) if vals else np.zeros(0)
+    cap = max(1, int(options._iterations or 1)) * int(np.max(np.diff(H.row_ptr)))
+    table = np.zeros(cap, np.float64)
+    rows, cols = ctypes.c_int32(), ctypes.c_int32()
+    check(lib().fecl_ldpc_scaling_table(_p(H.row_ptr), H.rows(), int(options._iterations or 0), _p(keys),
+                                        _p(lengths), _p(offsets), _p(values), len(keys), _p(table), cap,
+                                        ctypes.byref(rows), ctypes.byref(cols)))
+    return table[:rows.value * cols.value].reshape(rows.value, cols.value).copy()
+
+
+class _COptions(ctypes.Structure):
+    _fields_ = [("algorithm", ctypes.c_int32), ("iterations", ctypes.c_int32),
+                ("scaling", ctypes.c_void_p), ("scaling_rows", ctypes.c_int32),
+                ("scaling_cols", ctypes.c_int32), ("msg_format", ctypes.c_int32)]
+
+
+class Ldpc:
+    """fec::Ldpc over the HIP decoder.  workGroupSize is accepted for signature parity with
+    FeCl (src/Ldpc.h:82-84); frames run as one device batch regardless of it."""
+
+    def __init__(self, checkMatrix, decoderOptions=None, workGroupSize=8):
+        self._H = checkMatrix
+        self._wgs = int(workGroupSize)
+        self._h = ctypes.c_void_p()
+        self._options = None
+        opts, keep = self._c_options(decoderOptions or DecoderOptions().iterations(1))
+        check(lib().fecl_ldpc_create(_p(checkMatrix.row_ptr), _p(checkMatrix.col), checkMatrix.rows(),
+                                     checkMatrix.cols(), ctypes.byref(opts), ctypes.byref(self._h)))
+        self._options = decoderOptions
+        sizes = [ctypes.c_int64() for _ in range(4)]
+        check(lib().fecl_ldpc_sizes(self._h, *[ctypes.byref(s) for s in sizes]))
+        self._msg, self._syst, self._parity, self._state = (s.value for s in sizes)
+
+    def _c_options(self, o):
+        if o._iterations is None or o._iterations < 1:
+            raise ValueError("iterations must be >= 1")
+        table = scaling_table(self._H, o)
+        opts = _COptions(int(o._algorithm), int(o._iterations), table.ctypes.data, table.shape[0],
+                         table.shape[1], int(o._msg_format))
+        return opts, table
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                lib().fecl_ldpc_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    # sizes (src/Codec.h:60-63)
+    def msgSize(self):
+        return self._msg
+
+    def systSize(self):
+        return self._syst
+
+    def paritySize(self):
+        return self._parity
+
+    def stateSize(self):
+        return self._state
+
+    def getWorkGroupSize(self):
+        return self._wgs
+
+    def setWorkGroupSize(self, n):
+        self._wgs = int(n)
+
+    def checkMatrix(self):
+        return self._H
+
+    def setDecoderOptions(self, decoderOptions):
+        opts, keep = self._c_options(decoderOptions)
+        check(lib().fecl_ldpc_set_options(self._h, ctypes.byref(opts)))
+        self._options = decoderOptions
+
+    def getDecoderOptions(self):
+        o = _COptions()
+        cap = self._H.rows() and int(np.max(np.diff(self._H.row_ptr))) * 4096
+        buf = np.zeros(max(cap, 1), np.float64)
+        check(lib().fecl_ldpc_get_options(self._h, ctypes.byref(o), _p(buf), len(buf)))
+        table = buf[:o.scaling_rows * o.scaling_cols].reshape(o.scaling_rows, o.scaling_cols)
+        scaling = {d + 2: [float(x) for x in table[:, d]] for d in range(o.scaling_cols)}
+        return (DecoderOptions().algorithm(o.algorithm).iterations(o.iterations)
+                .scalingFactor(scaling).msgFormat(o.msg_format))
+
+    # ---- encode / check (host)
+    def encode(self, msg):
+        msg = np.ascontiguousarray(msg, dtype=np.uint8).ravel()
+        frames = msg.size // self._msg
+        if msg.size != frames * self._msg:
+            raise ValueError("Invalid size for message")
+        parity = np.zeros(frames * self._parity, np.uint8)
+        check(lib().fecl_ldpc_encode(self._h, _p(msg), frames, _p(parity)))
+        return parity
+
+    def check(self, parity):
+        parity = np.ascontiguousarray(parity, dtype=np.uint8).ravel()
+        frames = parity.size // self._parity
+        if parity.size != frames * self._parity:
+            raise ValueError("Invalid size for parity")
+        ok = ctypes.c_int32()
+        check(lib().fecl_ldpc_check(self._h, _p(parity), frames, ctypes.byref(ok)))
+        return bool(ok.value)
+
+    # ---- decode
+    def _frames(self, n, what="parity"):
+        frames = n // self._parity
+        if n != frames * self._parity:
+            raise ValueError("Invalid size for " + what)
+        return frames
+
+    def decode(self, parity, iterations=False, stream=None):
+        """Hard-decision decode.  numpy input: float64 LLRs (FeCl), returns uint8 [F*K].
+        torch CUDA input: float32 LLRs on the device, returns a CUDA uint8 tensor [F*K].
+        With iterations=True also returns the per-frame check-update counts."""
+        if _is_cuda_tensor(parity):
+            import torch
+            x = parity.contiguous().view(-1)
+            if x.dtype != torch.float32:
+                raise ValueError("device LLRs must be float32")
+            frames = self._frames(x.numel())
+            out = torch.empty(frames * self._msg, dtype=torch.uint8, device=x.device)
+            it = torch.empty(frames, dtype=torch.int32, device=x.device)
+            st = stream if stream is not None else torch.cuda.current_stream(x.device).cuda_stream
+            check(lib().fecl_ldpc_decode_device(self._h, ctypes.c_void_p(x.data_ptr()), frames,
+                                                ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(it.data_ptr()),
+                                                ctypes.c_void_p(st)))
+            return (out, it) if iterations else out
+        llr = np.ascontiguousarray(parity, dtype=np.float64).ravel()
+        frames = self._frames(llr.size)
+        out = np.zeros(frames * self._msg, np.uint8)
+        it = np.zeros(frames, np.int32)
+        check(lib().fecl_ldpc_decode(self._h, _p(llr), frames, _p(out), _p(it)))
+        return (out, it) if iterations else out
+
+    def soDecode(self, parity, syst=None, state=None,
+                 outputs=("msg", "syst", "parity", "state", "iterations"), stream=None):
+        """Soft-output decode (Codec::soDecode).  Returns a dict of the requested outputs:
+        msg [F*K] a-posteriori, syst [F*K] / parity [F*N] extrinsic, state [F*E]."""
+        if parity is None:
+            raise ValueError("Input must contains parity")
+        if _is_cuda_tensor(parity):
+            import torch
+            x = parity.contiguous().view(-1)
+            frames = self._frames(x.numel())
+            dev = x.device
+
+            def opt_in(a, n, what):
+                if a is None:
+                    return None, None
+                a = a.contiguous().view(-1).to(torch.float32)
+                if a.numel() != frames * n:
+                    raise ValueError("Invalid size for " + what)
+                return a, ctypes.c_void_p(a.data_ptr())
+            s_in, s_ptr = opt_in(syst, self._syst, "msg")
+            st_in, st_ptr = opt_in(state, self._state, "state")
+            res, ptrs = {}, {}
+            for name, n, dt in (("msg", self._msg, torch.float32), ("syst", self._syst, torch.float32),
+                                ("parity", self._parity, torch.float32), ("state", self._state, torch.float32),
+                                ("iterations", 1, torch.int32)):
+                if name in outputs:
+                    res[name] = torch.empty(frames * n, dtype=dt, device=dev)
+                    ptrs[name] = ctypes.c_void_p(res[name].data_ptr())
+                else:
+                    ptrs[name] = None
+            st = stream if stream is not None else torch.cuda.current_stream(dev).cuda_stream
+            check(lib().fecl_ldpc_sodecode_device(self._h, ctypes.c_void_p(x.data_ptr()), s_ptr, st_ptr, frames,
+                                                  ptrs["msg"], ptrs["syst"], ptrs["parity"], ptrs["state"],
+                                                  ptrs["iterations"], ctypes.c_void_p(st)))
+            return res
+        llr = np.ascontiguousarray(parity, dtype=np.float64).ravel()
+        frames = self._frames(llr.size)
+        if syst is not None:
+            syst = np.ascontiguousarray(syst, dtype=np.float64).ravel()
+            if syst.size != frames * self._syst:
+                raise ValueError("Invalid size for msg")
+        if state is not None:
+            state = np.ascontiguousarray(state, dtype=np.float64).ravel()
+            if state.size != frames * self._state:
+                raise ValueError("Invalid size for state")
+        res = {}
+        for name, n, dt in (("msg", self._msg, np.float64), ("syst", self._syst, np.float64),
+                            ("parity", self._parity, np.float64), ("state", self._state, np.float64),
+                            ("iterations", 1, np.int32)):
+            if name in outputs:
+                res[name] = np.zeros(frames * n, dt)
+        check(lib().fecl_ldpc_sodecode(self._h, _p(llr), _p(syst), _p(state), frames, _p(res.get("msg")),
+                                       _p(res.get("syst")), _p(res.get("parity")), _p(res.get("state")),
+                                       _p(res.get("iterations"))))
+        return res
+
+
+def _is_cuda_tensor(x):
+    try:
+        import torch
+    except ImportError:
+        return False
+    return isinstance(x, torch.Tensor) and x.is_cuda

@@ -1,0 +1,135 @@
+/* fecl_ldpc.h — C ABI of the MI355X LDPC belief-propagation decoder.
+ *
+ * Drop-in boundary for FeCl's LDPC decode path: the functions below are what
+ * fec::Ldpc::decodeBlocks / soDecodeBlocks (src/Ldpc.h:102-103, src/Ldpc.cpp:49-59) and the
+ * public surface around them (src/Ldpc.h:59-108, src/Codec.h:60-82, src/detail/Ldpc.h:56-74)
+ * bind to.  Plain pointers and sizes only; no exceptions cross this boundary: every call
+ * returns a fecl_status and fecl_last_error() holds the message (thread-local).
+ *
+ * Conventions (FeCl's):
+ *   - LLR L = ln p(x=1|y) / p(x=0|y) (src/Codec.h:204-205); hard decision bit = (L >= 0).
+ *   - Frames are contiguous: LLR [frames][N], hard output [frames][K] (one byte per bit,
+ *     0 or 1, in place of BitField<size_t>), state [frames][E] in the parity-check matrix's
+ *     row-major edge order.
+ *   - A code is H in CSR form (rows = checks); K = N - M (src/detail/Ldpc.cpp:46-55).
+ */
+#ifndef FECL_LDPC_H
+#define FECL_LDPC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  FECL_OK = 0,
+  FECL_INVALID_ARGUMENT = 1, /* FeCl throws std::invalid_argument here */
+  FECL_OUT_OF_MEMORY = 2,
+  FECL_DEVICE_ERROR = 3,
+  FECL_NOT_SUPPORTED = 4
+} fecl_status;
+
+/* fec::DecoderAlgorithm (src/DecoderAlgorithm.h:30-34). */
+typedef enum { FECL_EXACT = 0, FECL_LINEAR = 1, FECL_APPROXIMATE = 2 } fecl_algorithm;
+
+/* Storage format of the edge messages on the device.  Check-node and variable-node
+ * arithmetic is FP32 in every format; 16-bit formats saturate at the format's largest
+ * finite value and round to nearest even. */
+typedef enum { FECL_MSG_F32 = 0, FECL_MSG_F16 = 1, FECL_MSG_BF16 = 2 } fecl_msg_format;
+
+/* Ldpc::DecoderOptions (src/detail/Ldpc.h:56-74), flattened.  `scaling` is the table FeCl
+ * builds from its {degree -> factors} map (Structure::scalingMapToVector,
+ * src/detail/Ldpc.cpp:64-112): [scaling_rows][scaling_cols], scaling_cols = max row
+ * degree - 1, factor for check update t on a row of degree d = scaling[(t % rows)*cols + d-2].
+ * NULL means 1.0 everywhere.  Build one from a map with fecl_ldpc_scaling_table(). */
+typedef struct {
+  int32_t algorithm;      /* fecl_algorithm; FeCl's default is FECL_APPROXIMATE */
+  int32_t iterations;     /* >= 1 */
+  const double* scaling;  /* may be NULL */
+  int32_t scaling_rows;   /* 1 or iterations */
+  int32_t scaling_cols;   /* max row degree - 1 */
+  int32_t msg_format;     /* fecl_msg_format */
+} fecl_decoder_options;
+
+typedef struct fecl_ldpc fecl_ldpc;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* fecl_last_error(void);
+
+/* Ldpc::DvbS2::matrix(n, rate) (src/Ldpc.h:66-75, src/DvbS2.cpp:1595-1645), correct for every
+ * normal (64800) and short (16200) DVB-S2 rate.  dims = {M, N, K, E}; pass NULL row_ptr/col to
+ * query sizes (row_ptr has M+1 entries, col has E). */
+int fecl_dvbs2_matrix(int32_t n, int32_t rate_num, int32_t rate_den, int64_t* dims,
+                      int32_t* row_ptr, int32_t* col);
+
+/* Structure::scalingMapToVector (src/detail/Ldpc.cpp:64-112): map entry i is
+ * degree keys[i] (0 = default) -> values[offsets[i] .. offsets[i]+lengths[i]).
+ * Writes rows*cols doubles into `table` (capacity `table_capacity`). */
+int fecl_ldpc_scaling_table(const int32_t* row_ptr, int64_t M, int64_t iterations,
+                            const int64_t* keys, const int64_t* lengths, const int64_t* offsets,
+                            const double* values, int64_t entries,
+                            double* table, int64_t table_capacity, int32_t* rows, int32_t* cols);
+
+/* fec::Ldpc(EncoderOptions(H), DecoderOptions) (src/Ldpc.h:82-84): copies H to the device. */
+int fecl_ldpc_create(const int32_t* row_ptr, const int32_t* col, int64_t M, int64_t N,
+                     const fecl_decoder_options* options, fecl_ldpc** out);
+int fecl_ldpc_destroy(fecl_ldpc* h);
+
+/* Ldpc::setDecoderOptions / getDecoderOptions (src/Ldpc.h:91-92).  get copies the scaling
+ * table into `scaling` (capacity `capacity` doubles) when non-NULL. */
+int fecl_ldpc_set_options(fecl_ldpc* h, const fecl_decoder_options* options);
+int fecl_ldpc_get_options(const fecl_ldpc* h, fecl_decoder_options* options, double* scaling,
+                          int64_t capacity);
+
+/* Codec::msgSize/systSize/paritySize/stateSize (src/Codec.h:60-63). */
+int fecl_ldpc_sizes(const fecl_ldpc* h, int64_t* msg, int64_t* syst, int64_t* parity,
+                    int64_t* state);
+
+/* Codec::encode / Codec::check (src/Codec.h:68-72; src/detail/Ldpc.cpp:160-209) for
+ * staircase codes such as DVB-S2 (host computation). */
+int fecl_ldpc_encode(const fecl_ldpc* h, const uint8_t* msg, int64_t frames, uint8_t* parity);
+int fecl_ldpc_check(const fecl_ldpc* h, const uint8_t* parity, int64_t frames, int32_t* ok);
+
+/* Codec::decode -> Ldpc::decodeBlocks (src/Codec.h:210-238, src/Ldpc.cpp:55-59), host buffers:
+ * llr [frames][N] double in, msg [frames][K] (0/1 bytes) out.  iterations (nullable)
+ * receives the per-frame count of check-node updates executed (FeCl computes it but does not
+ * expose it: src/detail/BpDecoder/BpDecoderImpl.cpp:48-61). */
+int fecl_ldpc_decode(fecl_ldpc* h, const double* llr, int64_t frames, uint8_t* msg,
+                     int32_t* iterations);
+
+/* Codec::soDecode -> Ldpc::soDecodeBlocks (src/Codec.h:249-307), host buffers.
+ * Inputs: parity [F][N] (required), syst [F][K] and state [F][E] (nullable).
+ * Outputs (each nullable): msg [F][K] a-posteriori, syst [F][K] and parity [F][N]
+ * extrinsic, state [F][E] check-to-variable messages. */
+int fecl_ldpc_sodecode(fecl_ldpc* h, const double* parity_in, const double* syst_in,
+                       const double* state_in, int64_t frames, double* msg_out,
+                       double* syst_out, double* parity_out, double* state_out,
+                       int32_t* iterations);
+
+/* Device-resident forms for callers that keep data in HBM (same semantics, FP32 LLRs,
+ * asynchronous on `stream` (a hipStream_t; NULL = the null stream)).  All pointers are device
+ * pointers; nullable ones as above. */
+int fecl_ldpc_decode_device(fecl_ldpc* h, const float* llr, int64_t frames, uint8_t* msg,
+                            int32_t* iterations, void* stream);
+int fecl_ldpc_sodecode_device(fecl_ldpc* h, const float* parity_in, const float* syst_in,
+                              const float* state_in, int64_t frames, float* msg_out,
+                              float* syst_out, float* parity_out, float* state_out,
+                              int32_t* iterations, void* stream);
+
+/* Kernel timing (no counterpart in the reference; instrumentation for benchmarks).
+ * While enabled, each decode records HIP events on its launch stream around the check-node,
+ * variable-node and syndrome/control launches of every step and waits for the batch to finish.
+ * profile_read returns the accumulated milliseconds and launch counts of those three groups.
+ * Enabling resets the totals. */
+int fecl_ldpc_profile(fecl_ldpc* h, int32_t enable);
+int fecl_ldpc_profile_read(fecl_ldpc* h, double* ms /*[3]*/, int64_t* launches /*[3]*/);
+
+/* Releases the cached device workspaces of a handle (they are reused across calls). */
+int fecl_ldpc_trim(fecl_ldpc* h);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FECL_LDPC_H */

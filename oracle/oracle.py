@@ -113,7 +113,7 @@ def decode(code, llr, alg, iterations, sf=None, prec=64, msg_fmt=MSG_F32, syst_i
         syst_in = np.ascontiguousarray(syst_in, dtype=np.float64).reshape(F, code.K)
     if state_in is not None:
         state_in = np.ascontiguousarray(state_in, dtype=np.float64).reshape(F, code.E)
-    threads = threads or (os.cpu_count() or 1)
+    threads = threads or min(16, os.cpu_count() or 1)
     rc = lib().orc_bp_decode(_p(code.row_ptr), _p(code.col), code.M, code.N, code.K, _p(llr), F,
                              alg, iterations, _p(sf), sf.shape[0], sf.shape[1], prec, msg_fmt,
                              _p(syst_in), _p(state_in), _p(out.get("hard")), _p(out.get("msg")),

@@ -1,0 +1,115 @@
+"""CPU-side checks of the product library: it loads, exports every C-ABI symbol the header
+declares, and its host logic (DVB-S2 construction, scaling-table rules, option builder,
+argument validation) behaves as FeCl's.  No decode runs here (no GPU)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import fecl_amd as F
+from fecl_amd._lib import LIB_PATH, SIGNATURES, lib
+from oracle import oracle as O
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+HEADER = os.path.join(HERE, "..", "include", "fecl_ldpc.h")
+GOLD = np.load(os.path.join(HERE, "golden", "fecl_golden.npz"))
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\*?\s+\*?(fecl_\w+)\s*\(", text, flags=re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    assert os.path.exists(LIB_PATH)
+    L = ctypes.CDLL(LIB_PATH)
+    names = declared_functions()
+    assert len(names) >= 17
+    for n in names:
+        assert hasattr(L, n), n
+    assert set(names) == set(SIGNATURES), "binding table out of sync with the header"
+
+
+def test_dvbs2_r12_is_fecl_matrix():
+    import hashlib
+    H = F.DvbS2.matrix(64800, 0.5)
+    h = hashlib.sha256(H.row_ptr.tobytes()).hexdigest() + hashlib.sha256(H.col.tobytes()).hexdigest()
+    assert h == GOLD["dvbs2_r12_csr_sha"].tobytes().decode()
+
+
+@pytest.mark.parametrize("n,rate", [(n, r) for n, rs in F.ldpc._DVBS2_RATES.items() for r in rs])
+def test_dvbs2_all_rates_match_independent_construction(n, rate):
+    H = F.DvbS2.matrix(n, rate)
+    oc = O.dvbs2(n, *rate)
+    assert np.array_equal(H.row_ptr, oc.row_ptr) and np.array_equal(H.col, oc.col)
+    # K info bits, M = N - K checks, staircase parity, codewords satisfy H c = 0
+    K = H.cols() - H.rows()
+    rng = np.random.default_rng(n + rate[0])
+    info = rng.integers(0, 2, (2, K), dtype=np.uint8)
+    cw = O.encode(oc, info)
+    assert O.syndrome_weight(oc, cw[0]) == 0 and O.syndrome_weight(oc, cw[1]) == 0
+
+
+def test_dvbs2_invalid():
+    with pytest.raises(ValueError, match="Invalid length"):
+        F.DvbS2.matrix(1000, 0.5)
+    with pytest.raises(ValueError, match="Invalid rate"):
+        F.DvbS2.matrix(64800, 0.7)
+    with pytest.raises(ValueError, match="Invalid rate"):
+        F.DvbS2.matrix(16200, 0.9)  # 9/10 exists only for normal frames
+
+
+def test_scaling_table_rules():
+    """Structure::scalingMapToVector (src/detail/Ldpc.cpp:64-112)."""
+    H = F.DvbS2.matrix(64800, 0.5)  # row degrees 6 and 7 -> 6 columns
+    t = F.scaling_table(H, F.DecoderOptions().iterations(3).scalingFactor(0.75))
+    assert t.shape == (1, 6) and np.all(t == 0.75)
+    t = F.scaling_table(H, F.DecoderOptions().iterations(3).scalingFactor({0: [0.5, 0.6, 0.7], 7: [1, 2, 3]}))
+    assert t.shape == (3, 6)
+    assert np.allclose(t[:, 0], [0.5, 0.6, 0.7]) and np.allclose(t[:, 5], [1, 2, 3])
+    t = F.scaling_table(H, F.DecoderOptions().iterations(3).scalingFactor({6: [0.25], 7: [0.5]}))
+    assert t.shape == (1, 6) and t[0, 4] == 0.25 and t[0, 5] == 0.5
+    with pytest.raises(ValueError, match="Wrong size for scaling factor"):
+        F.scaling_table(H, F.DecoderOptions().iterations(3).scalingFactor({0: [0.5, 0.6]}))
+    with pytest.raises(ValueError, match="Wrong size for scaling factor"):
+        F.scaling_table(H, F.DecoderOptions().iterations(3).scalingFactor({0: [0.5], 7: [1, 2]}))
+    with pytest.raises(ValueError, match="Scaling factor not defined and no default"):
+        F.scaling_table(H, F.DecoderOptions().iterations(3).scalingFactor({7: [0.5]}))
+
+
+def test_decoder_options_defaults():
+    """Ldpc::DecoderOptions defaults (src/detail/Ldpc.h:56-74)."""
+    o = F.DecoderOptions()
+    assert o.algorithm() == F.Approximate
+    assert o.iterations() is None
+    assert o.scalingFactor() == {0: [1.0]}
+    assert o.msgFormat() == F.MsgFormat.F32
+    assert int(F.Exact) == 0 and int(F.Linear) == 1 and int(F.Approximate) == 2
+
+
+def test_create_reports_errors_without_gpu():
+    """No GPU here: handle creation reports a device error, never crashes; bad graphs are
+    rejected before any device call."""
+    H = F.DvbS2.matrix(16200, 0.5)
+    opts = F.DecoderOptions().iterations(10)
+    bad = F.CheckMatrix(np.array([0, 1], np.int32), np.array([0], np.int32), 4)
+    with pytest.raises(ValueError, match="at least two edges"):
+        F.Ldpc(bad, opts)
+    oob = F.CheckMatrix(np.array([0, 2], np.int32), np.array([0, 9], np.int32), 4)
+    with pytest.raises(ValueError, match="out of range"):
+        F.Ldpc(oob, opts)
+    try:
+        import torch
+        has_gpu = torch.cuda.is_available()
+    except Exception:
+        has_gpu = False
+    if not has_gpu:
+        with pytest.raises(Exception):
+            F.Ldpc(H, opts)
+
+
+def test_last_error_is_a_string():
+    assert isinstance(lib().fecl_last_error(), bytes)

@@ -1,0 +1,165 @@
+"""GPU parity: the HIP decoder (through the C ABI) against the CPU oracle on the same LLRs.
+
+Bar, per path:
+  * min-sum FP32 and Linear FP32: bit-identical to the oracle's FP32 restatement (which is
+    bit-identical to FeCl's own decoder instantiated with a float LLR type): hard bits,
+    a-posteriori/extrinsic values, state and iteration counts.
+  * min-sum / Linear with FP16 or BF16 message storage: bit-identical to the oracle's FP32
+    restatement with the same storage rounding.
+  * Exact FP32 (stable box-plus, no tanh saturation) against FeCl's FP64 decoder: identical
+    hard decisions and iteration counts on these inputs, a-posteriori LLRs within 1e-4
+    relative (|d| <= 1e-4 * max(1, |ref|)) where FeCl's value is finite.
+"""
+import numpy as np
+import pytest
+
+import fecl_amd as F
+from oracle import oracle as O
+from tests.golden.vectors import dvbs2_llr, gallager_llr
+
+pytestmark = pytest.mark.gpu
+
+GOLD = np.load(__import__("os").path.join(__import__("os").path.dirname(__file__), "golden", "fecl_golden.npz"))
+
+
+def gallager():
+    N, K = (int(x) for x in GOLD["gallager_nk"])
+    oc = O.Code(GOLD["gallager_row_ptr"], GOLD["gallager_col"], N, K)
+    return oc, F.CheckMatrix(oc.row_ptr, oc.col, N)
+
+
+def dvbs2(n=64800, rate=(1, 2)):
+    oc = O.dvbs2(n, *rate)
+    return oc, F.CheckMatrix(oc.row_ptr, oc.col, oc.N)
+
+
+def gpu_decoder(H, alg, iters, sf=1.0, fmt=F.MsgFormat.F32):
+    opt = F.DecoderOptions().algorithm(alg).iterations(iters).scalingFactor(sf).msgFormat(fmt)
+    return F.Ldpc(H, opt)
+
+
+def oracle_run(oc, llr, alg, iters, sf=1.0, prec=32, fmt=O.MSG_F32, **kw):
+    return O.decode(oc, llr, int(alg), iters, sf=O.scaling_table(oc, sf), prec=prec, msg_fmt=int(fmt), **kw)
+
+
+def bits(a):
+    return np.ascontiguousarray(a).view(np.uint64 if a.dtype == np.float64 else np.uint32)
+
+
+def assert_bit_exact(gpu, ref, frames, K, N, E):
+    assert np.array_equal(gpu["iterations"], ref["iters"]), "iteration counts"
+    for name, n in (("msg", K), ("syst", K), ("parity", N), ("state", E)):
+        g = np.asarray(gpu[name]).reshape(frames, n)
+        r = ref[name].reshape(frames, n)
+        # the oracle's FP32 values are exact in float64
+        assert np.array_equal(bits(g.astype(np.float32)), bits(r.astype(np.float32))), name
+
+
+@pytest.mark.parametrize("alg", [F.Approximate, F.Linear])
+@pytest.mark.parametrize("iters,sf", [(1, 1.0), (2, 1.0), (5, 1.0), (30, 1.0), (30, 0.75), (30, 0.9),
+                                      (5, [0.5, 0.625, 0.75, 0.875, 1.0])])
+def test_gallager_fp32_bit_exact(alg, iters, sf):
+    oc, H = gallager()
+    llr = np.concatenate([gallager_llr(oc.N, oc.K, 3, 2.0, 11), gallager_llr(oc.N, oc.K, 2, -1.0, 12)])
+    llr32 = llr.astype(np.float32).astype(np.float64)
+    ref = oracle_run(oc, llr32, alg, iters, sf)
+    dec = gpu_decoder(H, alg, iters, {0: sf} if isinstance(sf, list) else sf)
+    out = dec.soDecode(llr32)
+    assert_bit_exact(out, ref, llr.shape[0], oc.K, oc.N, oc.E)
+    hard, it = dec.decode(llr32, iterations=True)
+    assert np.array_equal(hard.reshape(-1, oc.K), ref["hard"])
+    assert np.array_equal(it, ref["iters"])
+
+
+@pytest.mark.parametrize("frames", [1, 63, 65, 130])
+def test_dvbs2_minsum_fp32_ragged_frames(frames):
+    oc, H = dvbs2()
+    llr = dvbs2_llr(frames, 1.4, seed=100 + frames).astype(np.float32).astype(np.float64)
+    ref = oracle_run(oc, llr, F.Approximate, 50)
+    dec = gpu_decoder(H, F.Approximate, 50)
+    out = dec.soDecode(llr)
+    assert_bit_exact(out, ref, frames, oc.K, oc.N, oc.E)
+    hard, it = dec.decode(llr, iterations=True)
+    assert np.array_equal(hard.reshape(frames, oc.K), ref["hard"])
+
+
+def test_dvbs2_minsum_fp32_1024_frames_device_path():
+    """The config-2 shape: 1024 frames of DVB-S2 r1/2, FP32 LLRs in HBM, min-sum, 50 iterations."""
+    import torch
+    oc, H = dvbs2()
+    rng_frames = [dvbs2_llr(256, e, seed=300 + i) for i, e in enumerate((0.6, 1.0, 1.4, 2.0))]
+    llr = np.concatenate(rng_frames).astype(np.float32)
+    ref = oracle_run(oc, llr.astype(np.float64), F.Approximate, 50, want=("hard", "iters"), threads=16)
+    dec = gpu_decoder(H, F.Approximate, 50)
+    d_llr = torch.from_numpy(llr).cuda()
+    hard, it = dec.decode(d_llr, iterations=True)
+    torch.cuda.synchronize()
+    assert np.array_equal(hard.cpu().numpy().reshape(-1, oc.K), ref["hard"])
+    assert np.array_equal(it.cpu().numpy(), ref["iters"])
+
+
+@pytest.mark.parametrize("fmt,ofmt", [(F.MsgFormat.F16, O.MSG_F16), (F.MsgFormat.BF16, O.MSG_BF16)])
+@pytest.mark.parametrize("alg", [F.Approximate, F.Linear])
+def test_16bit_messages_bit_exact(fmt, ofmt, alg):
+    oc, H = dvbs2()
+    llr = dvbs2_llr(5, 1.2, seed=7).astype(np.float32).astype(np.float64)
+    ref = oracle_run(oc, llr, alg, 50, fmt=ofmt)
+    out = gpu_decoder(H, alg, 50, fmt=fmt).soDecode(llr)
+    assert_bit_exact(out, ref, 5, oc.K, oc.N, oc.E)
+
+
+@pytest.mark.parametrize("code", ["gallager", "dvbs2"])
+def test_exact_fp32_against_fp64(code):
+    oc, H = gallager() if code == "gallager" else dvbs2()
+    if code == "gallager":
+        llr = gallager_llr(oc.N, oc.K, 4, 2.0, 11)
+    else:
+        llr = dvbs2_llr(3, 1.0, seed=21)
+    llr = llr.astype(np.float32).astype(np.float64)
+    ref = oracle_run(oc, llr, F.Exact, 50, prec=64)
+    out = gpu_decoder(H, F.Exact, 50).soDecode(llr)
+    hard = gpu_decoder(H, F.Exact, 50).decode(llr)
+    assert np.array_equal(hard.reshape(-1, oc.K), ref["hard"])
+    assert np.array_equal(out["iterations"], ref["iters"])
+    r = ref["msg"].ravel()
+    g = out["msg"].astype(np.float64)
+    fin = np.isfinite(r)
+    err = np.abs(g[fin] - r[fin]) / np.maximum(1.0, np.abs(r[fin]))
+    assert err.max() <= 1e-4, err.max()
+    assert np.all(np.sign(g[~fin]) == np.sign(r[~fin]))
+
+
+def test_sodecode_state_and_syst_inputs():
+    """soDecode with a-priori state and systematic inputs (BpDecoderImpl.cpp:73-93)."""
+    oc, H = gallager()
+    llr = gallager_llr(oc.N, oc.K, 2, -1.0, 12).astype(np.float32).astype(np.float64)
+    first = oracle_run(oc, llr, F.Approximate, 7)
+    state = first["state"].astype(np.float32).astype(np.float64)
+    syst = np.random.default_rng(99).normal(0, 1, (2, oc.K)).astype(np.float32).astype(np.float64)
+    ref = oracle_run(oc, llr, F.Approximate, 7, state_in=state, syst_in=syst)
+    out = gpu_decoder(H, F.Approximate, 7).soDecode(llr, syst=syst, state=state)
+    assert_bit_exact(out, ref, 2, oc.K, oc.N, oc.E)
+
+
+def test_two_phase_resume_equals_single_run():
+    """test_soDecode_2phases (test/Cpp/operations.h:344-383): I iterations, state out, state
+    back in with I more == one 2I run, at a low SNR where nothing converges."""
+    oc, H = gallager()
+    llr = gallager_llr(oc.N, oc.K, 2, -5.0, 5).astype(np.float32).astype(np.float64)
+    d1 = gpu_decoder(H, F.Approximate, 10)
+    a = d1.soDecode(llr, outputs=("state",))
+    b = d1.soDecode(llr, state=a["state"], outputs=("msg",))
+    c = gpu_decoder(H, F.Approximate, 20).soDecode(llr, outputs=("msg",))
+    assert np.array_equal(b["msg"], c["msg"])
+
+
+def test_zero_frames_and_errors():
+    oc, H = gallager()
+    dec = gpu_decoder(H, F.Approximate, 5)
+    assert dec.decode(np.zeros(0)).size == 0
+    with pytest.raises(ValueError):
+        dec.decode(np.zeros(oc.N + 1))
+    with pytest.raises(ValueError):
+        dec.soDecode(np.zeros(oc.N), syst=np.zeros(3))
+    with pytest.raises(ValueError):
+        F.Ldpc(H, F.DecoderOptions().iterations(0))
