@@ -168,6 +168,9 @@ DevOpts make_opts(const fecl_ldpc* h) {
   o.sf_mode = h->sf_mode;
   o.iterations = h->iterations;
   o.algorithm = h->algorithm;
+  // With one scaling row the check update after a passing syndrome equals FeCl's final one
+  // (factor row I-1), so the syndrome can ride in the next check kernel.
+  o.fused_syn = h->sf_rows == 1 ? 1 : 0;
   // FeCl's Linearlog1pexpm table in float: y[i] = log(1 + exp(-i/2)) evaluated in double
   // (src/detail/LinearTable.h:40-44, 63), slopes y[i+1] - y[i] in float.
   float y[8];
@@ -272,9 +275,9 @@ int run_batch(fecl_ldpc* h, Workspace* ws, hipStream_t st, const BatchIO& io, in
     if (prof) HIP_TRY(hipEventRecord(ws->pev[step][0], st));
     HIP_TRY(kernels_check(fmt, h->dc, s, o, c.max_row_degree, st));
     if (prof) HIP_TRY(hipEventRecord(ws->pev[step][1], st));
-    HIP_TRY(kernels_variable(fmt, h->dc, s, c.max_var_degree, st));
+    HIP_TRY(kernels_variable(fmt, h->dc, s, out, c.max_var_degree, o.fused_syn, st));
     if (prof) HIP_TRY(hipEventRecord(ws->pev[step][2], st));
-    HIP_TRY(kernels_syndrome_control(fmt, h->dc, s, h->iterations, step, st));
+    HIP_TRY(kernels_syndrome_control(fmt, h->dc, s, h->iterations, o.fused_syn, step, st));
     if (prof) HIP_TRY(hipEventRecord(ws->pev[step][3], st));
     steps_run = step + 1;
     HIP_TRY(hipMemcpyAsync(&ws->host_active[step], &s.active[step], 4, hipMemcpyDeviceToHost, st));
@@ -286,7 +289,6 @@ int run_batch(fecl_ldpc* h, Workspace* ws, hipStream_t st, const BatchIO& io, in
       if (ws->host_active[step - 2] == 0) break;
     }
   }
-  HIP_TRY(kernels_output(fmt, h->dc, s, out, c.max_var_degree, st));
   HIP_TRY(kernels_results(fmt, h->dc, s, out, frames, frame0, io.res, io.dbl_out, st));
   if (prof) {
     // kernel durations from events on the launch stream: check, variable, syndrome+control

@@ -27,7 +27,9 @@ enum : uint8_t {
   PH_FRESH = 4,    // v2c not yet materialised: the next check update reads L[col(e)]
   PH_VNINIT = 8,   // state supplied: a variable-node pass must build v2c from it first
 };
-enum : uint32_t { TF_CN = 1, TF_VN = 2, TF_VNINIT = 4 };
+// Tile flags: any frame RUN or FINAL (check and variable kernels work), any frame RUN
+// (syndrome works), any frame waiting for its state-seeded variable pass.
+enum : uint32_t { TF_LIVE = 1, TF_RUN = 2, TF_VNINIT = 4 };
 
 struct DevCode {
   const int32_t* row_ptr;
@@ -56,6 +58,7 @@ struct DevOpts {
   int32_t sf_mode;         // 0: all factors 1.0; 1: all exactly representable in float; 2: general
   int32_t iterations;
   int32_t algorithm;
+  int32_t fused_syn;       // syndrome folded into the check kernel (scaling rows == 1)
   float lin_y[8];          // Linear table knots y[i] = log(1 + exp(-i/2)) (float)
   float lin_s[8];          // slopes y[i+1] - y[i] (float)
 };

@@ -85,38 +85,79 @@ __device__ static inline float apply_sf(const DevOpts& o, int row, int dcol, flo
 }
 
 // ------------------------------------------------------------------------------------------
-// Check-node update, min-sum (DecoderAlgorithm::Approximate).  MinBoxSum is sign * min and is
-// associative, so FeCl's forward/backward pass equals "min of the others, xor of the others'
-// signs" exactly: output j has magnitude min_{i != j} |x_i| and sign bit
-// ((d-2) & 1) ^ xor_{i != j} signbit(x_i)  (a box-sum of two operands sets the sign bit
-// 1 ^ s_a ^ s_b).  One wavefront per (check row, tile); two passes, no per-edge registers.
-template <class Fmt>
-__global__ __launch_bounds__(kBlock) void cn_minsum_kernel(DevCode c, DevState s, DevOpts o) {
-  constexpr int FPL = Fmt::FPL, TW = kWave * FPL;
-  const int t = blockIdx.y;
-  if (!(s.tile_flags[t] & TF_CN)) return;
-  const int r = uniform(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6));
-  if (r >= c.M) return;
-  const int lane = threadIdx.x & 63;
+// Per-lane view of the frames a check-node wavefront serves.
+template <int FPL>
+struct CnLanes {
+  bool act[FPL];     // frame takes a check update this step (RUN or FINAL)
+  bool fresh[FPL];   // its v2c is still L[col(e)]
+  int sfrow[FPL];    // scaling-factor row: iteration count, or I-1 for the final update
+  bool lane_any, lane_all, wave_fresh, wave_seen;  // wave_seen: some frame has a prior VN
+};
 
-  bool act[FPL], fresh[FPL];
-  int sfrow[FPL];
-  bool lane_any = false, lane_all = true, any_fresh = false;
+template <int FPL>
+__device__ static inline bool cn_lanes(const DevState& s, const DevOpts& o, int t, int lane, CnLanes<FPL>& L) {
+  constexpr int TW = kWave * FPL;
+  L.lane_any = false;
+  L.lane_all = true;
+  bool any_fresh = false, seen = false;
 #pragma unroll
   for (int k = 0; k < FPL; ++k) {
     const int64_t p = (int64_t)t * TW + lane * FPL + k;
     const uint8_t ph = s.phase[p];
     const int base = ph & PH_MASK;
-    act[k] = base == PH_RUN || base == PH_FINAL;
-    fresh[k] = act[k] && (ph & PH_FRESH);
-    const int tt = base == PH_FINAL ? o.iterations - 1 : s.iter[p];
-    sfrow[k] = tt % o.sf_rows;
-    lane_any |= act[k];
-    lane_all &= act[k];
-    any_fresh |= fresh[k];
+    const int it = s.iter[p];
+    L.act[k] = base == PH_RUN || base == PH_FINAL;
+    L.fresh[k] = L.act[k] && (ph & PH_FRESH);
+    L.sfrow[k] = (base == PH_FINAL ? o.iterations - 1 : it) % o.sf_rows;
+    L.lane_any |= L.act[k];
+    L.lane_all &= L.act[k];
+    any_fresh |= L.fresh[k];
+    seen |= base == PH_RUN && it >= 1;
   }
-  if (__ballot(lane_any) == 0) return;
-  const bool wave_fresh = __ballot(any_fresh) != 0;
+  if (__ballot(L.lane_any) == 0) return false;
+  L.wave_fresh = __ballot(any_fresh) != 0;
+  L.wave_seen = __ballot(seen) != 0;
+  return true;
+}
+
+// Fused syndrome (used when the scaling factors do not change with the iteration, so the
+// check update that follows a successful syndrome equals FeCl's final one): the row's parity
+// over the hard decisions of the previous variable update, or-ed into the tile's fail mask.
+template <int FPL>
+__device__ static inline void cn_syndrome(const DevCode& c, const DevState& s, int t, int e0, int d, int lane) {
+  uint64_t syn[FPL];
+#pragma unroll
+  for (int k = 0; k < FPL; ++k) syn[k] = 0;
+  const uint64_t* ht = s.hard + (int64_t)t * c.N * FPL;
+  for (int j = 0; j < d; ++j) {
+    const int v = uniform(c.col[e0 + j]);
+#pragma unroll
+    for (int k = 0; k < FPL; ++k) syn[k] ^= ht[(int64_t)v * FPL + k];
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < FPL; ++k)
+      if (syn[k]) atomicOr((unsigned long long*)&s.fail[t * FPL + k], (unsigned long long)syn[k]);
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// Check-node update, min-sum (DecoderAlgorithm::Approximate).  MinBoxSum is sign * min and is
+// associative, so FeCl's forward/backward pass equals "min of the others, xor of the others'
+// signs" exactly: output j has magnitude min_{i != j} |x_i| and sign bit
+// ((d-2) & 1) ^ xor_{i != j} signbit(x_i)  (a box-sum of two operands sets the sign bit
+// 1 ^ s_a ^ s_b).  One wavefront per (check row, tile).  DMAX > 0: the row's d words are
+// loaded into registers at once (d <= DMAX); DMAX == 0: two streaming passes (any d <= 64).
+template <class Fmt, int DMAX>
+__global__ __launch_bounds__(kBlock) void cn_minsum_kernel(DevCode c, DevState s, DevOpts o) {
+  constexpr int FPL = Fmt::FPL, TW = kWave * FPL;
+  const int t = blockIdx.y;
+  if (!(s.tile_flags[t] & TF_LIVE)) return;
+  const int r = uniform(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6));
+  if (r >= c.M) return;
+  const int lane = threadIdx.x & 63;
+  CnLanes<FPL> ln;
+  if (!cn_lanes<FPL>(s, o, t, lane, ln)) return;
 
   const int e0 = uniform(c.row_ptr[r]);
   const int d = uniform(c.row_ptr[r + 1]) - e0;
@@ -128,20 +169,7 @@ __global__ __launch_bounds__(kBlock) void cn_minsum_kernel(DevCode c, DevState s
   uint64_t sg[FPL];
 #pragma unroll
   for (int k = 0; k < FPL; ++k) { m1[k] = __builtin_inff(); m2[k] = __builtin_inff(); i1[k] = 0; sg[k] = 0; }
-
-  for (int j = 0; j < d; ++j) {
-    float x[FPL];
-    if (lane_any) Fmt::unpack(mrow[(int64_t)j * kWave], x);
-    else {
-#pragma unroll
-      for (int k = 0; k < FPL; ++k) x[k] = 0.0f;
-    }
-    if (wave_fresh) {
-      const int v = uniform(c.col[e0 + j]);
-#pragma unroll
-      for (int k = 0; k < FPL; ++k)
-        if (fresh[k]) x[k] = Fmt::quantize(Lt[(int64_t)v * TW + k]);
-    }
+  auto accumulate = [&](int j, const float (&x)[FPL]) {
 #pragma unroll
     for (int k = 0; k < FPL; ++k) {
       const float a = fabsf(x[k]);
@@ -149,33 +177,67 @@ __global__ __launch_bounds__(kBlock) void cn_minsum_kernel(DevCode c, DevState s
       if (a < m1[k]) { m2[k] = m1[k]; m1[k] = a; i1[k] = j; }
       else if (a < m2[k]) { m2[k] = a; }
     }
+  };
+  auto load_x = [&](int j, float (&x)[FPL]) {
+    Fmt::unpack(mrow[(int64_t)j * kWave], x);
+    if (ln.wave_fresh) {
+      const int v = uniform(c.col[e0 + j]);
+#pragma unroll
+      for (int k = 0; k < FPL; ++k)
+        if (ln.fresh[k]) x[k] = Fmt::quantize(Lt[(int64_t)v * TW + k]);
+    }
+  };
+
+  float xs[DMAX > 0 ? DMAX : 1][FPL];
+  if (DMAX > 0) {
+#pragma unroll
+    for (int j = 0; j < DMAX; ++j)
+      if (j < d) load_x(j, xs[j]);
+  }
+  if (o.fused_syn && ln.wave_seen) cn_syndrome<FPL>(c, s, t, e0, d, lane);
+  if (DMAX > 0) {
+#pragma unroll
+    for (int j = 0; j < DMAX; ++j)
+      if (j < d) accumulate(j, xs[j]);
+  } else {
+    for (int j = 0; j < d; ++j) {
+      float x[FPL];
+      load_x(j, x);
+      accumulate(j, x);
+    }
   }
   float m1s[FPL], m2s[FPL];
   uint32_t par[FPL];
 #pragma unroll
   for (int k = 0; k < FPL; ++k) {
     par[k] = (uint32_t)(((d - 2) & 1) ^ (__popcll(sg[k]) & 1));
-    m1s[k] = apply_sf(o, sfrow[k], d - 2, m1[k]);
-    m2s[k] = apply_sf(o, sfrow[k], d - 2, m2[k]);
+    m1s[k] = apply_sf(o, ln.sfrow[k], d - 2, m1[k]);
+    m2s[k] = apply_sf(o, ln.sfrow[k], d - 2, m2[k]);
   }
-  if (!lane_any) return;
-  for (int j = 0; j < d; ++j) {
+  if (!ln.lane_any) return;
+  auto emit = [&](int j, const float (&old)[FPL]) {
     float y[FPL];
 #pragma unroll
     for (int k = 0; k < FPL; ++k) {
       const float mag = (j == i1[k]) ? m2s[k] : m1s[k];
       const uint32_t sb = par[k] ^ (uint32_t)((sg[k] >> j) & 1);
-      y[k] = sb ? -mag : mag;
+      y[k] = ln.act[k] ? (sb ? -mag : mag) : old[k];   // other frames keep their word
     }
-    uint32_t* dst = mrow + (int64_t)j * kWave;
-    if (lane_all) {
-      *dst = Fmt::pack(y);
-    } else {
-      float old[FPL];
-      Fmt::unpack(*dst, old);
+    mrow[(int64_t)j * kWave] = Fmt::pack(y);
+  };
+  if (DMAX > 0) {
 #pragma unroll
-      for (int k = 0; k < FPL; ++k) if (!act[k]) y[k] = old[k];
-      *dst = Fmt::pack(y);
+    for (int j = 0; j < DMAX; ++j)
+      if (j < d) emit(j, xs[j]);
+  } else {
+    for (int j = 0; j < d; ++j) {
+      float old[FPL];
+      if (!ln.lane_all) Fmt::unpack(mrow[(int64_t)j * kWave], old);
+      else {
+#pragma unroll
+        for (int k = 0; k < FPL; ++k) old[k] = 0.0f;
+      }
+      emit(j, old);
     }
   }
 }
@@ -219,41 +281,29 @@ template <class Fmt, int DMAX, int ALG>
 __global__ __launch_bounds__(kBlock) void cn_box_kernel(DevCode c, DevState s, DevOpts o) {
   constexpr int FPL = Fmt::FPL, TW = kWave * FPL;
   const int t = blockIdx.y;
-  if (!(s.tile_flags[t] & TF_CN)) return;
+  if (!(s.tile_flags[t] & TF_LIVE)) return;
   const int r = uniform(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6));
   if (r >= c.M) return;
   const int lane = threadIdx.x & 63;
-
-  bool act[FPL], fresh[FPL];
-  int sfrow[FPL];
-  bool lane_any = false, lane_all = true, any_fresh = false;
-#pragma unroll
-  for (int k = 0; k < FPL; ++k) {
-    const int64_t p = (int64_t)t * TW + lane * FPL + k;
-    const uint8_t ph = s.phase[p];
-    const int base = ph & PH_MASK;
-    act[k] = base == PH_RUN || base == PH_FINAL;
-    fresh[k] = act[k] && (ph & PH_FRESH);
-    const int tt = base == PH_FINAL ? o.iterations - 1 : s.iter[p];
-    sfrow[k] = tt % o.sf_rows;
-    lane_any |= act[k];
-    lane_all &= act[k];
-    any_fresh |= fresh[k];
-  }
-  if (__ballot(lane_any) == 0) return;
-  const bool wave_fresh = __ballot(any_fresh) != 0;
+  CnLanes<FPL> ln;
+  if (!cn_lanes<FPL>(s, o, t, lane, ln)) return;
+  const bool* act = ln.act;
+  const bool* fresh = ln.fresh;
+  const int* sfrow = ln.sfrow;
+  const bool lane_any = ln.lane_any, lane_all = ln.lane_all, wave_fresh = ln.wave_fresh;
 
   const int e0 = uniform(c.row_ptr[r]);
   const int d = uniform(c.row_ptr[r + 1]) - e0;
   uint32_t* mrow = s.msg + ((int64_t)t * c.E + e0) * kWave + lane;
   const float* Lt = s.L + (int64_t)t * c.N * TW + lane * FPL;
+  if (o.fused_syn && ln.wave_seen) cn_syndrome<FPL>(c, s, t, e0, d, lane);
 
   float x[DMAX][FPL];
   uint32_t wold[DMAX];
 #pragma unroll
   for (int j = 0; j < DMAX; ++j) {
     if (j < d) {
-      wold[j] = lane_any ? mrow[(int64_t)j * kWave] : 0u;
+      wold[j] = mrow[(int64_t)j * kWave];
       Fmt::unpack(wold[j], x[j]);
       if (wave_fresh) {
         const int v = uniform(c.col[e0 + j]);
@@ -330,25 +380,39 @@ __global__ __launch_bounds__(kBlock) void cn_box_kernel(DevCode c, DevState s, D
 // Variable-node update in FeCl's order (BpDecoderImpl.cpp:162-179):
 //   v2c_j = (0 + c_1 + ... + c_{j-1}) + (L + c_dv + ... + c_{j+1}),  APP = L + c_dv + ... + c_1
 // with edges of the variable in ascending edge order, then hard = APP >= 0 per frame.
-// One wavefront per (variable, tile).  init_pass builds v2c from a supplied state
-// (BpDecoderImpl.cpp:81-88) for frames flagged PH_VNINIT.
+// One wavefront per (variable, tile).  The same pass forms the outputs of frames whose final
+// check update ran in this step (BpDecoderImpl.cpp:61-69, :109-129): with the final c2v,
+//   decode   : app = L + c_1 + ... + c_dv (ascending), bit = app >= 0
+//   soDecode : ext = 0 + c_1 + ... + c_dv, msg = L + ext
+// and leaves their messages (the final state) untouched.  init_pass builds v2c from a
+// supplied state (BpDecoderImpl.cpp:81-88) for frames flagged PH_VNINIT.
 template <class Fmt, int VMAX>
-__global__ __launch_bounds__(kBlock) void vn_kernel(DevCode c, DevState s, int init_pass) {
+__global__ __launch_bounds__(kBlock) void vn_kernel(DevCode c, DevState s, DevOut out, int fused_syn, int init_pass) {
   constexpr int FPL = Fmt::FPL, TW = kWave * FPL;
   const int t = blockIdx.y;
-  if (!(s.tile_flags[t] & (init_pass ? TF_VNINIT : TF_VN))) return;
+  if (!(s.tile_flags[t] & (init_pass ? TF_VNINIT : TF_LIVE))) return;
   const int v = uniform(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6));
   if (v >= c.N) return;
   const int lane = threadIdx.x & 63;
-  bool run[FPL];
-  bool lane_any = false;
+  bool run[FPL], fin[FPL];
+  bool lane_run = false, lane_fin = false;
 #pragma unroll
   for (int k = 0; k < FPL; ++k) {
-    const uint8_t ph = s.phase[(int64_t)t * TW + lane * FPL + k];
-    run[k] = init_pass ? ((ph & PH_VNINIT) != 0) : ((ph & PH_MASK) == PH_RUN);
-    lane_any |= run[k];
+    const int64_t p = (int64_t)t * TW + lane * FPL + k;
+    const uint8_t ph = s.phase[p];
+    const int base = ph & PH_MASK;
+    // fused syndrome: a RUN frame whose previous hard decision satisfied every check has just
+    // had its final check update (see cn_syndrome)
+    const bool passed = fused_syn && !init_pass && base == PH_RUN && s.iter[p] >= 1 &&
+                        ((s.fail[t * FPL + k] >> lane) & 1ull) == 0;
+    run[k] = init_pass ? ((ph & PH_VNINIT) != 0) : (base == PH_RUN && !passed);
+    fin[k] = !init_pass && (base == PH_FINAL || passed);
+    lane_run |= run[k];
+    lane_fin |= fin[k];
   }
-  if (__ballot(lane_any) == 0) return;
+  const bool wave_run = __ballot(lane_run) != 0;
+  const bool wave_fin = __ballot(lane_fin) != 0;
+  if (!wave_run && !wave_fin) return;
   const int s0 = uniform(c.var_ptr[v]);
   const int dv = uniform(c.var_ptr[v + 1]) - s0;
   float Lv[FPL];
@@ -357,16 +421,48 @@ __global__ __launch_bounds__(kBlock) void vn_kernel(DevCode c, DevState s, int i
   for (int k = 0; k < FPL; ++k) Lv[k] = Lp[k];
   uint32_t* mt = s.msg + (int64_t)t * c.E * kWave + lane;
   float cv[VMAX][FPL];
-  uint32_t wold[VMAX];
   int eid[VMAX];
 #pragma unroll
   for (int j = 0; j < VMAX; ++j) {
     if (j < dv) {
       eid[j] = uniform(c.var_edges[s0 + j]);
-      wold[j] = mt[(int64_t)eid[j] * kWave];
-      Fmt::unpack(wold[j], cv[j]);
+      Fmt::unpack(mt[(int64_t)eid[j] * kWave], cv[j]);
     }
   }
+  if (wave_fin) {
+    float app[FPL], ext[FPL];
+#pragma unroll
+    for (int k = 0; k < FPL; ++k) { app[k] = Lv[k]; ext[k] = 0.0f; }
+#pragma unroll
+    for (int j = 0; j < VMAX; ++j) {
+      if (j < dv) {
+#pragma unroll
+        for (int k = 0; k < FPL; ++k) { app[k] = app[k] + cv[j][k]; ext[k] = ext[k] + cv[j][k]; }
+      }
+    }
+    if (v < c.K) {
+#pragma unroll
+      for (int k = 0; k < FPL; ++k) {
+        const uint64_t bits = __ballot(app[k] >= 0.0f);
+        const uint64_t fm = __ballot(fin[k]);
+        if (lane == 0 && fm) {
+          uint64_t* w = &out.hard[((int64_t)t * c.K + v) * FPL + k];
+          *w = (*w & ~fm) | (bits & fm);
+        }
+      }
+    }
+    if (out.ext) {
+#pragma unroll
+      for (int k = 0; k < FPL; ++k)
+        if (fin[k]) out.ext[((int64_t)t * c.N + v) * TW + lane * FPL + k] = ext[k];
+    }
+    if (out.app && v < c.K) {
+#pragma unroll
+      for (int k = 0; k < FPL; ++k)
+        if (fin[k]) out.app[((int64_t)t * c.K + v) * TW + lane * FPL + k] = Lv[k] + ext[k];
+    }
+  }
+  if (!wave_run) return;
   float pre[VMAX][FPL];
   float acc[FPL];
 #pragma unroll
@@ -388,9 +484,9 @@ __global__ __launch_bounds__(kBlock) void vn_kernel(DevCode c, DevState s, int i
       for (int k = 0; k < FPL; ++k) {
         y[k] = pre[j][k] + acc[k];
         acc[k] = acc[k] + cv[j][k];
-        if (!run[k]) y[k] = cv[j][k];
+        if (!run[k]) y[k] = cv[j][k];   // other frames keep their word (exact round trip)
       }
-      if (lane_any) mt[(int64_t)eid[j] * kWave] = Fmt::pack(y);
+      if (lane_run) mt[(int64_t)eid[j] * kWave] = Fmt::pack(y);
     }
   }
   if (init_pass) return;
@@ -407,7 +503,7 @@ __global__ __launch_bounds__(kBlock) void vn_kernel(DevCode c, DevState s, int i
 template <int FPL>
 __global__ __launch_bounds__(kBlock) void syndrome_kernel(DevCode c, DevState s) {
   const int t = blockIdx.y;
-  if (!(s.tile_flags[t] & TF_VN)) return;
+  if (!(s.tile_flags[t] & TF_RUN)) return;
   const int r = blockIdx.x * kBlock + threadIdx.x;
   uint64_t f[FPL];
 #pragma unroll
@@ -434,7 +530,7 @@ __global__ __launch_bounds__(kBlock) void syndrome_kernel(DevCode c, DevState s)
 // I-1 regular updates are used up; FINAL frames (their last update used factor row I-1)
 // turn DONE.  The iteration count is the number of check-node updates executed.
 template <int FPL>
-__global__ __launch_bounds__(kWave) void control_kernel(DevState s, int iterations, int step) {
+__global__ __launch_bounds__(kWave) void control_kernel(DevState s, int iterations, int fused_syn, int step) {
   constexpr int TW = kWave * FPL;
   const int t = blockIdx.x;
   const int lane = threadIdx.x;
@@ -447,10 +543,14 @@ __global__ __launch_bounds__(kWave) void control_kernel(DevState s, int iteratio
     uint8_t ph = s.phase[p];
     const int base = ph & PH_MASK;
     if (base == PH_RUN) {
-      const int it = s.iter[p] + 1;
+      const int it0 = s.iter[p];
+      const int it = it0 + 1;
       s.iter[p] = it;
       const bool ok = ((fw >> lane) & 1ull) == 0;
-      ph = (ok || it >= iterations - 1) ? PH_FINAL : PH_RUN;
+      if (fused_syn)  // the syndrome seen in this step is the previous update's
+        ph = (it0 >= 1 && ok) ? PH_DONE : (it >= iterations - 1 ? PH_FINAL : PH_RUN);
+      else
+        ph = (ok || it >= iterations - 1) ? PH_FINAL : PH_RUN;
     } else if (base == PH_FINAL) {
       s.iter[p] = s.iter[p] + 1;
       ph = PH_DONE;
@@ -468,7 +568,7 @@ __global__ __launch_bounds__(kWave) void control_kernel(DevState s, int iteratio
   if (lane == 0) {
 #pragma unroll
     for (int k = 0; k < FPL; ++k) s.fail[t * FPL + k] = 0;
-    s.tile_flags[t] = (bcn ? TF_CN : 0u) | (bvn ? TF_VN : 0u);
+    s.tile_flags[t] = (bcn ? TF_LIVE : 0u) | (bvn ? TF_RUN : 0u);
     if (alive) atomicAdd(&s.active[step], alive);
   }
 }
@@ -517,7 +617,7 @@ __global__ __launch_bounds__(kBlock) void ingest_kernel(DevCode c, DevState s, c
       uint32_t fl = 0;
       if (live > 0) {
         const uint8_t base = init_phase & PH_MASK;
-        fl = (base == PH_RUN || base == PH_FINAL ? TF_CN : 0u) | (base == PH_RUN ? TF_VN : 0u) |
+        fl = (base == PH_RUN || base == PH_FINAL ? TF_LIVE : 0u) | (base == PH_RUN ? TF_RUN : 0u) |
              ((init_phase & PH_VNINIT) ? TF_VNINIT : 0u);
       }
       s.tile_flags[t] = fl;
@@ -551,49 +651,6 @@ __global__ __launch_bounds__(kBlock) void state_in_kernel(DevCode c, DevState s,
       if (e < c.E) s.msg[((int64_t)t * c.E + e) * kWave + fc / FPL + ll] = Fmt::pack(x);
     }
     __syncthreads();
-  }
-}
-
-// ------------------------------------------------------------------------------------------
-// Outputs of finished frames (BpDecoderImpl.cpp:61-69 and :109-129): with c the final c2v,
-//   decode   : app = L + c_1 + ... + c_dv (ascending), hard = app >= 0 for v < K
-//   soDecode : ext = 0 + c_1 + ... + c_dv, app_msg = L + ext for v < K
-template <class Fmt>
-__global__ __launch_bounds__(kBlock) void output_kernel(DevCode c, DevState s, DevOut out) {
-  constexpr int FPL = Fmt::FPL, TW = kWave * FPL;
-  const int t = blockIdx.y;
-  const int v = uniform(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6));
-  if (v >= c.N) return;
-  const int lane = threadIdx.x & 63;
-  const int s0 = uniform(c.var_ptr[v]);
-  const int dv = uniform(c.var_ptr[v + 1]) - s0;
-  const float* Lp = s.L + ((int64_t)t * c.N + v) * TW + lane * FPL;
-  const uint32_t* mt = s.msg + (int64_t)t * c.E * kWave + lane;
-  float app[FPL], ext[FPL];
-#pragma unroll
-  for (int k = 0; k < FPL; ++k) { app[k] = Lp[k]; ext[k] = 0.0f; }
-  for (int j = 0; j < dv; ++j) {
-    const int e = uniform(c.var_edges[s0 + j]);
-    float cv[FPL];
-    Fmt::unpack(mt[(int64_t)e * kWave], cv);
-#pragma unroll
-    for (int k = 0; k < FPL; ++k) { app[k] = app[k] + cv[k]; ext[k] = ext[k] + cv[k]; }
-  }
-  if (v < c.K && out.hard) {
-#pragma unroll
-    for (int k = 0; k < FPL; ++k) {
-      const uint64_t m = __ballot(app[k] >= 0.0f);
-      if (lane == 0) out.hard[((int64_t)t * c.K + v) * FPL + k] = m;
-    }
-  }
-  if (out.ext) {
-#pragma unroll
-    for (int k = 0; k < FPL; ++k) out.ext[((int64_t)t * c.N + v) * TW + lane * FPL + k] = ext[k];
-  }
-  if (out.app && v < c.K) {
-#pragma unroll
-    for (int k = 0; k < FPL; ++k)
-      out.app[((int64_t)t * c.K + v) * TW + lane * FPL + k] = Lp[k] + ext[k];
   }
 }
 
@@ -684,12 +741,13 @@ __global__ void fill_i32_kernel(int32_t* p, int64_t n, int32_t v) {
 // Launchers (host side).
 
 template <class Fmt>
-static hipError_t launch_vn(const DevCode& c, const DevState& s, int max_dv, int init, hipStream_t st) {
+static hipError_t launch_vn(const DevCode& c, const DevState& s, const DevOut& o, int max_dv, int fused, int init,
+                            hipStream_t st) {
   dim3 grid((c.N + kWavesPerBlock - 1) / kWavesPerBlock, s.tiles);
-  if (max_dv <= 4) hipLaunchKernelGGL((vn_kernel<Fmt, 4>), grid, dim3(kBlock), 0, st, c, s, init);
-  else if (max_dv <= 8) hipLaunchKernelGGL((vn_kernel<Fmt, 8>), grid, dim3(kBlock), 0, st, c, s, init);
-  else if (max_dv <= 16) hipLaunchKernelGGL((vn_kernel<Fmt, 16>), grid, dim3(kBlock), 0, st, c, s, init);
-  else if (max_dv <= 32) hipLaunchKernelGGL((vn_kernel<Fmt, 32>), grid, dim3(kBlock), 0, st, c, s, init);
+  if (max_dv <= 4) hipLaunchKernelGGL((vn_kernel<Fmt, 4>), grid, dim3(kBlock), 0, st, c, s, o, fused, init);
+  else if (max_dv <= 8) hipLaunchKernelGGL((vn_kernel<Fmt, 8>), grid, dim3(kBlock), 0, st, c, s, o, fused, init);
+  else if (max_dv <= 16) hipLaunchKernelGGL((vn_kernel<Fmt, 16>), grid, dim3(kBlock), 0, st, c, s, o, fused, init);
+  else if (max_dv <= 32) hipLaunchKernelGGL((vn_kernel<Fmt, 32>), grid, dim3(kBlock), 0, st, c, s, o, fused, init);
   else return hipErrorInvalidValue;
   return hipGetLastError();
 }
@@ -706,28 +764,26 @@ static hipError_t launch_cn_box(const DevCode& c, const DevState& s, const DevOp
 
 template <class Fmt>
 static hipError_t launch_cn(const DevCode& c, const DevState& s, const DevOpts& o, int max_dc, hipStream_t st) {
+  dim3 grid((c.M + kWavesPerBlock - 1) / kWavesPerBlock, s.tiles);
   if (o.algorithm == 2) {
-    if (max_dc > 64) return hipErrorInvalidValue;
-    dim3 grid((c.M + kWavesPerBlock - 1) / kWavesPerBlock, s.tiles);
-    hipLaunchKernelGGL((cn_minsum_kernel<Fmt>), grid, dim3(kBlock), 0, st, c, s, o);
+    if (max_dc <= 8) hipLaunchKernelGGL((cn_minsum_kernel<Fmt, 8>), grid, dim3(kBlock), 0, st, c, s, o);
+    else if (max_dc <= 16) hipLaunchKernelGGL((cn_minsum_kernel<Fmt, 16>), grid, dim3(kBlock), 0, st, c, s, o);
+    else if (max_dc <= 32) hipLaunchKernelGGL((cn_minsum_kernel<Fmt, 32>), grid, dim3(kBlock), 0, st, c, s, o);
+    else if (max_dc <= 64) hipLaunchKernelGGL((cn_minsum_kernel<Fmt, 0>), grid, dim3(kBlock), 0, st, c, s, o);
+    else return hipErrorInvalidValue;
     return hipGetLastError();
   }
   if (o.algorithm == 1) return launch_cn_box<Fmt, 1>(c, s, o, max_dc, st);
   return launch_cn_box<Fmt, 2>(c, s, o, max_dc, st);
 }
 
+// Syndrome (unless fused into the check kernel) and the per-frame state machine.
 template <int FPL>
-static hipError_t launch_syn_ctrl(const DevCode& c, const DevState& s, int iterations, int step, hipStream_t st) {
-  hipLaunchKernelGGL((syndrome_kernel<FPL>), dim3((c.M + kBlock - 1) / kBlock, s.tiles), dim3(kBlock), 0, st, c, s);
-  hipLaunchKernelGGL((control_kernel<FPL>), dim3(s.tiles), dim3(kWave), 0, st, s, iterations, step);
-  return hipGetLastError();
-}
-
-template <class Fmt>
-static hipError_t launch_output(const DevCode& c, const DevState& s, const DevOut& out, int max_dv, hipStream_t st) {
-  dim3 grid((c.N + kWavesPerBlock - 1) / kWavesPerBlock, s.tiles);
-  (void)max_dv;
-  hipLaunchKernelGGL((output_kernel<Fmt>), grid, dim3(kBlock), 0, st, c, s, out);
+static hipError_t launch_syn_ctrl(const DevCode& c, const DevState& s, int iterations, int fused, int step,
+                                  hipStream_t st) {
+  if (!fused)
+    hipLaunchKernelGGL((syndrome_kernel<FPL>), dim3((c.M + kBlock - 1) / kBlock, s.tiles), dim3(kBlock), 0, st, c, s);
+  hipLaunchKernelGGL((control_kernel<FPL>), dim3(s.tiles), dim3(kWave), 0, st, s, iterations, fused, step);
   return hipGetLastError();
 }
 
@@ -810,22 +866,20 @@ hipError_t kernels_state_in(int fmt, const DevCode& c, const DevState& s, const 
   FECL_DISPATCH(fmt, launch_state_in<F>(c, s, state, dbl, frames, frame0, st));
 }
 hipError_t kernels_vn_init(int fmt, const DevCode& c, const DevState& s, int max_dv, hipStream_t st) {
-  FECL_DISPATCH(fmt, launch_vn<F>(c, s, max_dv, 1, st));
+  FECL_DISPATCH(fmt, launch_vn<F>(c, s, DevOut{}, max_dv, 0, 1, st));
 }
 hipError_t kernels_check(int fmt, const DevCode& c, const DevState& s, const DevOpts& o, int max_dc, hipStream_t st) {
   FECL_DISPATCH(fmt, launch_cn<F>(c, s, o, max_dc, st));
 }
-hipError_t kernels_variable(int fmt, const DevCode& c, const DevState& s, int max_dv, hipStream_t st) {
-  FECL_DISPATCH(fmt, launch_vn<F>(c, s, max_dv, 0, st));
+hipError_t kernels_variable(int fmt, const DevCode& c, const DevState& s, const DevOut& out, int max_dv,
+                            int fused, hipStream_t st) {
+  FECL_DISPATCH(fmt, launch_vn<F>(c, s, out, max_dv, fused, 0, st));
 }
-hipError_t kernels_syndrome_control(int fmt, const DevCode& c, const DevState& s, int iterations, int step,
-                                    hipStream_t st) {
-  if (fmt == 0) return launch_syn_ctrl<1>(c, s, iterations, step, st);
-  if (fmt == 1 || fmt == 2) return launch_syn_ctrl<2>(c, s, iterations, step, st);
+hipError_t kernels_syndrome_control(int fmt, const DevCode& c, const DevState& s, int iterations, int fused,
+                                    int step, hipStream_t st) {
+  if (fmt == 0) return launch_syn_ctrl<1>(c, s, iterations, fused, step, st);
+  if (fmt == 1 || fmt == 2) return launch_syn_ctrl<2>(c, s, iterations, fused, step, st);
   return hipErrorInvalidValue;
-}
-hipError_t kernels_output(int fmt, const DevCode& c, const DevState& s, const DevOut& out, int max_dv, hipStream_t st) {
-  FECL_DISPATCH(fmt, launch_output<F>(c, s, out, max_dv, st));
 }
 hipError_t kernels_results(int fmt, const DevCode& c, const DevState& s, const DevOut& out, int64_t frames,
                            int64_t frame0, const BatchResults& r, bool dbl, hipStream_t st) {

@@ -28,11 +28,10 @@ hipError_t kernels_state_in(int fmt, const DevCode& c, const DevState& s, const 
 hipError_t kernels_vn_init(int fmt, const DevCode& c, const DevState& s, int max_dv, hipStream_t st);
 hipError_t kernels_check(int fmt, const DevCode& c, const DevState& s, const DevOpts& o, int max_dc,
                          hipStream_t st);
-hipError_t kernels_variable(int fmt, const DevCode& c, const DevState& s, int max_dv, hipStream_t st);
-hipError_t kernels_syndrome_control(int fmt, const DevCode& c, const DevState& s, int iterations, int step,
-                                    hipStream_t st);
-hipError_t kernels_output(int fmt, const DevCode& c, const DevState& s, const DevOut& out, int max_dv,
-                          hipStream_t st);
+hipError_t kernels_variable(int fmt, const DevCode& c, const DevState& s, const DevOut& out, int max_dv,
+                            int fused, hipStream_t st);
+hipError_t kernels_syndrome_control(int fmt, const DevCode& c, const DevState& s, int iterations, int fused,
+                                    int step, hipStream_t st);
 hipError_t kernels_results(int fmt, const DevCode& c, const DevState& s, const DevOut& out, int64_t frames,
                            int64_t frame0, const BatchResults& r, bool dbl, hipStream_t st);
 

@@ -46,11 +46,23 @@ static inline REAL FN(sum_min)(REAL a, REAL b) {
   return (REAL)(-m);
 }
 
+/* Exact box-plus evaluated in the LLR domain (not one of FeCl's algorithms: the same
+ * mathematical value as BoxSum without tanh's saturation, used to measure conditioning):
+ * -s_a s_b min(|a|,|b|) - log1p(e^-|a+b|) + log1p(e^-|a-b|). */
+static inline REAL FN(sum_stable)(REAL a, REAL b) {
+  REAL m = FN(rmin)((REAL)fabs(a), (REAL)fabs(b));
+  REAL f1 = (REAL)log1p(exp(-(double)fabs(a + b)));
+  REAL f2 = (REAL)log1p(exp(-(double)fabs(a - b)));
+  REAL t = ((signbit(a) != 0) ^ (signbit(b) != 0)) ? m - f1 : -m - f1;
+  return t + f2;
+}
+
 static inline REAL FN(prior)(int alg, REAL x) { return alg == ORC_EXACT ? FN(prior_exact)(x) : x; }
 static inline REAL FN(post)(int alg, REAL x) { return alg == ORC_EXACT ? FN(post_exact)(x) : x; }
 static inline REAL FN(sum)(int alg, REAL a, REAL b) {
   if (alg == ORC_EXACT) return a * b;
   if (alg == ORC_LINEAR) return FN(sum_linear)(a, b);
+  if (alg == ORC_EXACT_STABLE) return FN(sum_stable)(a, b);
   return FN(sum_min)(a, b);
 }
 

@@ -135,7 +135,7 @@ int orc_bp_decode(const int32_t* row_ptr, const int32_t* col, int64_t M, int64_t
                   const double* syst_in, const double* state_in,
                   uint8_t* hard, double* msg, double* parity, double* syst, double* state,
                   int32_t* iters, int nthreads) {
-  if (iterations < 1 || frames < 0 || sf_rows < 1 || alg < 0 || alg > 2) return -1;
+  if (iterations < 1 || frames < 0 || sf_rows < 1 || alg < 0 || alg > 3) return -1;
   if (prec != 32 && prec != 64) return -1;
   if (prec == 64 && msg_fmt != ORC_MSG_F32) return -1;
   for (int64_t r = 0; r < M; ++r) {

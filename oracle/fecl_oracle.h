@@ -28,7 +28,8 @@ extern "C" {
 #endif
 
 /* Algorithms (src/DecoderAlgorithm.h:30-34). */
-enum { ORC_EXACT = 0, ORC_LINEAR = 1, ORC_APPROXIMATE = 2 };
+enum { ORC_EXACT = 0, ORC_LINEAR = 1, ORC_APPROXIMATE = 2,
+       ORC_EXACT_STABLE = 3 /* BoxSum's value in the LLR domain, for conditioning checks */ };
 /* Stored message formats for the FP32 restatement (precision form). */
 enum { ORC_MSG_F32 = 0, ORC_MSG_F16 = 1, ORC_MSG_BF16 = 2 };
 

@@ -11,7 +11,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
 
-EXACT, LINEAR, APPROXIMATE = 0, 1, 2
+EXACT, LINEAR, APPROXIMATE, EXACT_STABLE = 0, 1, 2, 3
 MSG_F32, MSG_F16, MSG_BF16 = 0, 1, 2
 
 

@@ -110,6 +110,13 @@ def test_16bit_messages_bit_exact(fmt, ofmt, alg):
 
 @pytest.mark.parametrize("code", ["gallager", "dvbs2"])
 def test_exact_fp32_against_fp64(code):
+    """Exact: FeCl evaluates BoxSum as -log((1+p)/(1-p)) with p a product of tanh(-x/2)
+    (src/detail/LlrMetrics.h:126-140).  In FP64 that form loses relative accuracy as |p| -> 1
+    (1-p cancels), so FeCl's own APP deviates from the exact box-plus value by up to ~4e-4 at
+    |APP| ~ 60.  The GPU evaluates the same value stably in FP32.  Bars:
+      * hard decisions and iteration counts identical to FeCl FP64;
+      * APP within 2e-6 relative of the exact value (the oracle's FP64 stable evaluation);
+      * APP within 1e-4 relative of FeCl FP64 plus FeCl's own distance from the exact value."""
     oc, H = gallager() if code == "gallager" else dvbs2()
     if code == "gallager":
         llr = gallager_llr(oc.N, oc.K, 4, 2.0, 11)
@@ -117,15 +124,18 @@ def test_exact_fp32_against_fp64(code):
         llr = dvbs2_llr(3, 1.0, seed=21)
     llr = llr.astype(np.float32).astype(np.float64)
     ref = oracle_run(oc, llr, F.Exact, 50, prec=64)
+    exact = oracle_run(oc, llr, O.EXACT_STABLE, 50, prec=64)
     out = gpu_decoder(H, F.Exact, 50).soDecode(llr)
     hard = gpu_decoder(H, F.Exact, 50).decode(llr)
     assert np.array_equal(hard.reshape(-1, oc.K), ref["hard"])
     assert np.array_equal(out["iterations"], ref["iters"])
-    r = ref["msg"].ravel()
+    r, x = ref["msg"].ravel(), exact["msg"].ravel()
     g = out["msg"].astype(np.float64)
+    scale = np.maximum(1.0, np.abs(x))
+    assert (np.abs(g - x) / scale).max() <= 2e-6
     fin = np.isfinite(r)
-    err = np.abs(g[fin] - r[fin]) / np.maximum(1.0, np.abs(r[fin]))
-    assert err.max() <= 1e-4, err.max()
+    own = np.abs(r[fin] - x[fin])
+    assert np.all(np.abs(g[fin] - r[fin]) <= own + 1e-4 * np.maximum(1.0, np.abs(r[fin])))
     assert np.all(np.sign(g[~fin]) == np.sign(r[~fin]))
 
 

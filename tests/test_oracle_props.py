@@ -61,6 +61,24 @@ def test_exact_check_matches_high_precision_boxplus(d):
             assert abs(y[j] - ref) <= 1e-9 * max(1.0, abs(ref)), (x, j, y[j], ref)
 
 
+@pytest.mark.parametrize("d", [3, 5, 7])
+def test_exact_stable_check_matches_high_precision_boxplus(d):
+    """The LLR-domain evaluation of BoxSum (the exact value FeCl's tanh form approximates)."""
+    mp = pytest.importorskip("mpmath")
+    mp.mp.dps = 50
+    rng = np.random.default_rng(200 + d)
+    for _ in range(60):
+        x = rng.normal(0, 8, d)
+        y = cn(x, O.EXACT_STABLE)
+        for j in range(d):
+            p = mp.mpf(1)
+            for i in range(d):
+                if i != j:
+                    p *= mp.tanh(-mp.mpf(x[i]) / 2)
+            ref = float(-2 * mp.atanh(p))
+            assert abs(y[j] - ref) <= 1e-12 * max(1.0, abs(ref)), (x, j, y[j], ref)
+
+
 def test_linear_pairwise_is_table_form():
     """Degree 3 row: out_0 = x_2 # x_1 with # = LinearBoxSum; f is the 8-knot interpolant of
     log(1+e^-x) at x = i/2, zero from x = 3.5 (src/detail/LinearTable.h:57-81)."""
