@@ -107,6 +107,8 @@ def main():
     ap.add_argument("--cpu-frames", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--words", type=int, default=-1,
+                    help="lane layout: 1, 2, 4 words per lane per edge; 0 automatic; -1 autotune")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the decoder's frames are independent, so N GPUs would "
@@ -127,6 +129,14 @@ def main():
     H = F.DvbS2.matrix(64800, 0.5)
     dec = F.Ldpc(H, F.DecoderOptions().algorithm(alg).iterations(args.iterations).msgFormat(fmt))
     K, N, E = dec.msgSize(), dec.paritySize(), dec.stateSize()
+    tune = None
+    if args.words >= 0:
+        dec.setLayout(args.words)
+    else:
+        best, times = dec.autotune(args.frames, probe_iterations=5)
+        tune = {"picked_words": best, "probe_ms": {str(k): round(v, 3) for k, v in times.items()}}
+        print(f"[bench] autotune: {tune}", file=sys.stderr, flush=True)
+    words, frames_per_tile = dec.layout()
     print(f"[bench] building {len(EBN0)} batches of {args.frames} frames", file=sys.stderr, flush=True)
     batches = make_batches(F, dec, H, args.frames, args.seed, device)
     torch.cuda.synchronize()
@@ -205,7 +215,8 @@ def main():
                                f"{args.iterations} max iterations with syndrome early exit, "
                                f"Eb/N0 {list(EBN0)} dB",
                    "frames_per_step": args.frames, "iterations": args.iterations, "algorithm": args.alg,
-                   "msg_format": args.msg, "code": "dvbs2 64800 1/2"},
+                   "msg_format": args.msg, "code": "dvbs2 64800 1/2",
+                   "lane_words": words, "frames_per_tile": frames_per_tile, "autotune": tune},
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
                        "p99": round(float(np.percentile(lat, 99)) * 1e3, 3), "unit": "per batch decode"},
         "fer": round(errs / max(1, nfr), 5),

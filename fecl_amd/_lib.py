@@ -28,6 +28,9 @@ SIGNATURES = {
     "fecl_ldpc_sodecode_device": [P, P, P, P, I64, P, P, P, P, P, P],
     "fecl_ldpc_profile": [P, I32],
     "fecl_ldpc_profile_read": [P, P, P],
+    "fecl_ldpc_set_layout": [P, I32],
+    "fecl_ldpc_get_layout": [P, P, P],
+    "fecl_ldpc_autotune": [P, I64, I32, P, P],
     "fecl_ldpc_trim": [P],
 }
 

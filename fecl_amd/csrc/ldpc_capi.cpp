@@ -99,6 +99,8 @@ struct fecl_ldpc {
   std::vector<double> sf;
   int32_t sf_rows = 1, sf_cols = 1;
   int32_t sf_mode = 0;
+  int32_t words = 1;        // device layout: 32-bit words per lane per edge (1, 2 or 4)
+  int32_t words_pinned = 0; // set by fecl_ldpc_set_layout / autotune; 0 = automatic
   std::mutex mu;
   std::vector<std::unique_ptr<Workspace>> pool;
   // kernel timing (fecl_ldpc_profile): check, variable, syndrome+control
@@ -115,6 +117,30 @@ namespace {
 
 // Frames decoded per batch: bounded so a workspace stays a few GiB at most.
 constexpr int64_t kMaxChunkFrames = 8192;
+
+// Widest lane layout the kernels allow for this code and options (unless one was pinned).
+void choose_layout(fecl_ldpc* h) {
+  const int cand[3] = {4, 2, 1};
+  if (h->words_pinned) {
+    Layout lay;
+    lay.fmt = h->msg_format;
+    lay.words = h->words_pinned;
+    if (layout_fits(lay, h->algorithm, h->code.max_row_degree, h->code.max_var_degree)) {
+      h->words = h->words_pinned;
+      return;
+    }
+  }
+  for (int w : cand) {
+    Layout lay;
+    lay.fmt = h->msg_format;
+    lay.words = w;
+    if (layout_fits(lay, h->algorithm, h->code.max_row_degree, h->code.max_var_degree)) {
+      h->words = w;
+      return;
+    }
+  }
+  h->words = 1;
+}
 
 int set_options(fecl_ldpc* h, const fecl_decoder_options* o) {
   if (!o) return fail(FECL_INVALID_ARGUMENT, "null options");
@@ -156,6 +182,7 @@ int set_options(fecl_ldpc* h, const fecl_decoder_options* o) {
   h->sf_rows = rows;
   h->sf_cols = cols;
   h->sf_mode = mode;
+  choose_layout(h);
   return FECL_OK;
 }
 
@@ -213,14 +240,20 @@ struct BatchIO {
   bool dbl_out;
 };
 
-// Decodes frames [frame0, frame0 + frames) of io on stream st with workspace ws.
-int run_batch(fecl_ldpc* h, Workspace* ws, hipStream_t st, const BatchIO& io, int64_t frames, int64_t frame0) {
-  const int fmt = h->msg_format;
-  const int FPL = frames_per_lane(fmt), TW = tile_width(fmt);
+// Device layout of the handle's current options.
+Layout layout_of(const fecl_ldpc* h) {
+  Layout lay;
+  lay.fmt = h->msg_format;
+  lay.words = h->words;
+  return lay;
+}
+
+// Sizes the workspace for `frames` frames and returns the device views of it.
+int prepare(fecl_ldpc* h, Workspace* ws, const Layout& lay, int64_t frames, bool soft, DevState* sp, DevOut* op) {
   const Code& c = h->code;
+  const int FPL = lay.fpl(), TW = lay.tw();
   const int64_t tiles = (frames + TW - 1) / TW;
-  const bool soft = io.res.msg || io.res.syst || io.res.parity;
-  HIP_TRY(ws->msg.ensure((size_t)tiles * c.E * 64 * 4));
+  HIP_TRY(ws->msg.ensure((size_t)tiles * c.E * lay.rw() * 4));
   HIP_TRY(ws->L.ensure((size_t)tiles * c.N * TW * 4));
   HIP_TRY(ws->hard.ensure((size_t)tiles * c.N * FPL * 8));
   HIP_TRY(ws->phase.ensure((size_t)tiles * TW));
@@ -233,13 +266,8 @@ int run_batch(fecl_ldpc* h, Workspace* ws, hipStream_t st, const BatchIO& io, in
     HIP_TRY(ws->out_ext.ensure((size_t)tiles * c.N * TW * 4));
     HIP_TRY(ws->out_app.ensure((size_t)tiles * c.K * TW * 4));
   }
-  if (ws->host_active_cap < h->iterations) {
-    if (ws->host_active) (void)hipHostFree(ws->host_active);
-    ws->host_active = nullptr;
-    HIP_TRY(hipHostMalloc((void**)&ws->host_active, (size_t)h->iterations * 4, hipHostMallocDefault));
-    ws->host_active_cap = h->iterations;
-  }
-  DevState s{};
+  DevState& s = *sp;
+  s = DevState{};
   s.msg = ws->msg.as<uint32_t>();
   s.L = ws->L.as<float>();
   s.hard = ws->hard.as<uint64_t>();
@@ -249,12 +277,33 @@ int run_batch(fecl_ldpc* h, Workspace* ws, hipStream_t st, const BatchIO& io, in
   s.tile_flags = ws->flags.as<uint32_t>();
   s.active = ws->active.as<int32_t>();
   s.tiles = (int32_t)tiles;
-  DevOut out{};
+  DevOut& out = *op;
+  out = DevOut{};
   out.hard = ws->out_hard.as<uint64_t>();
   out.ext = soft ? ws->out_ext.as<float>() : nullptr;
   out.app = soft ? ws->out_app.as<float>() : nullptr;
+  return FECL_OK;
+}
+
+// Decodes frames [frame0, frame0 + frames) of io on stream st with workspace ws.
+// capture: the launches are being recorded into a hipGraph, so every step is enqueued (frames
+// that finish early make the remaining steps cheap no-ops) and nothing waits on the host.
+int run_batch(fecl_ldpc* h, Workspace* ws, hipStream_t st, const BatchIO& io, int64_t frames, int64_t frame0,
+              bool capture = false) {
+  const Layout lay = layout_of(h);
+  const Code& c = h->code;
+  const bool soft = io.res.msg || io.res.syst || io.res.parity;
+  DevState s;
+  DevOut out;
+  if (int rc = prepare(h, ws, lay, frames, soft, &s, &out)) return rc;
+  if (!capture && ws->host_active_cap < h->iterations) {
+    if (ws->host_active) (void)hipHostFree(ws->host_active);
+    ws->host_active = nullptr;
+    HIP_TRY(hipHostMalloc((void**)&ws->host_active, (size_t)h->iterations * 4, hipHostMallocDefault));
+    ws->host_active_cap = h->iterations;
+  }
   const DevOpts o = make_opts(h);
-  const bool prof = h->profiling;
+  const bool prof = h->profiling && !capture;
   if (prof) {
     while ((int)ws->pev.size() < h->iterations) {
       std::array<hipEvent_t, 4> ev{};
@@ -266,20 +315,21 @@ int run_batch(fecl_ldpc* h, Workspace* ws, hipStream_t st, const BatchIO& io, in
 
   HIP_TRY(kernels_fill_i32(s.active, h->iterations, 0, st));
   uint8_t init = (uint8_t)((h->iterations == 1 ? PH_FINAL : PH_RUN) | (io.state ? PH_VNINIT : PH_FRESH));
-  HIP_TRY(kernels_ingest(fmt, h->dc, s, io.llr, io.syst, io.dbl_in, frames, frame0, init, st));
+  HIP_TRY(kernels_ingest(lay, h->dc, s, io.llr, io.syst, io.dbl_in, frames, frame0, init, st));
   if (io.state) {
-    HIP_TRY(kernels_state_in(fmt, h->dc, s, io.state, io.dbl_in, frames, frame0, st));
-    HIP_TRY(kernels_vn_init(fmt, h->dc, s, c.max_var_degree, st));
+    HIP_TRY(kernels_state_in(lay, h->dc, s, io.state, io.dbl_in, frames, frame0, st));
+    HIP_TRY(kernels_vn_init(lay, h->dc, s, c.max_var_degree, st));
   }
   for (int step = 0; step < h->iterations; ++step) {
     if (prof) HIP_TRY(hipEventRecord(ws->pev[step][0], st));
-    HIP_TRY(kernels_check(fmt, h->dc, s, o, c.max_row_degree, st));
+    HIP_TRY(kernels_check(lay, h->dc, s, o, c.max_row_degree, st));
     if (prof) HIP_TRY(hipEventRecord(ws->pev[step][1], st));
-    HIP_TRY(kernels_variable(fmt, h->dc, s, out, c.max_var_degree, o.fused_syn, st));
+    HIP_TRY(kernels_variable(lay, h->dc, s, out, c.max_var_degree, o.fused_syn, st));
     if (prof) HIP_TRY(hipEventRecord(ws->pev[step][2], st));
-    HIP_TRY(kernels_syndrome_control(fmt, h->dc, s, h->iterations, o.fused_syn, step, st));
+    HIP_TRY(kernels_syndrome_control(lay, h->dc, s, h->iterations, o.fused_syn, step, st));
     if (prof) HIP_TRY(hipEventRecord(ws->pev[step][3], st));
     steps_run = step + 1;
+    if (capture) continue;
     HIP_TRY(hipMemcpyAsync(&ws->host_active[step], &s.active[step], 4, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipEventRecord(ws->ev[step % 4], st));
     // Stop enqueuing once every frame of the batch has finished: the count of step-2 is read
@@ -289,7 +339,7 @@ int run_batch(fecl_ldpc* h, Workspace* ws, hipStream_t st, const BatchIO& io, in
       if (ws->host_active[step - 2] == 0) break;
     }
   }
-  HIP_TRY(kernels_results(fmt, h->dc, s, out, frames, frame0, io.res, io.dbl_out, st));
+  HIP_TRY(kernels_results(lay, h->dc, s, out, frames, frame0, io.res, io.dbl_out, st));
   if (prof) {
     // kernel durations from events on the launch stream: check, variable, syndrome+control
     HIP_TRY(hipStreamSynchronize(st));
@@ -573,6 +623,95 @@ int fecl_ldpc_profile_read(fecl_ldpc* h, double* ms, int64_t* launches) {
     ms[k] = h->prof_ms[k];
     launches[k] = h->prof_launches[k];
   }
+  return FECL_OK;
+}
+
+int fecl_ldpc_set_layout(fecl_ldpc* h, int32_t words) {
+  if (int rc = check_handle(h)) return rc;
+  if (words != 0 && words != 1 && words != 2 && words != 4) return fail(FECL_INVALID_ARGUMENT, "words must be 0, 1, 2 or 4");
+  if (words) {
+    Layout lay;
+    lay.fmt = h->msg_format;
+    lay.words = words;
+    if (!layout_fits(lay, h->algorithm, h->code.max_row_degree, h->code.max_var_degree))
+      return fail(FECL_NOT_SUPPORTED, "layout exceeds the register budget for this code's degrees");
+  }
+  std::lock_guard<std::mutex> lk(h->mu);
+  h->words_pinned = words;
+  choose_layout(h);
+  return FECL_OK;
+}
+
+int fecl_ldpc_get_layout(const fecl_ldpc* h, int32_t* words, int32_t* frames_per_tile) {
+  if (int rc = check_handle(h)) return rc;
+  const Layout lay = layout_of(h);
+  if (words) *words = lay.words;
+  if (frames_per_tile) *frames_per_tile = lay.tw();
+  return FECL_OK;
+}
+
+int fecl_ldpc_autotune(fecl_ldpc* h, int64_t frames, int32_t probe_iterations, int32_t* best_words, double* ms) {
+  if (int rc = check_handle(h)) return rc;
+  if (frames < 1 || probe_iterations < 1) return fail(FECL_INVALID_ARGUMENT, "frames and probe_iterations must be >= 1");
+  const Code& c = h->code;
+  HIP_TRY(hipSetDevice(h->device));
+  // Noise-only LLRs: no frame converges, so every probe iteration runs for every frame.
+  std::vector<float> host((size_t)frames * c.N);
+  uint64_t x = 0x9E3779B97F4A7C15ull;
+  for (auto& v : host) {
+    x ^= x << 13; x ^= x >> 7; x ^= x << 17;
+    v = (float)((int64_t)(x >> 40) - (1 << 23)) / (float)(1 << 22);
+  }
+  DevBuf llr, bits;
+  HIP_TRY(llr.ensure(host.size() * 4));
+  HIP_TRY(bits.ensure((size_t)frames * c.K));
+  HIP_TRY(hipMemcpy(llr.p, host.data(), host.size() * 4, hipMemcpyHostToDevice));
+  const int32_t saved_iters = h->iterations, saved_pin = h->words_pinned;
+  const int32_t saved_words = h->words;
+  const bool saved_prof = h->profiling;
+  h->iterations = probe_iterations;
+  h->profiling = false;
+  hipStream_t st = nullptr;
+  HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  double best = 1e30;
+  int32_t best_w = saved_words;
+  const int cand[3] = {1, 2, 4};
+  int rc = FECL_OK;
+  for (int i = 0; i < 3 && rc == FECL_OK; ++i) {
+    Layout lay;
+    lay.fmt = h->msg_format;
+    lay.words = cand[i];
+    if (ms) ms[i] = -1.0;
+    if (!layout_fits(lay, h->algorithm, c.max_row_degree, c.max_var_degree)) continue;
+    h->words = cand[i];
+    float t = 0.0f;
+    for (int rep = 0; rep < 2 && rc == FECL_OK; ++rep) {  // first run sizes the workspace
+      hipEventRecord(e0, st);
+      rc = device_decode(h, llr.as<float>(), nullptr, nullptr, frames, bits.as<uint8_t>(), nullptr, nullptr,
+                         nullptr, nullptr, nullptr, st);
+      hipEventRecord(e1, st);
+      hipEventSynchronize(e1);
+      hipEventElapsedTime(&t, e0, e1);
+    }
+    if (ms) ms[i] = t;
+    if (rc == FECL_OK && t < best) { best = t; best_w = cand[i]; }
+  }
+  h->iterations = saved_iters;
+  h->profiling = saved_prof;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipStreamDestroy(st);
+  if (rc != FECL_OK) {
+    h->words = saved_words;
+    h->words_pinned = saved_pin;
+    return rc;
+  }
+  h->words_pinned = best_w;
+  choose_layout(h);
+  if (best_words) *best_words = h->words;
   return FECL_OK;
 }
 

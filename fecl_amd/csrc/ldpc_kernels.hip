@@ -10,17 +10,22 @@
 //   MinBoxSum / LinearBoxSum / BoxSum   src/detail/LlrMetrics.h:126-196
 //   linear table     src/detail/LinearTable.h:57-81
 //   syndrome test    src/detail/Ldpc.cpp:160-172
+//
+// Message rows.  A (tile, edge) row holds the edge's message for the TW frames of the tile,
+// packed PW per 32-bit word (PW = 1 for FP32, 2 for FP16/BF16): RW = TW / PW words.  Each lane
+// owns W consecutive words (FPL = W * PW frames), so a wavefront moves 256*W contiguous bytes
+// per edge; W is the layout knob the autotuner picks (frames per workgroup = 4 * 64 * FPL).
 #include "ldpc_kernels.h"
 #include "ldpc_launch.h"
 
 namespace fecl {
 
 // ------------------------------------------------------------------------------------------
-// Message formats.  A lane holds FPL frames packed in one 32-bit word.
-struct FmtF32 {
-  static constexpr int FPL = 1;
-  __device__ static inline void unpack(uint32_t w, float (&x)[1]) { x[0] = __uint_as_float(w); }
-  __device__ static inline uint32_t pack(const float (&x)[1]) { return __float_as_uint(x[0]); }
+// Element formats: PW messages per 32-bit word.
+struct ElemF32 {
+  static constexpr int PW = 1;
+  __device__ static inline void unpack(uint32_t w, float* x) { x[0] = __uint_as_float(w); }
+  __device__ static inline uint32_t pack(const float* x) { return __float_as_uint(x[0]); }
   __device__ static inline float quantize(float x) { return x; }
 };
 
@@ -34,13 +39,13 @@ __device__ static inline uint32_t f32_to_f16_bits(float x) {
 __device__ static inline float f16_bits_to_f32(uint32_t b) {
   return (float)__builtin_bit_cast(_Float16, (uint16_t)b);
 }
-struct FmtF16 {
-  static constexpr int FPL = 2;
-  __device__ static inline void unpack(uint32_t w, float (&x)[2]) {
+struct ElemF16 {
+  static constexpr int PW = 2;
+  __device__ static inline void unpack(uint32_t w, float* x) {
     x[0] = f16_bits_to_f32(w & 0xffffu);
     x[1] = f16_bits_to_f32(w >> 16);
   }
-  __device__ static inline uint32_t pack(const float (&x)[2]) {
+  __device__ static inline uint32_t pack(const float* x) {
     return f32_to_f16_bits(x[0]) | (f32_to_f16_bits(x[1]) << 16);
   }
   __device__ static inline float quantize(float x) { return f16_bits_to_f32(f32_to_f16_bits(x)); }
@@ -56,16 +61,74 @@ __device__ static inline uint32_t f32_to_bf16_bits(float x) {
   u += 0x7fffu + ((u >> 16) & 1u);
   return u >> 16;
 }
-struct FmtBF16 {
-  static constexpr int FPL = 2;
-  __device__ static inline void unpack(uint32_t w, float (&x)[2]) {
+struct ElemBF16 {
+  static constexpr int PW = 2;
+  __device__ static inline void unpack(uint32_t w, float* x) {
     x[0] = __uint_as_float(w << 16);
     x[1] = __uint_as_float(w & 0xffff0000u);
   }
-  __device__ static inline uint32_t pack(const float (&x)[2]) {
+  __device__ static inline uint32_t pack(const float* x) {
     return f32_to_bf16_bits(x[0]) | (f32_to_bf16_bits(x[1]) << 16);
   }
   __device__ static inline float quantize(float x) { return __uint_as_float(f32_to_bf16_bits(x) << 16); }
+};
+
+template <int W>
+__device__ static inline void load_words(const uint32_t* p, uint32_t (&w)[W]) {
+  if constexpr (W == 1) {
+    w[0] = p[0];
+  } else if constexpr (W == 2) {
+    const uint2 v = *reinterpret_cast<const uint2*>(p);
+    w[0] = v.x; w[1] = v.y;
+  } else {
+    static_assert(W == 4, "W is 1, 2 or 4");
+    const uint4 v = *reinterpret_cast<const uint4*>(p);
+    w[0] = v.x; w[1] = v.y; w[2] = v.z; w[3] = v.w;
+  }
+}
+template <int W>
+__device__ static inline void store_words(uint32_t* p, const uint32_t (&w)[W]) {
+  if constexpr (W == 1) {
+    p[0] = w[0];
+  } else if constexpr (W == 2) {
+    *reinterpret_cast<uint2*>(p) = make_uint2(w[0], w[1]);
+  } else {
+    *reinterpret_cast<uint4*>(p) = make_uint4(w[0], w[1], w[2], w[3]);
+  }
+}
+template <int N>
+__device__ static inline void load_floats(const float* p, float (&x)[N]) {
+  if constexpr (N == 1) {
+    x[0] = p[0];
+  } else if constexpr (N == 2) {
+    const float2 v = *reinterpret_cast<const float2*>(p);
+    x[0] = v.x; x[1] = v.y;
+  } else {
+#pragma unroll
+    for (int i = 0; i < N; i += 4) {
+      const float4 v = *reinterpret_cast<const float4*>(p + i);
+      x[i] = v.x; x[i + 1] = v.y; x[i + 2] = v.z; x[i + 3] = v.w;
+    }
+  }
+}
+
+template <class E, int W_>
+struct Fmt {
+  using Elem = E;
+  static constexpr int W = W_, PW = E::PW, FPL = W_ * E::PW, TW = kWave * FPL, RW = kWave * W_;
+  __device__ static inline void load(const uint32_t* p, float (&x)[FPL]) {
+    uint32_t w[W];
+    load_words<W>(p, w);
+#pragma unroll
+    for (int i = 0; i < W; ++i) E::unpack(w[i], &x[i * PW]);
+  }
+  __device__ static inline void store(uint32_t* p, const float (&x)[FPL]) {
+    uint32_t w[W];
+#pragma unroll
+    for (int i = 0; i < W; ++i) w[i] = E::pack(&x[i * PW]);
+    store_words<W>(p, w);
+  }
+  __device__ static inline float quantize(float x) { return E::quantize(x); }
 };
 
 __device__ static inline int uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
@@ -123,6 +186,7 @@ __device__ static inline bool cn_lanes(const DevState& s, const DevOpts& o, int 
 // Fused syndrome (used when the scaling factors do not change with the iteration, so the
 // check update that follows a successful syndrome equals FeCl's final one): the row's parity
 // over the hard decisions of the previous variable update, or-ed into the tile's fail mask.
+// Only bits not yet set are sent as atomics, so a tile costs a few atomics per frame.
 template <int FPL>
 __device__ static inline void cn_syndrome(const DevCode& c, const DevState& s, int t, int e0, int d, int lane) {
   uint64_t syn[FPL];
@@ -136,8 +200,12 @@ __device__ static inline void cn_syndrome(const DevCode& c, const DevState& s, i
   }
   if (lane == 0) {
 #pragma unroll
-    for (int k = 0; k < FPL; ++k)
-      if (syn[k]) atomicOr((unsigned long long*)&s.fail[t * FPL + k], (unsigned long long)syn[k]);
+    for (int k = 0; k < FPL; ++k) {
+      if (!syn[k]) continue;
+      uint64_t* f = &s.fail[t * FPL + k];
+      const uint64_t seen = __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (syn[k] & ~seen) atomicOr((unsigned long long*)f, (unsigned long long)syn[k]);
+    }
   }
 }
 
@@ -148,9 +216,9 @@ __device__ static inline void cn_syndrome(const DevCode& c, const DevState& s, i
 // ((d-2) & 1) ^ xor_{i != j} signbit(x_i)  (a box-sum of two operands sets the sign bit
 // 1 ^ s_a ^ s_b).  One wavefront per (check row, tile).  DMAX > 0: the row's d words are
 // loaded into registers at once (d <= DMAX); DMAX == 0: two streaming passes (any d <= 64).
-template <class Fmt, int DMAX>
+template <class F, int DMAX>
 __global__ __launch_bounds__(kBlock) void cn_minsum_kernel(DevCode c, DevState s, DevOpts o) {
-  constexpr int FPL = Fmt::FPL, TW = kWave * FPL;
+  constexpr int FPL = F::FPL, TW = F::TW, RW = F::RW;
   const int t = blockIdx.y;
   if (!(s.tile_flags[t] & TF_LIVE)) return;
   const int r = uniform(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6));
@@ -161,7 +229,7 @@ __global__ __launch_bounds__(kBlock) void cn_minsum_kernel(DevCode c, DevState s
 
   const int e0 = uniform(c.row_ptr[r]);
   const int d = uniform(c.row_ptr[r + 1]) - e0;
-  uint32_t* mrow = s.msg + ((int64_t)t * c.E + e0) * kWave + lane;
+  uint32_t* mrow = s.msg + ((int64_t)t * c.E + e0) * RW + lane * F::W;
   const float* Lt = s.L + (int64_t)t * c.N * TW + lane * FPL;
 
   float m1[FPL], m2[FPL];
@@ -179,23 +247,25 @@ __global__ __launch_bounds__(kBlock) void cn_minsum_kernel(DevCode c, DevState s
     }
   };
   auto load_x = [&](int j, float (&x)[FPL]) {
-    Fmt::unpack(mrow[(int64_t)j * kWave], x);
+    F::load(mrow + (int64_t)j * RW, x);
     if (ln.wave_fresh) {
       const int v = uniform(c.col[e0 + j]);
+      float l[FPL];
+      load_floats<FPL>(Lt + (int64_t)v * TW, l);
 #pragma unroll
       for (int k = 0; k < FPL; ++k)
-        if (ln.fresh[k]) x[k] = Fmt::quantize(Lt[(int64_t)v * TW + k]);
+        if (ln.fresh[k]) x[k] = F::quantize(l[k]);
     }
   };
 
   float xs[DMAX > 0 ? DMAX : 1][FPL];
-  if (DMAX > 0) {
+  if constexpr (DMAX > 0) {
 #pragma unroll
     for (int j = 0; j < DMAX; ++j)
       if (j < d) load_x(j, xs[j]);
   }
   if (o.fused_syn && ln.wave_seen) cn_syndrome<FPL>(c, s, t, e0, d, lane);
-  if (DMAX > 0) {
+  if constexpr (DMAX > 0) {
 #pragma unroll
     for (int j = 0; j < DMAX; ++j)
       if (j < d) accumulate(j, xs[j]);
@@ -223,20 +293,16 @@ __global__ __launch_bounds__(kBlock) void cn_minsum_kernel(DevCode c, DevState s
       const uint32_t sb = par[k] ^ (uint32_t)((sg[k] >> j) & 1);
       y[k] = ln.act[k] ? (sb ? -mag : mag) : old[k];   // other frames keep their word
     }
-    mrow[(int64_t)j * kWave] = Fmt::pack(y);
+    F::store(mrow + (int64_t)j * RW, y);
   };
-  if (DMAX > 0) {
+  if constexpr (DMAX > 0) {
 #pragma unroll
     for (int j = 0; j < DMAX; ++j)
       if (j < d) emit(j, xs[j]);
   } else {
     for (int j = 0; j < d; ++j) {
       float old[FPL];
-      if (!ln.lane_all) Fmt::unpack(mrow[(int64_t)j * kWave], old);
-      else {
-#pragma unroll
-        for (int k = 0; k < FPL; ++k) old[k] = 0.0f;
-      }
+      F::load(mrow + (int64_t)j * RW, old);
       emit(j, old);
     }
   }
@@ -277,9 +343,9 @@ __device__ static inline float box_sum(const DevOpts& o, float a, float b) {
 // Check-node update for Linear (ALG 1) and Exact (ALG 2 here) in FeCl's forward/backward
 // order: prefix P_j = ((x_0 # x_1) # ...) # x_{j-1}; suffix from x_{d-1} downwards;
 // out_{d-1} = P_{d-1}, out_j = P_j # S_j, out_0 = S_0, each times the scaling factor.
-template <class Fmt, int DMAX, int ALG>
+template <class F, int DMAX, int ALG>
 __global__ __launch_bounds__(kBlock) void cn_box_kernel(DevCode c, DevState s, DevOpts o) {
-  constexpr int FPL = Fmt::FPL, TW = kWave * FPL;
+  constexpr int FPL = F::FPL, TW = F::TW, RW = F::RW;
   const int t = blockIdx.y;
   if (!(s.tile_flags[t] & TF_LIVE)) return;
   const int r = uniform(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6));
@@ -287,33 +353,29 @@ __global__ __launch_bounds__(kBlock) void cn_box_kernel(DevCode c, DevState s, D
   const int lane = threadIdx.x & 63;
   CnLanes<FPL> ln;
   if (!cn_lanes<FPL>(s, o, t, lane, ln)) return;
-  const bool* act = ln.act;
-  const bool* fresh = ln.fresh;
-  const int* sfrow = ln.sfrow;
-  const bool lane_any = ln.lane_any, lane_all = ln.lane_all, wave_fresh = ln.wave_fresh;
 
   const int e0 = uniform(c.row_ptr[r]);
   const int d = uniform(c.row_ptr[r + 1]) - e0;
-  uint32_t* mrow = s.msg + ((int64_t)t * c.E + e0) * kWave + lane;
+  uint32_t* mrow = s.msg + ((int64_t)t * c.E + e0) * RW + lane * F::W;
   const float* Lt = s.L + (int64_t)t * c.N * TW + lane * FPL;
   if (o.fused_syn && ln.wave_seen) cn_syndrome<FPL>(c, s, t, e0, d, lane);
 
   float x[DMAX][FPL];
-  uint32_t wold[DMAX];
 #pragma unroll
   for (int j = 0; j < DMAX; ++j) {
     if (j < d) {
-      wold[j] = mrow[(int64_t)j * kWave];
-      Fmt::unpack(wold[j], x[j]);
-      if (wave_fresh) {
+      F::load(mrow + (int64_t)j * RW, x[j]);
+      if (ln.wave_fresh) {
         const int v = uniform(c.col[e0 + j]);
+        float l[FPL];
+        load_floats<FPL>(Lt + (int64_t)v * TW, l);
 #pragma unroll
         for (int k = 0; k < FPL; ++k)
-          if (fresh[k]) x[j][k] = Fmt::quantize(Lt[(int64_t)v * TW + k]);
+          if (ln.fresh[k]) x[j][k] = F::quantize(l[k]);
       }
     }
   }
-  if (!lane_any) return;
+  if (!ln.lane_any) return;
   // forward prefixes: pre[j] for j = 1 .. d-2, P = prefix over 0 .. d-2
   float pre[DMAX][FPL];
   float P[FPL];
@@ -329,24 +391,20 @@ __global__ __launch_bounds__(kBlock) void cn_box_kernel(DevCode c, DevState s, D
   auto store = [&](int j, float (&y)[FPL]) {
 #pragma unroll
     for (int k = 0; k < FPL; ++k) {
-      y[k] = apply_sf(o, sfrow[k], d - 2, y[k]);
-      y[k] = Fmt::quantize(y[k]);
+      y[k] = F::quantize(apply_sf(o, ln.sfrow[k], d - 2, y[k]));
+      if (!ln.act[k]) y[k] = x[j][k];   // other frames keep their word
     }
-    if (!lane_all) {
-      float old[FPL];
-      Fmt::unpack(wold[j], old);
-#pragma unroll
-      for (int k = 0; k < FPL; ++k) if (!act[k]) y[k] = old[k];
-    }
-    mrow[(int64_t)j * kWave] = Fmt::pack(y);
+    F::store(mrow + (int64_t)j * RW, y);
   };
   // out_{d-1} = P
-  {
-    float y[FPL];
 #pragma unroll
-    for (int k = 0; k < FPL; ++k) y[k] = P[k];
+  for (int j = DMAX - 1; j >= 1; --j) {
+    if (j == d - 1) {
+      float y[FPL];
 #pragma unroll
-    for (int j = DMAX - 1; j >= 1; --j) if (j == d - 1) store(j, y);
+      for (int k = 0; k < FPL; ++k) y[k] = P[k];
+      store(j, y);
+    }
   }
   float S[FPL];
 #pragma unroll
@@ -386,10 +444,13 @@ __global__ __launch_bounds__(kBlock) void cn_box_kernel(DevCode c, DevState s, D
 //   soDecode : ext = 0 + c_1 + ... + c_dv, msg = L + ext
 // and leaves their messages (the final state) untouched.  init_pass builds v2c from a
 // supplied state (BpDecoderImpl.cpp:81-88) for frames flagged PH_VNINIT.
-template <class Fmt, int VMAX>
+template <class F, int VMAX>
 __global__ __launch_bounds__(kBlock) void vn_kernel(DevCode c, DevState s, DevOut out, int fused_syn, int init_pass) {
-  constexpr int FPL = Fmt::FPL, TW = kWave * FPL;
-  const int t = blockIdx.y;
+  constexpr int FPL = F::FPL, TW = F::TW, RW = F::RW;
+  // Tiles in the opposite order to the check kernel: workgroups are dispatched in grid order,
+  // so this pass starts on the tiles whose messages the check pass wrote last (still in the
+  // 256 MB Infinity Cache), and the next check pass starts where this one ended.
+  const int t = s.tiles - 1 - (int)blockIdx.y;
   if (!(s.tile_flags[t] & (init_pass ? TF_VNINIT : TF_LIVE))) return;
   const int v = uniform(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6));
   if (v >= c.N) return;
@@ -416,17 +477,15 @@ __global__ __launch_bounds__(kBlock) void vn_kernel(DevCode c, DevState s, DevOu
   const int s0 = uniform(c.var_ptr[v]);
   const int dv = uniform(c.var_ptr[v + 1]) - s0;
   float Lv[FPL];
-  const float* Lp = s.L + ((int64_t)t * c.N + v) * TW + lane * FPL;
-#pragma unroll
-  for (int k = 0; k < FPL; ++k) Lv[k] = Lp[k];
-  uint32_t* mt = s.msg + (int64_t)t * c.E * kWave + lane;
+  load_floats<FPL>(s.L + ((int64_t)t * c.N + v) * TW + lane * FPL, Lv);
+  uint32_t* mt = s.msg + (int64_t)t * c.E * RW + lane * F::W;
   float cv[VMAX][FPL];
   int eid[VMAX];
 #pragma unroll
   for (int j = 0; j < VMAX; ++j) {
     if (j < dv) {
       eid[j] = uniform(c.var_edges[s0 + j]);
-      Fmt::unpack(mt[(int64_t)eid[j] * kWave], cv[j]);
+      F::load(mt + (int64_t)eid[j] * RW, cv[j]);
     }
   }
   if (wave_fin) {
@@ -486,7 +545,7 @@ __global__ __launch_bounds__(kBlock) void vn_kernel(DevCode c, DevState s, DevOu
         acc[k] = acc[k] + cv[j][k];
         if (!run[k]) y[k] = cv[j][k];   // other frames keep their word (exact round trip)
       }
-      if (lane_run) mt[(int64_t)eid[j] * kWave] = Fmt::pack(y);
+      if (lane_run) F::store(mt + (int64_t)eid[j] * RW, y);
     }
   }
   if (init_pass) return;
@@ -498,8 +557,8 @@ __global__ __launch_bounds__(kBlock) void vn_kernel(DevCode c, DevState s, DevOu
 }
 
 // ------------------------------------------------------------------------------------------
-// Syndrome: one thread per (check row, tile) xors the hard words of its columns; the failing
-// frames of the tile are or-ed into fail[tile].
+// Syndrome (when not fused into the check kernel): one thread per (check row, tile) xors the
+// hard words of its columns; the failing frames of the tile are or-ed into fail[tile].
 template <int FPL>
 __global__ __launch_bounds__(kBlock) void syndrome_kernel(DevCode c, DevState s) {
   const int t = blockIdx.y;
@@ -528,13 +587,15 @@ __global__ __launch_bounds__(kBlock) void syndrome_kernel(DevCode c, DevState s)
 //   for t in 0 .. I-2: { CN(t); VN; if syndrome ok: break }  CN(I-1)
 // becomes: RUN frames count the update; they turn FINAL when the syndrome passes or the
 // I-1 regular updates are used up; FINAL frames (their last update used factor row I-1)
-// turn DONE.  The iteration count is the number of check-node updates executed.
+// turn DONE.  With the fused syndrome, the syndrome seen in a step is the one of the previous
+// variable update, so a passing RUN frame has just had its final update and turns DONE.
+// The iteration count is the number of check-node updates executed.
 template <int FPL>
 __global__ __launch_bounds__(kWave) void control_kernel(DevState s, int iterations, int fused_syn, int step) {
   constexpr int TW = kWave * FPL;
   const int t = blockIdx.x;
   const int lane = threadIdx.x;
-  bool any_cn = false, any_vn = false;
+  bool any_live = false, any_run = false;
   int alive = 0;
 #pragma unroll
   for (int k = 0; k < FPL; ++k) {
@@ -547,7 +608,7 @@ __global__ __launch_bounds__(kWave) void control_kernel(DevState s, int iteratio
       const int it = it0 + 1;
       s.iter[p] = it;
       const bool ok = ((fw >> lane) & 1ull) == 0;
-      if (fused_syn)  // the syndrome seen in this step is the previous update's
+      if (fused_syn)
         ph = (it0 >= 1 && ok) ? PH_DONE : (it >= iterations - 1 ? PH_FINAL : PH_RUN);
       else
         ph = (ok || it >= iterations - 1) ? PH_FINAL : PH_RUN;
@@ -558,33 +619,31 @@ __global__ __launch_bounds__(kWave) void control_kernel(DevState s, int iteratio
       ph = base;
     }
     s.phase[p] = ph;
-    any_cn |= ph == PH_RUN || ph == PH_FINAL;
-    any_vn |= ph == PH_RUN;
+    any_live |= ph == PH_RUN || ph == PH_FINAL;
+    any_run |= ph == PH_RUN;
     alive += (ph == PH_RUN || ph == PH_FINAL) ? 1 : 0;
   }
-  const uint64_t bcn = __ballot(any_cn), bvn = __ballot(any_vn);
+  const uint64_t blive = __ballot(any_live), brun = __ballot(any_run);
 #pragma unroll
   for (int o = 32; o >= 1; o >>= 1) alive += __shfl_xor(alive, o, 64);
   if (lane == 0) {
 #pragma unroll
     for (int k = 0; k < FPL; ++k) s.fail[t * FPL + k] = 0;
-    s.tile_flags[t] = (bcn ? TF_LIVE : 0u) | (bvn ? TF_RUN : 0u);
+    s.tile_flags[t] = (blive ? TF_LIVE : 0u) | (brun ? TF_RUN : 0u);
     if (alive) atomicAdd(&s.active[step], alive);
   }
 }
 
 // ------------------------------------------------------------------------------------------
-// Batch set-up: LLR [F][N] (row stride ld) -> L tiles, optional a-priori on the first K bits
+// Batch set-up: LLR [F][N] (row stride N) -> L tiles, optional a-priori on the first K bits
 // (FeCl: parity_[i] += syst[i] in the LLR type), per-frame phase/iteration.
-template <typename In, int FPL>
+template <typename In, int TW>
 __global__ __launch_bounds__(kBlock) void ingest_kernel(DevCode c, DevState s, const In* llr,
                                                         const In* syst, int64_t frames,
-                                                        int64_t frame0, uint8_t init_phase) {
-  constexpr int TW = kWave * FPL;
+                                                        int64_t frame0, uint8_t init_phase, int fpl) {
   __shared__ float tile[64][65];
   const int t = blockIdx.y;
   const int v0 = blockIdx.x * 64;
-  // 64 variables x TW frames, in chunks of 64 frames
   for (int fc = 0; fc < TW; fc += 64) {
     for (int i = threadIdx.x; i < 64 * 64; i += kBlock) {
       const int fl = i / 64, vl = i % 64;                 // read: frame-major, coalesced in v
@@ -611,7 +670,7 @@ __global__ __launch_bounds__(kBlock) void ingest_kernel(DevCode c, DevState s, c
       s.phase[f] = f < frames ? init_phase : (uint8_t)PH_FREE;
       s.iter[f] = 0;
     }
-    if (threadIdx.x < FPL) s.fail[t * FPL + threadIdx.x] = 0;
+    if (threadIdx.x < fpl) s.fail[t * fpl + threadIdx.x] = 0;
     if (threadIdx.x == 0) {
       const int64_t live = frames - (int64_t)t * TW;
       uint32_t fl = 0;
@@ -625,11 +684,12 @@ __global__ __launch_bounds__(kBlock) void ingest_kernel(DevCode c, DevState s, c
   }
 }
 
-// State [F][E] (check-to-variable, FeCl's edge order) -> msg tiles, rounded to the format.
-template <typename In, class Fmt>
+// State [F][E] (check-to-variable, FeCl's edge order) -> message rows, rounded to the format.
+// Element p of a (tile, edge) row sits in word p / PW, half p % PW.
+template <typename In, class F>
 __global__ __launch_bounds__(kBlock) void state_in_kernel(DevCode c, DevState s, const In* state,
                                                           int64_t frames, int64_t frame0) {
-  constexpr int FPL = Fmt::FPL, TW = kWave * FPL;
+  constexpr int TW = F::TW, RW = F::RW, PW = F::PW;
   __shared__ float tile[64][65];
   const int t = blockIdx.y;
   const int e0 = blockIdx.x * 64;
@@ -641,14 +701,13 @@ __global__ __launch_bounds__(kBlock) void state_in_kernel(DevCode c, DevState s,
       tile[fl][el] = (f < frames && e < c.E) ? (float)state[(frame0 + f) * c.E + e] : 0.0f;
     }
     __syncthreads();
-    // each thread packs one word: edge el, lane (fc + fl) / FPL
-    for (int i = threadIdx.x; i < 64 * (64 / FPL); i += kBlock) {
-      const int el = i / (64 / FPL), ll = i % (64 / FPL);
+    for (int i = threadIdx.x; i < 64 * (64 / PW); i += kBlock) {
+      const int el = i / (64 / PW), wl = i % (64 / PW);
       const int e = e0 + el;
-      float x[FPL];
+      float x[PW];
 #pragma unroll
-      for (int k = 0; k < FPL; ++k) x[k] = tile[ll * FPL + k][el];
-      if (e < c.E) s.msg[((int64_t)t * c.E + e) * kWave + fc / FPL + ll] = Fmt::pack(x);
+      for (int h = 0; h < PW; ++h) x[h] = tile[wl * PW + h][el];
+      if (e < c.E) s.msg[((int64_t)t * c.E + e) * RW + fc / PW + wl] = F::Elem::pack(x);
     }
     __syncthreads();
   }
@@ -668,12 +727,10 @@ __global__ __launch_bounds__(kBlock) void unpack_hard_kernel(const uint64_t* har
   msg[(frame0 + f) * K + v] = (uint8_t)((w >> (p / FPL)) & 1ull);
 }
 
-// [tile][X][TW] float -> [F][X] (Out = float or double), through LDS.
-// Copies x < count of each frame; the output row stride is `count`.
-template <typename Out, int FPL>
+// [tile][X][TW] float -> [F][count] (Out = float or double), x < count, through LDS.
+template <typename Out, int TW>
 __global__ __launch_bounds__(kBlock) void tiles_to_frames_kernel(const float* in, int64_t X, int64_t count,
                                                                  int64_t frames, Out* out, int64_t frame0) {
-  constexpr int TW = kWave * FPL;
   __shared__ float tile[64][65];
   const int t = blockIdx.y;
   const int64_t x0 = (int64_t)blockIdx.x * 64;
@@ -694,26 +751,26 @@ __global__ __launch_bounds__(kBlock) void tiles_to_frames_kernel(const float* in
   }
 }
 
-// msg tiles (formatted words) -> state [F][E] (Out = float or double).
-template <typename Out, class Fmt>
+// Message rows -> state [F][E] (Out = float or double).
+template <typename Out, class F>
 __global__ __launch_bounds__(kBlock) void state_out_kernel(DevCode c, DevState s, int64_t frames,
                                                            Out* out, int64_t frame0) {
-  constexpr int FPL = Fmt::FPL, TW = kWave * FPL;
+  constexpr int TW = F::TW, RW = F::RW, PW = F::PW;
   __shared__ float tile[64][65];
   const int t = blockIdx.y;
   const int e0 = blockIdx.x * 64;
   for (int fc = 0; fc < TW; fc += 64) {
-    for (int i = threadIdx.x; i < 64 * (64 / FPL); i += kBlock) {
-      const int el = i / (64 / FPL), ll = i % (64 / FPL);
+    for (int i = threadIdx.x; i < 64 * (64 / PW); i += kBlock) {
+      const int el = i / (64 / PW), wl = i % (64 / PW);
       const int e = e0 + el;
-      float x[FPL];
-      if (e < c.E) Fmt::unpack(s.msg[((int64_t)t * c.E + e) * kWave + fc / FPL + ll], x);
+      float x[PW];
+      if (e < c.E) F::Elem::unpack(s.msg[((int64_t)t * c.E + e) * RW + fc / PW + wl], x);
       else {
 #pragma unroll
-        for (int k = 0; k < FPL; ++k) x[k] = 0.0f;
+        for (int h = 0; h < PW; ++h) x[h] = 0.0f;
       }
 #pragma unroll
-      for (int k = 0; k < FPL; ++k) tile[ll * FPL + k][el] = x[k];
+      for (int h = 0; h < PW; ++h) tile[wl * PW + h][el] = x[h];
     }
     __syncthreads();
     for (int i = threadIdx.x; i < 64 * 64; i += kBlock) {
@@ -738,43 +795,76 @@ __global__ void fill_i32_kernel(int32_t* p, int64_t n, int32_t v) {
 }
 
 // ------------------------------------------------------------------------------------------
-// Launchers (host side).
+// Launchers (host side).  Register-staged kernels are instantiated only where the staged
+// values fit the register budget (degree bound x frames per lane <= 64); the host picks a
+// layout that satisfies it (layout_fits()).
 
-template <class Fmt>
+template <class F>
 static hipError_t launch_vn(const DevCode& c, const DevState& s, const DevOut& o, int max_dv, int fused, int init,
                             hipStream_t st) {
+  constexpr int FPL = F::FPL;
   dim3 grid((c.N + kWavesPerBlock - 1) / kWavesPerBlock, s.tiles);
-  if (max_dv <= 4) hipLaunchKernelGGL((vn_kernel<Fmt, 4>), grid, dim3(kBlock), 0, st, c, s, o, fused, init);
-  else if (max_dv <= 8) hipLaunchKernelGGL((vn_kernel<Fmt, 8>), grid, dim3(kBlock), 0, st, c, s, o, fused, init);
-  else if (max_dv <= 16) hipLaunchKernelGGL((vn_kernel<Fmt, 16>), grid, dim3(kBlock), 0, st, c, s, o, fused, init);
-  else if (max_dv <= 32) hipLaunchKernelGGL((vn_kernel<Fmt, 32>), grid, dim3(kBlock), 0, st, c, s, o, fused, init);
-  else return hipErrorInvalidValue;
+  if (max_dv <= 4) hipLaunchKernelGGL((vn_kernel<F, 4>), grid, dim3(kBlock), 0, st, c, s, o, fused, init);
+  else if (max_dv <= 8) hipLaunchKernelGGL((vn_kernel<F, 8>), grid, dim3(kBlock), 0, st, c, s, o, fused, init);
+  else if (max_dv <= 16) {
+    if constexpr (16 * FPL <= 64) hipLaunchKernelGGL((vn_kernel<F, 16>), grid, dim3(kBlock), 0, st, c, s, o, fused, init);
+    else return hipErrorInvalidValue;
+  } else if (max_dv <= 32) {
+    if constexpr (32 * FPL <= 64) hipLaunchKernelGGL((vn_kernel<F, 32>), grid, dim3(kBlock), 0, st, c, s, o, fused, init);
+    else return hipErrorInvalidValue;
+  } else {
+    return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 
-template <class Fmt, int ALG>
+template <class F, int ALG>
 static hipError_t launch_cn_box(const DevCode& c, const DevState& s, const DevOpts& o, int max_dc, hipStream_t st) {
+  constexpr int FPL = F::FPL;
   dim3 grid((c.M + kWavesPerBlock - 1) / kWavesPerBlock, s.tiles);
-  if (max_dc <= 8) hipLaunchKernelGGL((cn_box_kernel<Fmt, 8, ALG>), grid, dim3(kBlock), 0, st, c, s, o);
-  else if (max_dc <= 16) hipLaunchKernelGGL((cn_box_kernel<Fmt, 16, ALG>), grid, dim3(kBlock), 0, st, c, s, o);
-  else if (max_dc <= 32) hipLaunchKernelGGL((cn_box_kernel<Fmt, 32, ALG>), grid, dim3(kBlock), 0, st, c, s, o);
-  else return hipErrorInvalidValue;
+  if (max_dc <= 8) {
+    if constexpr (8 * FPL <= 64) hipLaunchKernelGGL((cn_box_kernel<F, 8, ALG>), grid, dim3(kBlock), 0, st, c, s, o);
+    else return hipErrorInvalidValue;
+  } else if (max_dc <= 16) {
+    if constexpr (16 * FPL <= 64) hipLaunchKernelGGL((cn_box_kernel<F, 16, ALG>), grid, dim3(kBlock), 0, st, c, s, o);
+    else return hipErrorInvalidValue;
+  } else if (max_dc <= 32) {
+    if constexpr (32 * FPL <= 64) hipLaunchKernelGGL((cn_box_kernel<F, 32, ALG>), grid, dim3(kBlock), 0, st, c, s, o);
+    else return hipErrorInvalidValue;
+  } else {
+    return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 
-template <class Fmt>
+template <class F>
 static hipError_t launch_cn(const DevCode& c, const DevState& s, const DevOpts& o, int max_dc, hipStream_t st) {
+  constexpr int FPL = F::FPL;
   dim3 grid((c.M + kWavesPerBlock - 1) / kWavesPerBlock, s.tiles);
   if (o.algorithm == 2) {
-    if (max_dc <= 8) hipLaunchKernelGGL((cn_minsum_kernel<Fmt, 8>), grid, dim3(kBlock), 0, st, c, s, o);
-    else if (max_dc <= 16) hipLaunchKernelGGL((cn_minsum_kernel<Fmt, 16>), grid, dim3(kBlock), 0, st, c, s, o);
-    else if (max_dc <= 32) hipLaunchKernelGGL((cn_minsum_kernel<Fmt, 32>), grid, dim3(kBlock), 0, st, c, s, o);
-    else if (max_dc <= 64) hipLaunchKernelGGL((cn_minsum_kernel<Fmt, 0>), grid, dim3(kBlock), 0, st, c, s, o);
-    else return hipErrorInvalidValue;
+    bool done = false;
+    if (max_dc <= 8) {
+      hipLaunchKernelGGL((cn_minsum_kernel<F, 8>), grid, dim3(kBlock), 0, st, c, s, o);
+      done = true;
+    } else if (max_dc <= 16) {
+      if constexpr (16 * FPL <= 64) {
+        hipLaunchKernelGGL((cn_minsum_kernel<F, 16>), grid, dim3(kBlock), 0, st, c, s, o);
+        done = true;
+      }
+    } else if (max_dc <= 32) {
+      if constexpr (32 * FPL <= 64) {
+        hipLaunchKernelGGL((cn_minsum_kernel<F, 32>), grid, dim3(kBlock), 0, st, c, s, o);
+        done = true;
+      }
+    }
+    if (!done) {
+      if (max_dc > 64) return hipErrorInvalidValue;
+      hipLaunchKernelGGL((cn_minsum_kernel<F, 0>), grid, dim3(kBlock), 0, st, c, s, o);
+    }
     return hipGetLastError();
   }
-  if (o.algorithm == 1) return launch_cn_box<Fmt, 1>(c, s, o, max_dc, st);
-  return launch_cn_box<Fmt, 2>(c, s, o, max_dc, st);
+  if (o.algorithm == 1) return launch_cn_box<F, 1>(c, s, o, max_dc, st);
+  return launch_cn_box<F, 2>(c, s, o, max_dc, st);
 }
 
 // Syndrome (unless fused into the check kernel) and the per-frame state machine.
@@ -787,35 +877,34 @@ static hipError_t launch_syn_ctrl(const DevCode& c, const DevState& s, int itera
   return hipGetLastError();
 }
 
-template <class Fmt>
+template <class F>
 static hipError_t launch_ingest(const DevCode& c, const DevState& s, const void* llr, const void* syst,
                                 bool dbl, int64_t frames, int64_t frame0, uint8_t init_phase, hipStream_t st) {
-  constexpr int FPL = Fmt::FPL;
   dim3 grid((c.N + 63) / 64, s.tiles);
   if (dbl)
-    hipLaunchKernelGGL((ingest_kernel<double, FPL>), grid, dim3(kBlock), 0, st, c, s, (const double*)llr,
-                       (const double*)syst, frames, frame0, init_phase);
+    hipLaunchKernelGGL((ingest_kernel<double, F::TW>), grid, dim3(kBlock), 0, st, c, s, (const double*)llr,
+                       (const double*)syst, frames, frame0, init_phase, F::FPL);
   else
-    hipLaunchKernelGGL((ingest_kernel<float, FPL>), grid, dim3(kBlock), 0, st, c, s, (const float*)llr,
-                       (const float*)syst, frames, frame0, init_phase);
+    hipLaunchKernelGGL((ingest_kernel<float, F::TW>), grid, dim3(kBlock), 0, st, c, s, (const float*)llr,
+                       (const float*)syst, frames, frame0, init_phase, F::FPL);
   return hipGetLastError();
 }
 
-template <class Fmt>
+template <class F>
 static hipError_t launch_state_in(const DevCode& c, const DevState& s, const void* state, bool dbl,
                                   int64_t frames, int64_t frame0, hipStream_t st) {
   dim3 grid((c.E + 63) / 64, s.tiles);
   if (dbl)
-    hipLaunchKernelGGL((state_in_kernel<double, Fmt>), grid, dim3(kBlock), 0, st, c, s, (const double*)state, frames, frame0);
+    hipLaunchKernelGGL((state_in_kernel<double, F>), grid, dim3(kBlock), 0, st, c, s, (const double*)state, frames, frame0);
   else
-    hipLaunchKernelGGL((state_in_kernel<float, Fmt>), grid, dim3(kBlock), 0, st, c, s, (const float*)state, frames, frame0);
+    hipLaunchKernelGGL((state_in_kernel<float, F>), grid, dim3(kBlock), 0, st, c, s, (const float*)state, frames, frame0);
   return hipGetLastError();
 }
 
-template <class Fmt>
+template <class F>
 static hipError_t launch_results(const DevCode& c, const DevState& s, const DevOut& out, int64_t frames,
                                  int64_t frame0, const BatchResults& r, bool dbl, hipStream_t st) {
-  constexpr int FPL = Fmt::FPL;
+  constexpr int FPL = F::FPL, TW = F::TW;
   if (r.msg_bits)
     hipLaunchKernelGGL((unpack_hard_kernel<FPL>), dim3((c.K + kBlock - 1) / kBlock, (unsigned)frames), dim3(kBlock), 0, st,
                        out.hard, c.K, frames, r.msg_bits, frame0);
@@ -824,66 +913,87 @@ static hipError_t launch_results(const DevCode& c, const DevState& s, const DevO
                        r.iterations, frame0);
   auto to_frames = [&](const float* in, int64_t X, int64_t count, void* dst) {
     dim3 grid((unsigned)((count + 63) / 64), s.tiles);
-    if (dbl) hipLaunchKernelGGL((tiles_to_frames_kernel<double, FPL>), grid, dim3(kBlock), 0, st, in, X, count,
+    if (dbl) hipLaunchKernelGGL((tiles_to_frames_kernel<double, TW>), grid, dim3(kBlock), 0, st, in, X, count,
                                 frames, (double*)dst, frame0);
-    else hipLaunchKernelGGL((tiles_to_frames_kernel<float, FPL>), grid, dim3(kBlock), 0, st, in, X, count,
+    else hipLaunchKernelGGL((tiles_to_frames_kernel<float, TW>), grid, dim3(kBlock), 0, st, in, X, count,
                             frames, (float*)dst, frame0);
   };
   if (r.msg) to_frames(out.app, c.K, c.K, r.msg);
   if (r.parity) to_frames(out.ext, c.N, c.N, r.parity);
   if (r.syst) to_frames(out.ext, c.N, c.K, r.syst);
   if (r.state) {
-    if (dbl) hipLaunchKernelGGL((state_out_kernel<double, Fmt>), dim3((c.E + 63) / 64, s.tiles), dim3(kBlock), 0, st,
+    if (dbl) hipLaunchKernelGGL((state_out_kernel<double, F>), dim3((c.E + 63) / 64, s.tiles), dim3(kBlock), 0, st,
                                 c, s, frames, (double*)r.state, frame0);
-    else hipLaunchKernelGGL((state_out_kernel<float, Fmt>), dim3((c.E + 63) / 64, s.tiles), dim3(kBlock), 0, st,
+    else hipLaunchKernelGGL((state_out_kernel<float, F>), dim3((c.E + 63) / 64, s.tiles), dim3(kBlock), 0, st,
                             c, s, frames, (float*)r.state, frame0);
   }
   return hipGetLastError();
 }
 
 // ------------------------------------------------------------------------------------------
-// Format dispatch.
+// Format / layout dispatch.
 hipError_t kernels_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t st) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(fill_i32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, v);
   return hipGetLastError();
 }
 
-#define FECL_DISPATCH(fmt, CALL)                      \
-  switch (fmt) {                                      \
-    case 0: { using F = FmtF32; return CALL; }        \
-    case 1: { using F = FmtF16; return CALL; }        \
-    case 2: { using F = FmtBF16; return CALL; }       \
-    default: return hipErrorInvalidValue;             \
+#define FECL_DISPATCH_W(E, lay, CALL)                  \
+  switch (lay.words) {                                 \
+    case 1: { using F = Fmt<E, 1>; return CALL; }      \
+    case 2: { using F = Fmt<E, 2>; return CALL; }      \
+    case 4: { using F = Fmt<E, 4>; return CALL; }      \
+    default: return hipErrorInvalidValue;              \
+  }
+#define FECL_DISPATCH(lay, CALL)                                    \
+  switch (lay.fmt) {                                                \
+    case 0: FECL_DISPATCH_W(ElemF32, lay, CALL)                     \
+    case 1: FECL_DISPATCH_W(ElemF16, lay, CALL)                     \
+    case 2: FECL_DISPATCH_W(ElemBF16, lay, CALL)                    \
+    default: return hipErrorInvalidValue;                           \
   }
 
-hipError_t kernels_ingest(int fmt, const DevCode& c, const DevState& s, const void* llr, const void* syst, bool dbl,
-                          int64_t frames, int64_t frame0, uint8_t init_phase, hipStream_t st) {
-  FECL_DISPATCH(fmt, launch_ingest<F>(c, s, llr, syst, dbl, frames, frame0, init_phase, st));
+static int degree_bound(int d) { return d <= 4 ? 4 : d <= 8 ? 8 : d <= 16 ? 16 : d <= 32 ? 32 : 1 << 20; }
+
+bool layout_fits(const Layout& lay, int algorithm, int max_dc, int max_dv) {
+  const int fpl = lay.fpl();
+  if (max_dv > 32 || (max_dv > 8 && degree_bound(max_dv) * fpl > 64)) return false;
+  if (algorithm == 2) return max_dc <= 64;  // the streaming form covers rows above the budget
+  return max_dc <= 32 && degree_bound(max_dc) * fpl <= 64;
 }
-hipError_t kernels_state_in(int fmt, const DevCode& c, const DevState& s, const void* state, bool dbl,
+
+hipError_t kernels_ingest(const Layout& lay, const DevCode& c, const DevState& s, const void* llr, const void* syst,
+                          bool dbl, int64_t frames, int64_t frame0, uint8_t init_phase, hipStream_t st) {
+  FECL_DISPATCH(lay, launch_ingest<F>(c, s, llr, syst, dbl, frames, frame0, init_phase, st));
+}
+hipError_t kernels_state_in(const Layout& lay, const DevCode& c, const DevState& s, const void* state, bool dbl,
                             int64_t frames, int64_t frame0, hipStream_t st) {
-  FECL_DISPATCH(fmt, launch_state_in<F>(c, s, state, dbl, frames, frame0, st));
+  FECL_DISPATCH(lay, launch_state_in<F>(c, s, state, dbl, frames, frame0, st));
 }
-hipError_t kernels_vn_init(int fmt, const DevCode& c, const DevState& s, int max_dv, hipStream_t st) {
-  FECL_DISPATCH(fmt, launch_vn<F>(c, s, DevOut{}, max_dv, 0, 1, st));
+hipError_t kernels_vn_init(const Layout& lay, const DevCode& c, const DevState& s, int max_dv, hipStream_t st) {
+  FECL_DISPATCH(lay, launch_vn<F>(c, s, DevOut{}, max_dv, 0, 1, st));
 }
-hipError_t kernels_check(int fmt, const DevCode& c, const DevState& s, const DevOpts& o, int max_dc, hipStream_t st) {
-  FECL_DISPATCH(fmt, launch_cn<F>(c, s, o, max_dc, st));
+hipError_t kernels_check(const Layout& lay, const DevCode& c, const DevState& s, const DevOpts& o, int max_dc,
+                         hipStream_t st) {
+  FECL_DISPATCH(lay, launch_cn<F>(c, s, o, max_dc, st));
 }
-hipError_t kernels_variable(int fmt, const DevCode& c, const DevState& s, const DevOut& out, int max_dv,
+hipError_t kernels_variable(const Layout& lay, const DevCode& c, const DevState& s, const DevOut& out, int max_dv,
                             int fused, hipStream_t st) {
-  FECL_DISPATCH(fmt, launch_vn<F>(c, s, out, max_dv, fused, 0, st));
+  FECL_DISPATCH(lay, launch_vn<F>(c, s, out, max_dv, fused, 0, st));
 }
-hipError_t kernels_syndrome_control(int fmt, const DevCode& c, const DevState& s, int iterations, int fused,
-                                    int step, hipStream_t st) {
-  if (fmt == 0) return launch_syn_ctrl<1>(c, s, iterations, fused, step, st);
-  if (fmt == 1 || fmt == 2) return launch_syn_ctrl<2>(c, s, iterations, fused, step, st);
-  return hipErrorInvalidValue;
+hipError_t kernels_syndrome_control(const Layout& lay, const DevCode& c, const DevState& s, int iterations,
+                                    int fused, int step, hipStream_t st) {
+  switch (lay.fpl()) {
+    case 1: return launch_syn_ctrl<1>(c, s, iterations, fused, step, st);
+    case 2: return launch_syn_ctrl<2>(c, s, iterations, fused, step, st);
+    case 4: return launch_syn_ctrl<4>(c, s, iterations, fused, step, st);
+    case 8: return launch_syn_ctrl<8>(c, s, iterations, fused, step, st);
+    default: return hipErrorInvalidValue;
+  }
 }
-hipError_t kernels_results(int fmt, const DevCode& c, const DevState& s, const DevOut& out, int64_t frames,
+hipError_t kernels_results(const Layout& lay, const DevCode& c, const DevState& s, const DevOut& out, int64_t frames,
                            int64_t frame0, const BatchResults& r, bool dbl, hipStream_t st) {
-  FECL_DISPATCH(fmt, launch_results<F>(c, s, out, frames, frame0, r, dbl, st));
+  FECL_DISPATCH(lay, launch_results<F>(c, s, out, frames, frame0, r, dbl, st));
 }
 
 }  // namespace fecl

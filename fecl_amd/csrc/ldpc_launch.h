@@ -17,22 +17,32 @@ struct BatchResults {
   void* state;           // [F][E]  check-to-variable
 };
 
-inline int frames_per_lane(int fmt) { return fmt == 0 ? 1 : 2; }
-inline int tile_width(int fmt) { return 64 * frames_per_lane(fmt); }
+// Device layout of a workspace: message format and the 32-bit words each lane owns per edge.
+struct Layout {
+  int fmt = 0;     // fecl_msg_format
+  int words = 1;   // 1, 2 or 4
+  int pw() const { return fmt == 0 ? 1 : 2; }        // messages per word
+  int fpl() const { return words * pw(); }           // frames per lane
+  int tw() const { return 64 * fpl(); }              // frames per tile
+  int rw() const { return 64 * words; }              // words per (tile, edge) row
+};
+
+// Whether the kernels for this layout stay within the register budget for the code's degrees.
+bool layout_fits(const Layout& lay, int algorithm, int max_dc, int max_dv);
 
 hipError_t kernels_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t st);
-hipError_t kernels_ingest(int fmt, const DevCode& c, const DevState& s, const void* llr, const void* syst,
+hipError_t kernels_ingest(const Layout& lay, const DevCode& c, const DevState& s, const void* llr, const void* syst,
                           bool dbl, int64_t frames, int64_t frame0, uint8_t init_phase, hipStream_t st);
-hipError_t kernels_state_in(int fmt, const DevCode& c, const DevState& s, const void* state, bool dbl,
+hipError_t kernels_state_in(const Layout& lay, const DevCode& c, const DevState& s, const void* state, bool dbl,
                             int64_t frames, int64_t frame0, hipStream_t st);
-hipError_t kernels_vn_init(int fmt, const DevCode& c, const DevState& s, int max_dv, hipStream_t st);
-hipError_t kernels_check(int fmt, const DevCode& c, const DevState& s, const DevOpts& o, int max_dc,
+hipError_t kernels_vn_init(const Layout& lay, const DevCode& c, const DevState& s, int max_dv, hipStream_t st);
+hipError_t kernels_check(const Layout& lay, const DevCode& c, const DevState& s, const DevOpts& o, int max_dc,
                          hipStream_t st);
-hipError_t kernels_variable(int fmt, const DevCode& c, const DevState& s, const DevOut& out, int max_dv,
+hipError_t kernels_variable(const Layout& lay, const DevCode& c, const DevState& s, const DevOut& out, int max_dv,
                             int fused, hipStream_t st);
-hipError_t kernels_syndrome_control(int fmt, const DevCode& c, const DevState& s, int iterations, int fused,
+hipError_t kernels_syndrome_control(const Layout& lay, const DevCode& c, const DevState& s, int iterations, int fused,
                                     int step, hipStream_t st);
-hipError_t kernels_results(int fmt, const DevCode& c, const DevState& s, const DevOut& out, int64_t frames,
+hipError_t kernels_results(const Layout& lay, const DevCode& c, const DevState& s, const DevOut& out, int64_t frames,
                            int64_t frame0, const BatchResults& r, bool dbl, hipStream_t st);
 
 }  // namespace fecl

@@ -228,6 +228,25 @@ class Ldpc:
         return (DecoderOptions().algorithm(o.algorithm).iterations(o.iterations)
                 .scalingFactor(scaling).msgFormat(o.msg_format))
 
+    # ---- device layout (no counterpart in FeCl)
+    def setLayout(self, words):
+        """Pin the lane layout: 32-bit words per lane per edge (1, 2, 4; 0 = automatic)."""
+        check(lib().fecl_ldpc_set_layout(self._h, int(words)))
+
+    def layout(self):
+        """(words per lane, frames per tile) of the current device layout."""
+        w, tw = ctypes.c_int32(), ctypes.c_int32()
+        check(lib().fecl_ldpc_get_layout(self._h, ctypes.byref(w), ctypes.byref(tw)))
+        return w.value, tw.value
+
+    def autotune(self, frames, probe_iterations=10):
+        """Times each eligible layout on a frames-frame batch and pins the fastest.
+        Returns (words, {words: ms})."""
+        best = ctypes.c_int32()
+        ms = (ctypes.c_double * 3)()
+        check(lib().fecl_ldpc_autotune(self._h, int(frames), int(probe_iterations), ctypes.byref(best), ms))
+        return best.value, {w: ms[i] for i, w in enumerate((1, 2, 4)) if ms[i] >= 0}
+
     # ---- encode / check (host)
     def encode(self, msg):
         msg = np.ascontiguousarray(msg, dtype=np.uint8).ravel()

@@ -125,6 +125,18 @@ int fecl_ldpc_sodecode_device(fecl_ldpc* h, const float* parity_in, const float*
 int fecl_ldpc_profile(fecl_ldpc* h, int32_t enable);
 int fecl_ldpc_profile_read(fecl_ldpc* h, double* ms /*[3]*/, int64_t* launches /*[3]*/);
 
+/* Device layout (no counterpart in the reference).  Frames are decoded in tiles of
+ * 64 * words * (1 for FP32, 2 for 16-bit) frames; each GPU lane moves `words` 32-bit words per
+ * edge.  set_layout pins words (1, 2, 4; 0 = widest that fits the code's degrees).
+ * autotune times `probe_iterations` iterations of a `frames`-frame decode for every layout
+ * that fits, pins the fastest and returns it; ms[3] (nullable) receives the times for
+ * words 1, 2, 4 (-1 = not eligible).  Neither may run concurrently with a decode on the
+ * same handle. */
+int fecl_ldpc_set_layout(fecl_ldpc* h, int32_t words);
+int fecl_ldpc_get_layout(const fecl_ldpc* h, int32_t* words, int32_t* frames_per_tile);
+int fecl_ldpc_autotune(fecl_ldpc* h, int64_t frames, int32_t probe_iterations, int32_t* best_words,
+                       double* ms);
+
 /* Releases the cached device workspaces of a handle (they are reused across calls). */
 int fecl_ldpc_trim(fecl_ldpc* h);
 

@@ -25,7 +25,7 @@ def gallager_llr(n, k, frames, ebn0, seed):
     return awgn_llr(np.zeros((frames, n), np.uint8), k / n, ebn0, rng)
 
 
-def dvbs2_llr(frames, ebn0, seed, n=64800, rate=(1, 2), return_info=False):
+def dvbs2_llr(frames, ebn0, seed, rate=(1, 2), n=64800, return_info=False):
     from oracle import oracle as O
     code = O.dvbs2(n, *rate)
     rng = np.random.default_rng(seed)

@@ -33,9 +33,12 @@ def dvbs2(n=64800, rate=(1, 2)):
     return oc, F.CheckMatrix(oc.row_ptr, oc.col, oc.N)
 
 
-def gpu_decoder(H, alg, iters, sf=1.0, fmt=F.MsgFormat.F32):
+def gpu_decoder(H, alg, iters, sf=1.0, fmt=F.MsgFormat.F32, words=0):
     opt = F.DecoderOptions().algorithm(alg).iterations(iters).scalingFactor(sf).msgFormat(fmt)
-    return F.Ldpc(H, opt)
+    dec = F.Ldpc(H, opt)
+    if words:
+        dec.setLayout(words)
+    return dec
 
 
 def oracle_run(oc, llr, alg, iters, sf=1.0, prec=32, fmt=O.MSG_F32, **kw):
@@ -56,14 +59,15 @@ def assert_bit_exact(gpu, ref, frames, K, N, E):
 
 
 @pytest.mark.parametrize("alg", [F.Approximate, F.Linear])
-@pytest.mark.parametrize("iters,sf", [(1, 1.0), (2, 1.0), (5, 1.0), (30, 1.0), (30, 0.75), (30, 0.9),
-                                      (5, [0.5, 0.625, 0.75, 0.875, 1.0])])
-def test_gallager_fp32_bit_exact(alg, iters, sf):
+@pytest.mark.parametrize("iters,sf,words", [(1, 1.0, 1), (2, 1.0, 4), (5, 1.0, 2), (30, 1.0, 1), (30, 1.0, 4),
+                                            (30, 0.75, 2), (30, 0.9, 4), (5, [0.5, 0.625, 0.75, 0.875, 1.0], 1),
+                                            (5, [0.5, 0.625, 0.75, 0.875, 1.0], 4)])
+def test_gallager_fp32_bit_exact(alg, iters, sf, words):
     oc, H = gallager()
     llr = np.concatenate([gallager_llr(oc.N, oc.K, 3, 2.0, 11), gallager_llr(oc.N, oc.K, 2, -1.0, 12)])
     llr32 = llr.astype(np.float32).astype(np.float64)
     ref = oracle_run(oc, llr32, alg, iters, sf)
-    dec = gpu_decoder(H, alg, iters, {0: sf} if isinstance(sf, list) else sf)
+    dec = gpu_decoder(H, alg, iters, {0: sf} if isinstance(sf, list) else sf, words=words)
     out = dec.soDecode(llr32)
     assert_bit_exact(out, ref, llr.shape[0], oc.K, oc.N, oc.E)
     hard, it = dec.decode(llr32, iterations=True)
@@ -71,12 +75,13 @@ def test_gallager_fp32_bit_exact(alg, iters, sf):
     assert np.array_equal(it, ref["iters"])
 
 
-@pytest.mark.parametrize("frames", [1, 63, 65, 130])
-def test_dvbs2_minsum_fp32_ragged_frames(frames):
+@pytest.mark.parametrize("frames,words", [(1, 4), (63, 1), (65, 2), (130, 4), (300, 4)])
+def test_dvbs2_minsum_fp32_ragged_frames(frames, words):
     oc, H = dvbs2()
     llr = dvbs2_llr(frames, 1.4, seed=100 + frames).astype(np.float32).astype(np.float64)
     ref = oracle_run(oc, llr, F.Approximate, 50)
-    dec = gpu_decoder(H, F.Approximate, 50)
+    dec = gpu_decoder(H, F.Approximate, 50, words=words)
+    assert dec.layout()[0] == words
     out = dec.soDecode(llr)
     assert_bit_exact(out, ref, frames, oc.K, oc.N, oc.E)
     hard, it = dec.decode(llr, iterations=True)
@@ -99,13 +104,24 @@ def test_dvbs2_minsum_fp32_1024_frames_device_path():
 
 
 @pytest.mark.parametrize("fmt,ofmt", [(F.MsgFormat.F16, O.MSG_F16), (F.MsgFormat.BF16, O.MSG_BF16)])
-@pytest.mark.parametrize("alg", [F.Approximate, F.Linear])
-def test_16bit_messages_bit_exact(fmt, ofmt, alg):
+@pytest.mark.parametrize("alg,words", [(F.Approximate, 1), (F.Approximate, 4), (F.Linear, 2), (F.Linear, 4)])
+def test_16bit_messages_bit_exact(fmt, ofmt, alg, words):
     oc, H = dvbs2()
     llr = dvbs2_llr(5, 1.2, seed=7).astype(np.float32).astype(np.float64)
     ref = oracle_run(oc, llr, alg, 50, fmt=ofmt)
-    out = gpu_decoder(H, alg, 50, fmt=fmt).soDecode(llr)
+    out = gpu_decoder(H, alg, 50, fmt=fmt, words=words).soDecode(llr)
     assert_bit_exact(out, ref, 5, oc.K, oc.N, oc.E)
+
+
+@pytest.mark.parametrize("rate", [(1, 4), (3, 5), (2, 3), (3, 4), (5, 6), (9, 10)])
+def test_all_normal_rates_minsum_f16(rate):
+    """Every DVB-S2 normal rate (row degrees up to 30, variable degrees up to 13) in the
+    widest layout the register budget allows."""
+    oc, H = dvbs2(64800, rate)
+    llr = dvbs2_llr(3, 2.5 + 2 * rate[0] / rate[1], seed=8, rate=rate).astype(np.float32).astype(np.float64)
+    ref = oracle_run(oc, llr, F.Approximate, 30, fmt=O.MSG_F16)
+    out = gpu_decoder(H, F.Approximate, 30, fmt=F.MsgFormat.F16).soDecode(llr)
+    assert_bit_exact(out, ref, 3, oc.K, oc.N, oc.E)
 
 
 @pytest.mark.parametrize("code", ["gallager", "dvbs2"])
@@ -139,7 +155,8 @@ def test_exact_fp32_against_fp64(code):
     assert np.all(np.sign(g[~fin]) == np.sign(r[~fin]))
 
 
-def test_sodecode_state_and_syst_inputs():
+@pytest.mark.parametrize("words", [1, 4])
+def test_sodecode_state_and_syst_inputs(words):
     """soDecode with a-priori state and systematic inputs (BpDecoderImpl.cpp:73-93)."""
     oc, H = gallager()
     llr = gallager_llr(oc.N, oc.K, 2, -1.0, 12).astype(np.float32).astype(np.float64)
@@ -147,8 +164,19 @@ def test_sodecode_state_and_syst_inputs():
     state = first["state"].astype(np.float32).astype(np.float64)
     syst = np.random.default_rng(99).normal(0, 1, (2, oc.K)).astype(np.float32).astype(np.float64)
     ref = oracle_run(oc, llr, F.Approximate, 7, state_in=state, syst_in=syst)
-    out = gpu_decoder(H, F.Approximate, 7).soDecode(llr, syst=syst, state=state)
+    out = gpu_decoder(H, F.Approximate, 7, words=words).soDecode(llr, syst=syst, state=state)
     assert_bit_exact(out, ref, 2, oc.K, oc.N, oc.E)
+
+
+def test_autotune_picks_an_eligible_layout():
+    oc, H = dvbs2()
+    dec = gpu_decoder(H, F.Approximate, 50)
+    best, times = dec.autotune(256, probe_iterations=3)
+    assert best in times and dec.layout()[0] == best
+    llr = dvbs2_llr(2, 1.4, seed=3).astype(np.float32).astype(np.float64)
+    ref = oracle_run(oc, llr, F.Approximate, 50)
+    hard, it = dec.decode(llr, iterations=True)
+    assert np.array_equal(hard.reshape(2, oc.K), ref["hard"]) and np.array_equal(it, ref["iters"])
 
 
 def test_two_phase_resume_equals_single_run():
