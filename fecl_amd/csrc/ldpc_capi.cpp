@@ -715,6 +715,96 @@ int fecl_ldpc_autotune(fecl_ldpc* h, int64_t frames, int32_t probe_iterations, i
   return FECL_OK;
 }
 
+}  // extern "C"
+
+// A captured multi-code batch: one branch per code, all launched as one hipGraph.
+struct fecl_ldpc_graph {
+  std::vector<fecl_ldpc*> handles;
+  std::vector<std::unique_ptr<Workspace>> ws;
+  std::vector<hipStream_t> streams;
+  std::vector<hipEvent_t> events;
+  hipStream_t capture = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~fecl_ldpc_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (auto e : events) (void)hipEventDestroy(e);
+    for (auto st : streams) (void)hipStreamDestroy(st);
+    if (capture) (void)hipStreamDestroy(capture);
+  }
+};
+
+extern "C" {
+
+int fecl_ldpc_graph_create(fecl_ldpc* const* handles, const int64_t* frames, int32_t count, const float* const* llr,
+                           uint8_t* const* msg, int32_t* const* iterations, fecl_ldpc_graph** out) {
+  if (!out || !handles || !frames || !llr || !msg || count < 1) return fail(FECL_INVALID_ARGUMENT, "invalid arguments");
+  *out = nullptr;
+  std::unique_ptr<fecl_ldpc_graph> g(new fecl_ldpc_graph());
+  const int dev = handles[0]->device;
+  HIP_TRY(hipSetDevice(dev));
+  for (int32_t i = 0; i < count; ++i) {
+    if (!handles[i] || frames[i] < 1 || !llr[i] || !msg[i]) return fail(FECL_INVALID_ARGUMENT, "invalid branch");
+    if (handles[i]->device != dev) return fail(FECL_INVALID_ARGUMENT, "all codes must live on one device");
+  }
+  HIP_TRY(hipStreamCreateWithFlags(&g->capture, hipStreamNonBlocking));
+  hipEvent_t fork = nullptr;
+  HIP_TRY(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+  g->events.push_back(fork);
+  // Size every workspace before capture: nothing may allocate while the graph is recorded.
+  for (int32_t i = 0; i < count; ++i) {
+    fecl_ldpc* h = handles[i];
+    g->handles.push_back(h);
+    std::unique_ptr<Workspace> ws(new Workspace());
+    DevState s;
+    DevOut o;
+    if (int rc = prepare(h, ws.get(), layout_of(h), frames[i], false, &s, &o)) return rc;
+    g->ws.push_back(std::move(ws));
+    hipStream_t st = nullptr;
+    HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    g->streams.push_back(st);
+    hipEvent_t join = nullptr;
+    HIP_TRY(hipEventCreateWithFlags(&join, hipEventDisableTiming));
+    g->events.push_back(join);
+  }
+  HIP_TRY(hipStreamBeginCapture(g->capture, hipStreamCaptureModeThreadLocal));
+  int rc = FECL_OK;
+  hipError_t e = hipEventRecord(fork, g->capture);
+  for (int32_t i = 0; i < count && rc == FECL_OK && e == hipSuccess; ++i) {
+    e = hipStreamWaitEvent(g->streams[i], fork, 0);
+    if (e != hipSuccess) break;
+    BatchIO io{};
+    io.llr = llr[i];
+    io.res.msg_bits = msg[i];
+    io.res.iterations = iterations ? iterations[i] : nullptr;
+    rc = run_batch(handles[i], g->ws[i].get(), g->streams[i], io, frames[i], 0, /*capture=*/true);
+    if (rc != FECL_OK) break;
+    e = hipEventRecord(g->events[i + 1], g->streams[i]);
+    if (e == hipSuccess) e = hipStreamWaitEvent(g->capture, g->events[i + 1], 0);
+  }
+  hipGraph_t graph = nullptr;
+  hipError_t e2 = hipStreamEndCapture(g->capture, &graph);
+  g->graph = graph;
+  if (rc != FECL_OK) return rc;
+  HIP_TRY(e);
+  HIP_TRY(e2);
+  HIP_TRY(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+  *out = g.release();
+  return FECL_OK;
+}
+
+int fecl_ldpc_graph_launch(fecl_ldpc_graph* g, void* stream) {
+  if (!g) return fail(FECL_INVALID_ARGUMENT, "null graph");
+  HIP_TRY(hipGraphLaunch(g->exec, (hipStream_t)stream));
+  return FECL_OK;
+}
+
+int fecl_ldpc_graph_destroy(fecl_ldpc_graph* g) {
+  delete g;
+  return FECL_OK;
+}
+
 int fecl_ldpc_trim(fecl_ldpc* h) {
   if (int rc = check_handle(h)) return rc;
   std::lock_guard<std::mutex> lk(h->mu);

@@ -360,3 +360,49 @@ def _is_cuda_tensor(x):
     except ImportError:
         return False
     return isinstance(x, torch.Tensor) and x.is_cuda
+
+
+class BatchGraph:
+    """Several codes' batches decoded by one hipGraph launch (fecl_ldpc_graph_*).
+
+    decoders: Ldpc handles; llr: CUDA float32 tensors [frames_i * N_i] that stay alive and are
+    re-read at every launch().  Outputs live in .msg[i] (uint8 [frames_i * K_i]) and
+    .iterations[i] (int32 [frames_i])."""
+
+    def __init__(self, decoders, llr):
+        import torch
+        if len(decoders) != len(llr) or not decoders:
+            raise ValueError("one LLR tensor per decoder")
+        self._decoders = list(decoders)
+        self.llr = [x.contiguous().view(-1) for x in llr]
+        for x in self.llr:
+            if not x.is_cuda or x.dtype != torch.float32:
+                raise ValueError("LLRs must be CUDA float32 tensors")
+        self.frames = [d._frames(x.numel()) for d, x in zip(self._decoders, self.llr)]
+        self.msg = [torch.empty(f * d.msgSize(), dtype=torch.uint8, device=x.device)
+                    for f, d, x in zip(self.frames, self._decoders, self.llr)]
+        self.iterations = [torch.empty(f, dtype=torch.int32, device=x.device)
+                           for f, x in zip(self.frames, self.llr)]
+        n = len(decoders)
+        P = ctypes.c_void_p
+        hs = (P * n)(*[d._h.value for d in self._decoders])
+        fr = (ctypes.c_int64 * n)(*self.frames)
+        ll = (P * n)(*[x.data_ptr() for x in self.llr])
+        ms = (P * n)(*[m.data_ptr() for m in self.msg])
+        it = (P * n)(*[i.data_ptr() for i in self.iterations])
+        self._g = P()
+        check(lib().fecl_ldpc_graph_create(hs, fr, n, ll, ms, it, ctypes.byref(self._g)))
+
+    def launch(self, stream=None):
+        import torch
+        st = stream if stream is not None else torch.cuda.current_stream(self.llr[0].device).cuda_stream
+        check(lib().fecl_ldpc_graph_launch(self._g, ctypes.c_void_p(st)))
+
+    def __del__(self):
+        g = getattr(self, "_g", None)
+        if g is not None and g.value:
+            try:
+                lib().fecl_ldpc_graph_destroy(g)
+            except Exception:
+                pass
+            self._g = None

@@ -137,6 +137,20 @@ int fecl_ldpc_get_layout(const fecl_ldpc* h, int32_t* words, int32_t* frames_per
 int fecl_ldpc_autotune(fecl_ldpc* h, int64_t frames, int32_t probe_iterations, int32_t* best_words,
                        double* ms);
 
+/* Multi-code batch in one launch (no counterpart in the reference: FeCl decodes one code per
+ * call).  Captures, for each i < count, a full decode of frames[i] frames of code handles[i]
+ * (device LLRs llr[i] [frames[i]][N_i] FP32 -> device msg[i] [frames[i]][K_i] bytes and
+ * iterations[i] [frames[i]], nullable) into one hipGraph whose branches run concurrently.
+ * Each launch decodes whatever the input buffers hold at that time.  Every step of every
+ * branch is recorded; frames that finish early leave the remaining steps as cheap no-ops.
+ * The options (and layout) of each handle are those at creation time. */
+typedef struct fecl_ldpc_graph fecl_ldpc_graph;
+int fecl_ldpc_graph_create(fecl_ldpc* const* handles, const int64_t* frames, int32_t count,
+                           const float* const* llr, uint8_t* const* msg, int32_t* const* iterations,
+                           fecl_ldpc_graph** out);
+int fecl_ldpc_graph_launch(fecl_ldpc_graph* g, void* stream);
+int fecl_ldpc_graph_destroy(fecl_ldpc_graph* g);
+
 /* Releases the cached device workspaces of a handle (they are reused across calls). */
 int fecl_ldpc_trim(fecl_ldpc* h);
 

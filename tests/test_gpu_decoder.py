@@ -201,3 +201,35 @@ def test_zero_frames_and_errors():
         dec.soDecode(np.zeros(oc.N), syst=np.zeros(3))
     with pytest.raises(ValueError):
         F.Ldpc(H, F.DecoderOptions().iterations(0))
+
+
+def test_batch_graph_multirate_replays():
+    """Several codes in one hipGraph launch; replayed with new inputs copied in eagerly between
+    launches, every replay checked against the oracle and against the eager decode."""
+    import torch
+    specs = [(64800, (1, 2), 40, 1.6), (64800, (3, 4), 70, 3.2), (16200, (1, 2), 130, 2.0)]
+    decs, tensors, codes = [], [], []
+    for n, rate, frames, _ in specs:
+        oc, H = dvbs2(n, rate)
+        codes.append(oc)
+        decs.append(gpu_decoder(H, F.Approximate, 50, fmt=F.MsgFormat.F16))
+        tensors.append(torch.zeros(frames * oc.N, dtype=torch.float32, device="cuda"))
+    g = F.BatchGraph(decs, tensors)
+    for rep in range(3):
+        refs = []
+        for i, (n, rate, frames, ebn0) in enumerate(specs):
+            llr = dvbs2_llr(frames, ebn0 + 0.2 * rep, seed=40 + 10 * rep + i, rate=rate, n=n).astype(np.float32)
+            tensors[i].copy_(torch.from_numpy(llr.ravel()))
+            refs.append(oracle_run(codes[i], llr.astype(np.float64), F.Approximate, 50, fmt=O.MSG_F16,
+                                   want=("hard", "iters")))
+        torch.cuda.synchronize()
+        g.launch()
+        torch.cuda.synchronize()
+        for i, (n, rate, frames, _) in enumerate(specs):
+            K = codes[i].K
+            got = g.msg[i].cpu().numpy().reshape(frames, K)
+            assert np.array_equal(got, refs[i]["hard"]), (rep, i)
+            assert np.array_equal(g.iterations[i].cpu().numpy(), refs[i]["iters"]), (rep, i)
+            eager, it = decs[i].decode(tensors[i], iterations=True)
+            torch.cuda.synchronize()
+            assert np.array_equal(eager.cpu().numpy().reshape(frames, K), got)
