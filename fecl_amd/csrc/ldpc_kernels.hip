@@ -157,25 +157,35 @@ struct CnLanes {
   bool lane_any, lane_all, wave_fresh, wave_seen;  // wave_seen: some frame has a prior VN
 };
 
+// Frame state loads and the decisions drawn from them are split so that a kernel can issue
+// its row loads in between: the wait for the state then does not wait for the rows.
 template <int FPL>
-__device__ static inline bool cn_lanes(const DevState& s, const DevOpts& o, int t, int lane, CnLanes<FPL>& L) {
+__device__ static inline void lane_state(const DevState& s, int t, int lane, uint8_t (&ph)[FPL], int (&it)[FPL]) {
   constexpr int TW = kWave * FPL;
+#pragma unroll
+  for (int k = 0; k < FPL; ++k) {
+    const int64_t p = (int64_t)t * TW + lane * FPL + k;
+    ph[k] = s.phase[p];
+    it[k] = s.iter[p];
+  }
+}
+
+template <int FPL>
+__device__ static inline bool cn_decide(const DevOpts& o, const uint8_t (&ph)[FPL], const int (&it)[FPL],
+                                        CnLanes<FPL>& L) {
   L.lane_any = false;
   L.lane_all = true;
   bool any_fresh = false, seen = false;
 #pragma unroll
   for (int k = 0; k < FPL; ++k) {
-    const int64_t p = (int64_t)t * TW + lane * FPL + k;
-    const uint8_t ph = s.phase[p];
-    const int base = ph & PH_MASK;
-    const int it = s.iter[p];
+    const int base = ph[k] & PH_MASK;
     L.act[k] = base == PH_RUN || base == PH_FINAL;
-    L.fresh[k] = L.act[k] && (ph & PH_FRESH);
-    L.sfrow[k] = (base == PH_FINAL ? o.iterations - 1 : it) % o.sf_rows;
+    L.fresh[k] = L.act[k] && (ph[k] & PH_FRESH);
+    L.sfrow[k] = (base == PH_FINAL ? o.iterations - 1 : it[k]) % o.sf_rows;
     L.lane_any |= L.act[k];
     L.lane_all &= L.act[k];
     any_fresh |= L.fresh[k];
-    seen |= base == PH_RUN && it >= 1;
+    seen |= base == PH_RUN && it[k] >= 1;
   }
   if (__ballot(L.lane_any) == 0) return false;
   L.wave_fresh = __ballot(any_fresh) != 0;
@@ -189,14 +199,20 @@ __device__ static inline bool cn_lanes(const DevState& s, const DevOpts& o, int 
 // Only bits not yet set are sent as atomics, so a tile costs a few atomics per frame.
 template <int FPL>
 __device__ static inline void cn_syndrome(const DevCode& c, const DevState& s, int t, int e0, int d, int lane) {
+  // lanes 0..d-1 each fetch one column's hard words; a wave-wide xor folds them
   uint64_t syn[FPL];
 #pragma unroll
   for (int k = 0; k < FPL; ++k) syn[k] = 0;
-  const uint64_t* ht = s.hard + (int64_t)t * c.N * FPL;
-  for (int j = 0; j < d; ++j) {
-    const int v = uniform(c.col[e0 + j]);
+  if (lane < d) {
+    const int v = c.col[e0 + lane];
+    const uint64_t* hv = s.hard + ((int64_t)t * c.N + v) * FPL;
 #pragma unroll
-    for (int k = 0; k < FPL; ++k) syn[k] ^= ht[(int64_t)v * FPL + k];
+    for (int k = 0; k < FPL; ++k) syn[k] = hv[k];
+  }
+#pragma unroll
+  for (int k = 0; k < FPL; ++k) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) syn[k] ^= __shfl_xor(syn[k], o, 64);
   }
   if (lane == 0) {
 #pragma unroll
@@ -224,13 +240,21 @@ __global__ __launch_bounds__(kBlock) void cn_minsum_kernel(DevCode c, DevState s
   const int r = uniform(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6));
   if (r >= c.M) return;
   const int lane = threadIdx.x & 63;
-  CnLanes<FPL> ln;
-  if (!cn_lanes<FPL>(s, o, t, lane, ln)) return;
-
   const int e0 = uniform(c.row_ptr[r]);
   const int d = uniform(c.row_ptr[r + 1]) - e0;
   uint32_t* mrow = s.msg + ((int64_t)t * c.E + e0) * RW + lane * F::W;
   const float* Lt = s.L + (int64_t)t * c.N * TW + lane * FPL;
+  uint8_t ph[FPL];
+  int it[FPL];
+  lane_state<FPL>(s, t, lane, ph, it);
+  float xs[DMAX > 0 ? DMAX : 1][FPL];
+  if constexpr (DMAX > 0) {
+#pragma unroll
+    for (int j = 0; j < DMAX; ++j)
+      if (j < d) F::load(mrow + (int64_t)j * RW, xs[j]);
+  }
+  CnLanes<FPL> ln;
+  if (!cn_decide<FPL>(o, ph, it, ln)) return;
 
   float m1[FPL], m2[FPL];
   int i1[FPL];
@@ -246,8 +270,7 @@ __global__ __launch_bounds__(kBlock) void cn_minsum_kernel(DevCode c, DevState s
       else if (a < m2[k]) { m2[k] = a; }
     }
   };
-  auto load_x = [&](int j, float (&x)[FPL]) {
-    F::load(mrow + (int64_t)j * RW, x);
+  auto fresh_x = [&](int j, float (&x)[FPL]) {   // frames whose v2c is still L[col(e)]
     if (ln.wave_fresh) {
       const int v = uniform(c.col[e0 + j]);
       float l[FPL];
@@ -257,12 +280,10 @@ __global__ __launch_bounds__(kBlock) void cn_minsum_kernel(DevCode c, DevState s
         if (ln.fresh[k]) x[k] = F::quantize(l[k]);
     }
   };
-
-  float xs[DMAX > 0 ? DMAX : 1][FPL];
   if constexpr (DMAX > 0) {
 #pragma unroll
     for (int j = 0; j < DMAX; ++j)
-      if (j < d) load_x(j, xs[j]);
+      if (j < d) fresh_x(j, xs[j]);
   }
   if (o.fused_syn && ln.wave_seen) cn_syndrome<FPL>(c, s, t, e0, d, lane);
   if constexpr (DMAX > 0) {
@@ -272,7 +293,8 @@ __global__ __launch_bounds__(kBlock) void cn_minsum_kernel(DevCode c, DevState s
   } else {
     for (int j = 0; j < d; ++j) {
       float x[FPL];
-      load_x(j, x);
+      F::load(mrow + (int64_t)j * RW, x);
+      fresh_x(j, x);
       accumulate(j, x);
     }
   }
@@ -351,21 +373,24 @@ __global__ __launch_bounds__(kBlock) void cn_box_kernel(DevCode c, DevState s, D
   const int r = uniform(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6));
   if (r >= c.M) return;
   const int lane = threadIdx.x & 63;
-  CnLanes<FPL> ln;
-  if (!cn_lanes<FPL>(s, o, t, lane, ln)) return;
-
   const int e0 = uniform(c.row_ptr[r]);
   const int d = uniform(c.row_ptr[r + 1]) - e0;
   uint32_t* mrow = s.msg + ((int64_t)t * c.E + e0) * RW + lane * F::W;
   const float* Lt = s.L + (int64_t)t * c.N * TW + lane * FPL;
-  if (o.fused_syn && ln.wave_seen) cn_syndrome<FPL>(c, s, t, e0, d, lane);
-
+  uint8_t ph[FPL];
+  int it[FPL];
+  lane_state<FPL>(s, t, lane, ph, it);
   float x[DMAX][FPL];
 #pragma unroll
-  for (int j = 0; j < DMAX; ++j) {
-    if (j < d) {
-      F::load(mrow + (int64_t)j * RW, x[j]);
-      if (ln.wave_fresh) {
+  for (int j = 0; j < DMAX; ++j)
+    if (j < d) F::load(mrow + (int64_t)j * RW, x[j]);
+  CnLanes<FPL> ln;
+  if (!cn_decide<FPL>(o, ph, it, ln)) return;
+  if (o.fused_syn && ln.wave_seen) cn_syndrome<FPL>(c, s, t, e0, d, lane);
+  if (ln.wave_fresh) {   // frames whose v2c is still L[col(e)]
+#pragma unroll
+    for (int j = 0; j < DMAX; ++j) {
+      if (j < d) {
         const int v = uniform(c.col[e0 + j]);
         float l[FPL];
         load_floats<FPL>(Lt + (int64_t)v * TW, l);
@@ -455,18 +480,36 @@ __global__ __launch_bounds__(kBlock) void vn_kernel(DevCode c, DevState s, DevOu
   const int v = uniform(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6));
   if (v >= c.N) return;
   const int lane = threadIdx.x & 63;
+  // graph indices, frame state and the variable's rows are all requested before any of them
+  // is waited on
+  const int s0 = uniform(c.var_ptr[v]);
+  const int dv = uniform(c.var_ptr[v + 1]) - s0;
+  int eid[VMAX];
+#pragma unroll
+  for (int j = 0; j < VMAX; ++j)
+    if (j < dv) eid[j] = uniform(c.var_edges[s0 + j]);
+  uint8_t ph[FPL];
+  int itv[FPL];
+  lane_state<FPL>(s, t, lane, ph, itv);
+  uint64_t fw[FPL];
+#pragma unroll
+  for (int k = 0; k < FPL; ++k) fw[k] = s.fail[t * FPL + k];
+  float Lv[FPL];
+  load_floats<FPL>(s.L + ((int64_t)t * c.N + v) * TW + lane * FPL, Lv);
+  uint32_t* mt = s.msg + (int64_t)t * c.E * RW + lane * F::W;
+  float cv[VMAX][FPL];
+#pragma unroll
+  for (int j = 0; j < VMAX; ++j)
+    if (j < dv) F::load(mt + (int64_t)eid[j] * RW, cv[j]);
   bool run[FPL], fin[FPL];
   bool lane_run = false, lane_fin = false;
 #pragma unroll
   for (int k = 0; k < FPL; ++k) {
-    const int64_t p = (int64_t)t * TW + lane * FPL + k;
-    const uint8_t ph = s.phase[p];
-    const int base = ph & PH_MASK;
+    const int base = ph[k] & PH_MASK;
     // fused syndrome: a RUN frame whose previous hard decision satisfied every check has just
     // had its final check update (see cn_syndrome)
-    const bool passed = fused_syn && !init_pass && base == PH_RUN && s.iter[p] >= 1 &&
-                        ((s.fail[t * FPL + k] >> lane) & 1ull) == 0;
-    run[k] = init_pass ? ((ph & PH_VNINIT) != 0) : (base == PH_RUN && !passed);
+    const bool passed = fused_syn && !init_pass && base == PH_RUN && itv[k] >= 1 && ((fw[k] >> lane) & 1ull) == 0;
+    run[k] = init_pass ? ((ph[k] & PH_VNINIT) != 0) : (base == PH_RUN && !passed);
     fin[k] = !init_pass && (base == PH_FINAL || passed);
     lane_run |= run[k];
     lane_fin |= fin[k];
@@ -474,20 +517,6 @@ __global__ __launch_bounds__(kBlock) void vn_kernel(DevCode c, DevState s, DevOu
   const bool wave_run = __ballot(lane_run) != 0;
   const bool wave_fin = __ballot(lane_fin) != 0;
   if (!wave_run && !wave_fin) return;
-  const int s0 = uniform(c.var_ptr[v]);
-  const int dv = uniform(c.var_ptr[v + 1]) - s0;
-  float Lv[FPL];
-  load_floats<FPL>(s.L + ((int64_t)t * c.N + v) * TW + lane * FPL, Lv);
-  uint32_t* mt = s.msg + (int64_t)t * c.E * RW + lane * F::W;
-  float cv[VMAX][FPL];
-  int eid[VMAX];
-#pragma unroll
-  for (int j = 0; j < VMAX; ++j) {
-    if (j < dv) {
-      eid[j] = uniform(c.var_edges[s0 + j]);
-      F::load(mt + (int64_t)eid[j] * RW, cv[j]);
-    }
-  }
   if (wave_fin) {
     float app[FPL], ext[FPL];
 #pragma unroll
