@@ -21,11 +21,11 @@
 
 #include "../../include/fecl_ldpc.h"
 #include "ldpc_code.h"
+#include "ldpc_engine.h"
 #include "ldpc_launch.h"
 
-namespace {
-
-using namespace fecl;
+namespace fecl {
+namespace engine {
 
 thread_local std::string g_err;
 
@@ -34,86 +34,6 @@ int fail(int status, const std::string& msg) {
   return status;
 }
 
-#define HIP_TRY(expr)                                                                            \
-  do {                                                                                           \
-    hipError_t e_ = (expr);                                                                      \
-    if (e_ != hipSuccess)                                                                        \
-      return fail(e_ == hipErrorOutOfMemory ? FECL_OUT_OF_MEMORY : FECL_DEVICE_ERROR,            \
-                  std::string(#expr) + ": " + hipGetErrorString(e_));                            \
-  } while (0)
-
-struct DevBuf {
-  void* p = nullptr;
-  size_t bytes = 0;
-  hipError_t ensure(size_t n) {
-    if (n <= bytes) return hipSuccess;
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    bytes = 0;
-    hipError_t e = hipMalloc(&p, n);
-    if (e == hipSuccess) bytes = n;
-    return e;
-  }
-  void release() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    bytes = 0;
-  }
-  template <typename T> T* as() const { return static_cast<T*>(p); }
-};
-
-struct Workspace {
-  DevBuf msg, L, hard, phase, iter, fail, flags, active, out_hard, out_ext, out_app;
-  DevBuf stage_in, stage_syst, stage_state, stage_out;
-  int32_t* host_active = nullptr;
-  int host_active_cap = 0;
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-  hipEvent_t done = nullptr;   // recorded after the last use, waited on by the next user's stream
-  std::vector<std::array<hipEvent_t, 4>> pev;   // per-step timing events (profiling only)
-  hipStream_t own_stream = nullptr;
-  ~Workspace() {
-    for (DevBuf* b : {&msg, &L, &hard, &phase, &iter, &fail, &flags, &active, &out_hard, &out_ext, &out_app,
-                      &stage_in, &stage_syst, &stage_state, &stage_out})
-      b->release();
-    if (host_active) (void)hipHostFree(host_active);
-    for (auto& e : ev)
-      if (e) (void)hipEventDestroy(e);
-    if (done) (void)hipEventDestroy(done);
-    for (auto& a : pev)
-      for (auto& e : a) (void)hipEventDestroy(e);
-    if (own_stream) (void)hipStreamDestroy(own_stream);
-  }
-};
-
-}  // namespace
-
-struct fecl_ldpc {
-  Code code;
-  int device = 0;
-  DevBuf d_row_ptr, d_col, d_var_ptr, d_var_edges, d_sf, d_sf32;
-  DevCode dc{};
-  // options
-  int32_t algorithm = FECL_APPROXIMATE;
-  int32_t iterations = 1;
-  int32_t msg_format = FECL_MSG_F32;
-  std::vector<double> sf;
-  int32_t sf_rows = 1, sf_cols = 1;
-  int32_t sf_mode = 0;
-  int32_t words = 1;        // device layout: 32-bit words per lane per edge (1, 2 or 4)
-  int32_t words_pinned = 0; // set by fecl_ldpc_set_layout / autotune; 0 = automatic
-  std::mutex mu;
-  std::vector<std::unique_ptr<Workspace>> pool;
-  // kernel timing (fecl_ldpc_profile): check, variable, syndrome+control
-  bool profiling = false;
-  double prof_ms[3] = {0, 0, 0};
-  int64_t prof_launches[3] = {0, 0, 0};
-  ~fecl_ldpc() {
-    pool.clear();
-    for (DevBuf* b : {&d_row_ptr, &d_col, &d_var_ptr, &d_var_edges, &d_sf, &d_sf32}) b->release();
-  }
-};
-
-namespace {
 
 // Frames decoded per batch: bounded so a workspace stays a few GiB at most.
 constexpr int64_t kMaxChunkFrames = 8192;
@@ -231,14 +151,6 @@ void release(fecl_ldpc* h, std::unique_ptr<Workspace> ws) {
   h->pool.push_back(std::move(ws));
 }
 
-struct BatchIO {
-  const void* llr;        // [F][N]
-  const void* syst;       // [F][K] or null
-  const void* state;      // [F][E] or null
-  bool dbl_in;
-  BatchResults res;
-  bool dbl_out;
-};
 
 // Device layout of the handle's current options.
 Layout layout_of(const fecl_ldpc* h) {
@@ -289,7 +201,7 @@ int prepare(fecl_ldpc* h, Workspace* ws, const Layout& lay, int64_t frames, bool
 // capture: the launches are being recorded into a hipGraph, so every step is enqueued (frames
 // that finish early make the remaining steps cheap no-ops) and nothing waits on the host.
 int run_batch(fecl_ldpc* h, Workspace* ws, hipStream_t st, const BatchIO& io, int64_t frames, int64_t frame0,
-              bool capture = false) {
+              bool capture) {
   const Layout lay = layout_of(h);
   const Code& c = h->code;
   const bool soft = io.res.msg || io.res.syst || io.res.parity;
@@ -407,7 +319,7 @@ int host_decode(fecl_ldpc* h, const double* llr, const double* syst_in, const do
     io.res.syst = syst ? (void*)(ob + o_syst) : nullptr;
     io.res.parity = parity ? (void*)(ob + o_par) : nullptr;
     io.res.state = state ? (void*)(ob + o_state) : nullptr;
-    if (int rc = run_batch(h, ws.get(), st, io, n, 0)) return rc;
+    if (int rc = run_batch(h, ws.get(), st, io, n, 0, false)) return rc;
     if (msg_bits) HIP_TRY(hipMemcpyAsync(msg_bits + f0 * c.K, ob + o_bits, (size_t)n * c.K, hipMemcpyDeviceToHost, st));
     if (iterations) HIP_TRY(hipMemcpyAsync(iterations + f0, ob + o_iter, (size_t)n * 4, hipMemcpyDeviceToHost, st));
     if (msg) HIP_TRY(hipMemcpyAsync(msg + f0 * c.K, ob + o_msg, (size_t)n * c.K * 8, hipMemcpyDeviceToHost, st));
@@ -442,13 +354,17 @@ int device_decode(fecl_ldpc* h, const float* llr, const float* syst_in, const fl
     io.dbl_in = false;
     io.dbl_out = false;
     io.res = BatchResults{msg_bits, iterations, msg, syst, parity, state};
-    if (int rc = run_batch(h, ws.get(), st, io, n, f0)) return rc;
+    if (int rc = run_batch(h, ws.get(), st, io, n, f0, false)) return rc;
   }
   HIP_TRY(hipEventRecord(ws->done, st));
   return FECL_OK;
 }
 
-}  // namespace
+}  // namespace engine
+}  // namespace fecl
+
+using namespace fecl;
+using namespace fecl::engine;
 
 extern "C" {
 

@@ -1,0 +1,124 @@
+// Engine internals shared by the C ABI (ldpc_capi.cpp) and the serving loop (ldpc_server.cpp):
+// device buffers, cached workspaces, the handle, and the batch runner.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <array>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/fecl_ldpc.h"
+#include "ldpc_code.h"
+#include "ldpc_launch.h"
+
+namespace fecl {
+namespace engine {
+
+// Records msg as the calling thread's last error and returns status.
+int fail(int status, const std::string& msg);
+
+#define HIP_TRY(expr)                                                                            \
+  do {                                                                                           \
+    hipError_t e_ = (expr);                                                                      \
+    if (e_ != hipSuccess)                                                                        \
+      return fail(e_ == hipErrorOutOfMemory ? FECL_OUT_OF_MEMORY : FECL_DEVICE_ERROR,            \
+                  std::string(#expr) + ": " + hipGetErrorString(e_));                            \
+  } while (0)
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  hipError_t ensure(size_t n) {
+    if (n <= bytes) return hipSuccess;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    hipError_t e = hipMalloc(&p, n);
+    if (e == hipSuccess) bytes = n;
+    return e;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <typename T> T* as() const { return static_cast<T*>(p); }
+};
+
+struct Workspace {
+  DevBuf msg, L, hard, phase, iter, fail, flags, active, out_hard, out_ext, out_app;
+  DevBuf stage_in, stage_syst, stage_state, stage_out;
+  int32_t* host_active = nullptr;
+  int host_active_cap = 0;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t done = nullptr;   // recorded after the last use, waited on by the next user's stream
+  std::vector<std::array<hipEvent_t, 4>> pev;   // per-step timing events (profiling only)
+  hipStream_t own_stream = nullptr;
+  ~Workspace() {
+    for (DevBuf* b : {&msg, &L, &hard, &phase, &iter, &fail, &flags, &active, &out_hard, &out_ext, &out_app,
+                      &stage_in, &stage_syst, &stage_state, &stage_out})
+      b->release();
+    if (host_active) (void)hipHostFree(host_active);
+    for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+    if (done) (void)hipEventDestroy(done);
+    for (auto& a : pev)
+      for (auto& e : a) (void)hipEventDestroy(e);
+    if (own_stream) (void)hipStreamDestroy(own_stream);
+  }
+};
+
+}  // namespace engine
+}  // namespace fecl
+
+struct fecl_ldpc {
+  fecl::Code code;
+  int device = 0;
+  fecl::engine::DevBuf d_row_ptr, d_col, d_var_ptr, d_var_edges, d_sf, d_sf32;
+  fecl::DevCode dc{};
+  // options
+  int32_t algorithm = FECL_APPROXIMATE;
+  int32_t iterations = 1;
+  int32_t msg_format = FECL_MSG_F32;
+  std::vector<double> sf;
+  int32_t sf_rows = 1, sf_cols = 1;
+  int32_t sf_mode = 0;
+  int32_t words = 1;        // device layout: 32-bit words per lane per edge (1, 2 or 4)
+  int32_t words_pinned = 0; // set by fecl_ldpc_set_layout / autotune; 0 = automatic
+  std::mutex mu;
+  std::vector<std::unique_ptr<fecl::engine::Workspace>> pool;
+  // kernel timing (fecl_ldpc_profile): check, variable, syndrome+control
+  bool profiling = false;
+  double prof_ms[3] = {0, 0, 0};
+  int64_t prof_launches[3] = {0, 0, 0};
+  ~fecl_ldpc() {
+    pool.clear();
+    for (fecl::engine::DevBuf* b : {&d_row_ptr, &d_col, &d_var_ptr, &d_var_edges, &d_sf, &d_sf32}) b->release();
+  }
+};
+
+namespace fecl {
+namespace engine {
+
+struct BatchIO {
+  const void* llr;        // [F][N]
+  const void* syst;       // [F][K] or null
+  const void* state;      // [F][E] or null
+  bool dbl_in;
+  BatchResults res;
+  bool dbl_out;
+};
+
+int check_handle(const fecl_ldpc* h);
+Layout layout_of(const fecl_ldpc* h);
+DevOpts make_opts(const fecl_ldpc* h);
+// Sizes ws for `frames` frames of h's layout and returns the device views.
+int prepare(fecl_ldpc* h, Workspace* ws, const Layout& lay, int64_t frames, bool soft, DevState* sp, DevOut* op);
+// Decodes frames [frame0, frame0 + frames) of io (see ldpc_capi.cpp).
+int run_batch(fecl_ldpc* h, Workspace* ws, hipStream_t st, const BatchIO& io, int64_t frames, int64_t frame0,
+              bool capture);
+
+}  // namespace engine
+}  // namespace fecl
