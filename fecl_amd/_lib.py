@@ -30,10 +30,17 @@ SIGNATURES = {
     "fecl_ldpc_profile_read": [P, P, P],
     "fecl_ldpc_set_layout": [P, I32],
     "fecl_ldpc_get_layout": [P, P, P],
+    "fecl_ldpc_set_fused_syndrome": [P, I32],
     "fecl_ldpc_autotune": [P, I64, I32, P, P],
     "fecl_ldpc_graph_create": [P, P, I32, P, P, P, P],
     "fecl_ldpc_graph_launch": [P, P],
     "fecl_ldpc_graph_destroy": [P],
+    "fecl_ldpc_server_create": [P, I32, P, I64, P],
+    "fecl_ldpc_server_slots": [P, I32, P],
+    "fecl_ldpc_server_submit": [P, I32, P, I64, P, P, P],
+    "fecl_ldpc_server_wait": [P, ctypes.c_uint64, ctypes.c_double, P, P],
+    "fecl_ldpc_server_stats": [P, P, P, P],
+    "fecl_ldpc_server_destroy": [P],
     "fecl_ldpc_trim": [P],
 }
 

@@ -117,7 +117,7 @@ DevOpts make_opts(const fecl_ldpc* h) {
   o.algorithm = h->algorithm;
   // With one scaling row the check update after a passing syndrome equals FeCl's final one
   // (factor row I-1), so the syndrome can ride in the next check kernel.
-  o.fused_syn = h->sf_rows == 1 ? 1 : 0;
+  o.fused_syn = (h->fused_syndrome && h->sf_rows == 1) ? 1 : 0;
   // FeCl's Linearlog1pexpm table in float: y[i] = log(1 + exp(-i/2)) evaluated in double
   // (src/detail/LinearTable.h:40-44, 63), slopes y[i+1] - y[i] in float.
   float y[8];
@@ -238,7 +238,7 @@ int run_batch(fecl_ldpc* h, Workspace* ws, hipStream_t st, const BatchIO& io, in
     if (prof) HIP_TRY(hipEventRecord(ws->pev[step][1], st));
     HIP_TRY(kernels_variable(lay, h->dc, s, out, c.max_var_degree, o.fused_syn, st));
     if (prof) HIP_TRY(hipEventRecord(ws->pev[step][2], st));
-    HIP_TRY(kernels_syndrome_control(lay, h->dc, s, h->iterations, o.fused_syn, step, st));
+    HIP_TRY(kernels_syndrome_control(lay, h->dc, s, h->iterations, o.fused_syn, step, c.max_row_degree, st));
     if (prof) HIP_TRY(hipEventRecord(ws->pev[step][3], st));
     steps_run = step + 1;
     if (capture) continue;
@@ -555,6 +555,13 @@ int fecl_ldpc_set_layout(fecl_ldpc* h, int32_t words) {
   std::lock_guard<std::mutex> lk(h->mu);
   h->words_pinned = words;
   choose_layout(h);
+  return FECL_OK;
+}
+
+int fecl_ldpc_set_fused_syndrome(fecl_ldpc* h, int32_t enable) {
+  if (int rc = check_handle(h)) return rc;
+  std::lock_guard<std::mutex> lk(h->mu);
+  h->fused_syndrome = enable != 0;
   return FECL_OK;
 }
 

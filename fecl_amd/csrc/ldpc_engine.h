@@ -87,6 +87,11 @@ struct fecl_ldpc {
   int32_t sf_mode = 0;
   int32_t words = 1;        // device layout: 32-bit words per lane per edge (1, 2 or 4)
   int32_t words_pinned = 0; // set by fecl_ldpc_set_layout / autotune; 0 = automatic
+  // Syndrome evaluated inside the next check kernel (one scaling row only) instead of a
+  // separate kernel after the variable pass.  Off: the separate kernel reduces per workgroup
+  // and sends few atomics, while the fused form puts one device-coherent load per row on the
+  // tile's fail word.
+  bool fused_syndrome = false;
   std::mutex mu;
   std::vector<std::unique_ptr<fecl::engine::Workspace>> pool;
   // kernel timing (fecl_ldpc_profile): check, variable, syndrome+control

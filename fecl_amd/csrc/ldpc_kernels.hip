@@ -15,6 +15,8 @@
 // packed PW per 32-bit word (PW = 1 for FP32, 2 for FP16/BF16): RW = TW / PW words.  Each lane
 // owns W consecutive words (FPL = W * PW frames), so a wavefront moves 256*W contiguous bytes
 // per edge; W is the layout knob the autotuner picks (frames per workgroup = 4 * 64 * FPL).
+#include <algorithm>
+
 #include "ldpc_kernels.h"
 #include "ldpc_launch.h"
 
@@ -587,9 +589,11 @@ __global__ __launch_bounds__(kBlock) void vn_kernel(DevCode c, DevState s, DevOu
 
 // ------------------------------------------------------------------------------------------
 // Syndrome (when not fused into the check kernel): one thread per (check row, tile) xors the
-// hard words of its columns; the failing frames of the tile are or-ed into fail[tile].
-template <int FPL>
+// hard words of its columns (all loads of a row issued together); the failing frames are or-ed
+// across the workgroup in LDS and sent as one atomic per workgroup into fail[tile].
+template <int FPL, int DMAX>
 __global__ __launch_bounds__(kBlock) void syndrome_kernel(DevCode c, DevState s) {
+  __shared__ uint64_t part[kWavesPerBlock][FPL];
   const int t = blockIdx.y;
   if (!(s.tile_flags[t] & TF_RUN)) return;
   const int r = blockIdx.x * kBlock + threadIdx.x;
@@ -598,16 +602,40 @@ __global__ __launch_bounds__(kBlock) void syndrome_kernel(DevCode c, DevState s)
   for (int k = 0; k < FPL; ++k) f[k] = 0;
   if (r < c.M) {
     const uint64_t* ht = s.hard + (int64_t)t * c.N * FPL;
-    for (int e = c.row_ptr[r]; e < c.row_ptr[r + 1]; ++e) {
-      const int v = c.col[e];
+    const int e0 = c.row_ptr[r];
+    const int d = c.row_ptr[r + 1] - e0;
+    if constexpr (DMAX > 0) {
+      int v[DMAX];
 #pragma unroll
-      for (int k = 0; k < FPL; ++k) f[k] ^= ht[(int64_t)v * FPL + k];
+      for (int j = 0; j < DMAX; ++j)
+        if (j < d) v[j] = c.col[e0 + j];
+#pragma unroll
+      for (int j = 0; j < DMAX; ++j) {
+        if (j < d) {
+#pragma unroll
+          for (int k = 0; k < FPL; ++k) f[k] ^= ht[(int64_t)v[j] * FPL + k];
+        }
+      }
+    } else {
+      for (int j = 0; j < d; ++j) {
+        const int v = c.col[e0 + j];
+#pragma unroll
+        for (int k = 0; k < FPL; ++k) f[k] ^= ht[(int64_t)v * FPL + k];
+      }
     }
   }
+  const int wave = threadIdx.x >> 6;
 #pragma unroll
   for (int k = 0; k < FPL; ++k) {
     const uint64_t w = wave_or(f[k]);
-    if ((threadIdx.x & 63) == 0 && w) atomicOr((unsigned long long*)&s.fail[t * FPL + k], (unsigned long long)w);
+    if ((threadIdx.x & 63) == 0) part[wave][k] = w;
+  }
+  __syncthreads();
+  if (threadIdx.x < FPL) {
+    uint64_t w = 0;
+#pragma unroll
+    for (int i = 0; i < kWavesPerBlock; ++i) w |= part[i][threadIdx.x];
+    if (w) atomicOr((unsigned long long*)&s.fail[t * FPL + threadIdx.x], (unsigned long long)w);
   }
 }
 
@@ -659,7 +687,7 @@ __global__ __launch_bounds__(kWave) void control_kernel(DevState s, int iteratio
 #pragma unroll
     for (int k = 0; k < FPL; ++k) s.fail[t * FPL + k] = 0;
     s.tile_flags[t] = (blive ? TF_LIVE : 0u) | (brun ? TF_RUN : 0u);
-    if (alive) atomicAdd(&s.active[step], alive);
+    if (alive && step >= 0) atomicAdd(&s.active[step], alive);
   }
 }
 
@@ -812,6 +840,43 @@ __global__ __launch_bounds__(kBlock) void state_out_kernel(DevCode c, DevState s
   }
 }
 
+// Serving: frames admitted into free slots.  stage holds their LLRs frame-major [n][N];
+// slot i of the list receives frame i (L rows written at the slot's position in its tile).
+template <int TW>
+__global__ __launch_bounds__(kBlock) void admit_kernel(DevCode c, DevState s, const float* stage,
+                                                       const int32_t* slots, int n, uint8_t init_phase) {
+  const int i = blockIdx.y;
+  if (i >= n) return;
+  const int p = slots[i];
+  const int t = p / TW, pos = p % TW;
+  for (int v = blockIdx.x * kBlock + threadIdx.x; v < c.N; v += gridDim.x * kBlock)
+    s.L[((int64_t)t * c.N + v) * TW + pos] = stage[(int64_t)i * c.N + v];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    s.phase[p] = init_phase;
+    s.iter[p] = 0;
+    const uint8_t base = init_phase & PH_MASK;
+    atomicOr(&s.tile_flags[t], (base == PH_RUN || base == PH_FINAL ? TF_LIVE : 0u) | (base == PH_RUN ? TF_RUN : 0u));
+  }
+}
+
+// Serving: results of finished slots -> bits [n][K] (0/1) and iterations [n].
+template <int FPL>
+__global__ __launch_bounds__(kBlock) void extract_kernel(const uint64_t* hard, const int32_t* iter, int K,
+                                                         const int32_t* slots, int n, uint8_t* bits,
+                                                         int32_t* iters) {
+  constexpr int TW = kWave * FPL;
+  const int i = blockIdx.y;
+  if (i >= n) return;
+  const int p = slots[i];
+  const int64_t t = p / TW;
+  const int pos = p % TW;
+  for (int v = blockIdx.x * kBlock + threadIdx.x; v < K; v += gridDim.x * kBlock) {
+    const uint64_t w = hard[(t * K + v) * FPL + (pos % FPL)];
+    bits[(int64_t)i * K + v] = (uint8_t)((w >> (pos / FPL)) & 1ull);
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) iters[i] = iter[p];
+}
+
 // Per-frame iteration counts -> iterations [F].
 __global__ void iters_out_kernel(const int32_t* iter, int64_t frames, int32_t* out, int64_t frame0) {
   const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -899,9 +964,14 @@ static hipError_t launch_cn(const DevCode& c, const DevState& s, const DevOpts& 
 // Syndrome (unless fused into the check kernel) and the per-frame state machine.
 template <int FPL>
 static hipError_t launch_syn_ctrl(const DevCode& c, const DevState& s, int iterations, int fused, int step,
-                                  hipStream_t st) {
-  if (!fused)
-    hipLaunchKernelGGL((syndrome_kernel<FPL>), dim3((c.M + kBlock - 1) / kBlock, s.tiles), dim3(kBlock), 0, st, c, s);
+                                  int max_dc, hipStream_t st) {
+  if (!fused) {
+    dim3 grid((c.M + kBlock - 1) / kBlock, s.tiles);
+    if (max_dc <= 8) hipLaunchKernelGGL((syndrome_kernel<FPL, 8>), grid, dim3(kBlock), 0, st, c, s);
+    else if (max_dc <= 16) hipLaunchKernelGGL((syndrome_kernel<FPL, 16>), grid, dim3(kBlock), 0, st, c, s);
+    else if (max_dc <= 32) hipLaunchKernelGGL((syndrome_kernel<FPL, 32>), grid, dim3(kBlock), 0, st, c, s);
+    else hipLaunchKernelGGL((syndrome_kernel<FPL, 0>), grid, dim3(kBlock), 0, st, c, s);
+  }
   hipLaunchKernelGGL((control_kernel<FPL>), dim3(s.tiles), dim3(kWave), 0, st, s, iterations, fused, step);
   return hipGetLastError();
 }
@@ -1011,15 +1081,43 @@ hipError_t kernels_variable(const Layout& lay, const DevCode& c, const DevState&
   FECL_DISPATCH(lay, launch_vn<F>(c, s, out, max_dv, fused, 0, st));
 }
 hipError_t kernels_syndrome_control(const Layout& lay, const DevCode& c, const DevState& s, int iterations,
-                                    int fused, int step, hipStream_t st) {
+                                    int fused, int step, int max_dc, hipStream_t st) {
   switch (lay.fpl()) {
-    case 1: return launch_syn_ctrl<1>(c, s, iterations, fused, step, st);
-    case 2: return launch_syn_ctrl<2>(c, s, iterations, fused, step, st);
-    case 4: return launch_syn_ctrl<4>(c, s, iterations, fused, step, st);
-    case 8: return launch_syn_ctrl<8>(c, s, iterations, fused, step, st);
+    case 1: return launch_syn_ctrl<1>(c, s, iterations, fused, step, max_dc, st);
+    case 2: return launch_syn_ctrl<2>(c, s, iterations, fused, step, max_dc, st);
+    case 4: return launch_syn_ctrl<4>(c, s, iterations, fused, step, max_dc, st);
+    case 8: return launch_syn_ctrl<8>(c, s, iterations, fused, step, max_dc, st);
     default: return hipErrorInvalidValue;
   }
 }
+hipError_t kernels_admit(const Layout& lay, const DevCode& c, const DevState& s, const float* stage,
+                         const int32_t* slots, int n, uint8_t init_phase, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  dim3 grid((unsigned)std::min<int64_t>((c.N + kBlock - 1) / kBlock, 64), (unsigned)n);
+  switch (lay.tw()) {
+    case 64: hipLaunchKernelGGL((admit_kernel<64>), grid, dim3(kBlock), 0, st, c, s, stage, slots, n, init_phase); break;
+    case 128: hipLaunchKernelGGL((admit_kernel<128>), grid, dim3(kBlock), 0, st, c, s, stage, slots, n, init_phase); break;
+    case 256: hipLaunchKernelGGL((admit_kernel<256>), grid, dim3(kBlock), 0, st, c, s, stage, slots, n, init_phase); break;
+    case 512: hipLaunchKernelGGL((admit_kernel<512>), grid, dim3(kBlock), 0, st, c, s, stage, slots, n, init_phase); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t kernels_extract(const Layout& lay, const DevCode& c, const DevState& s, const DevOut& out,
+                           const int32_t* slots, int n, uint8_t* bits, int32_t* iters, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  dim3 grid((unsigned)std::min<int64_t>((c.K + kBlock - 1) / kBlock, 64), (unsigned)n);
+  switch (lay.fpl()) {
+    case 1: hipLaunchKernelGGL((extract_kernel<1>), grid, dim3(kBlock), 0, st, out.hard, s.iter, c.K, slots, n, bits, iters); break;
+    case 2: hipLaunchKernelGGL((extract_kernel<2>), grid, dim3(kBlock), 0, st, out.hard, s.iter, c.K, slots, n, bits, iters); break;
+    case 4: hipLaunchKernelGGL((extract_kernel<4>), grid, dim3(kBlock), 0, st, out.hard, s.iter, c.K, slots, n, bits, iters); break;
+    case 8: hipLaunchKernelGGL((extract_kernel<8>), grid, dim3(kBlock), 0, st, out.hard, s.iter, c.K, slots, n, bits, iters); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 hipError_t kernels_results(const Layout& lay, const DevCode& c, const DevState& s, const DevOut& out, int64_t frames,
                            int64_t frame0, const BatchResults& r, bool dbl, hipStream_t st) {
   FECL_DISPATCH(lay, launch_results<F>(c, s, out, frames, frame0, r, dbl, st));

@@ -41,7 +41,12 @@ hipError_t kernels_check(const Layout& lay, const DevCode& c, const DevState& s,
 hipError_t kernels_variable(const Layout& lay, const DevCode& c, const DevState& s, const DevOut& out, int max_dv,
                             int fused, hipStream_t st);
 hipError_t kernels_syndrome_control(const Layout& lay, const DevCode& c, const DevState& s, int iterations, int fused,
-                                    int step, hipStream_t st);
+                                    int step, int max_dc, hipStream_t st);
+// Serving: write admitted frames' LLRs into their slots; read finished slots' results.
+hipError_t kernels_admit(const Layout& lay, const DevCode& c, const DevState& s, const float* stage,
+                         const int32_t* slots, int n, uint8_t init_phase, hipStream_t st);
+hipError_t kernels_extract(const Layout& lay, const DevCode& c, const DevState& s, const DevOut& out,
+                           const int32_t* slots, int n, uint8_t* bits, int32_t* iters, hipStream_t st);
 hipError_t kernels_results(const Layout& lay, const DevCode& c, const DevState& s, const DevOut& out, int64_t frames,
                            int64_t frame0, const BatchResults& r, bool dbl, hipStream_t st);
 

@@ -1,0 +1,387 @@
+// Continuous-batching decode server (C ABI in include/fecl_ldpc.h, fecl_ldpc_server_*).
+//
+// FeCl has no serving layer: Codec::decode blocks on one batch (src/Codec.h:210-238).  Here
+// each code owns a pool of device frame slots whose message state stays resident across
+// iterations.  A worker thread repeats one step for every pool:
+//   admit   queued frames into free slots (their LLRs are copied into the slots' tile rows)
+//   step    check-node, variable-node (+ outputs of finishing frames), syndrome/control
+//   retire  frames that reached DONE: read their hard bits and iteration counts, free the slots
+// so requests that arrive while others are on the GPU join at the next step boundary, and a
+// frame's arithmetic is exactly that of a batch decode (no cross-frame state).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "ldpc_engine.h"
+
+namespace fecl {
+namespace engine {
+
+using Clock = std::chrono::steady_clock;
+
+struct Request {
+  uint64_t id = 0;
+  int pool = 0;
+  const float* llr = nullptr;   // host [frames][N]
+  int64_t frames = 0;
+  uint8_t* msg = nullptr;       // host [frames][K]
+  int32_t* iters = nullptr;     // host [frames] or null
+  int64_t admitted = 0, done = 0;
+  Clock::time_point t_submit, t_done;
+  bool finished = false;
+};
+
+struct PinnedBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  hipError_t ensure(size_t n) {
+    if (n <= bytes) return hipSuccess;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+    hipError_t e = hipHostMalloc(&p, n, hipHostMallocDefault);
+    if (e == hipSuccess) bytes = n;
+    return e;
+  }
+  ~PinnedBuf() {
+    if (p) (void)hipHostFree(p);
+  }
+  template <typename T> T* as() const { return static_cast<T*>(p); }
+};
+
+struct Pool {
+  fecl_ldpc* h = nullptr;
+  Layout lay;
+  DevOpts opts{};
+  Workspace ws;
+  DevState s{};
+  DevOut out{};
+  int64_t nslots = 0;
+  int admit_cap = 0;
+  std::vector<int32_t> free_slots;
+  std::vector<std::pair<Request*, int64_t>> owner;  // slot -> (request, frame)
+  int64_t live = 0;
+  std::deque<Request*> queue;
+  DevBuf d_stage, d_admit, d_ext_slots, d_ext_bits, d_ext_iters;
+  PinnedBuf h_stage, h_admit, h_ext_slots, h_ext_bits, h_ext_iters, h_phase;
+  ~Pool() {
+    for (DevBuf* b : {&d_stage, &d_admit, &d_ext_slots, &d_ext_bits, &d_ext_iters}) b->release();
+  }
+};
+
+}  // namespace engine
+}  // namespace fecl
+
+using namespace fecl;
+using namespace fecl::engine;
+
+struct fecl_ldpc_server {
+  std::vector<std::unique_ptr<Pool>> pools;
+  std::mutex mu;                       // queues, requests, statistics
+  std::condition_variable work_cv, done_cv;
+  std::unordered_map<uint64_t, std::unique_ptr<Request>> requests;
+  uint64_t next_id = 1;
+  bool stop = false;
+  int error = FECL_OK;
+  std::string error_msg;
+  int64_t frames_done = 0, steps = 0, queued_frames = 0;
+  hipStream_t st = nullptr;
+  int device = 0;
+  std::thread worker;
+  ~fecl_ldpc_server() {
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      stop = true;
+    }
+    work_cv.notify_all();
+    if (worker.joinable()) worker.join();
+    (void)hipSetDevice(device);
+    if (st) (void)hipStreamSynchronize(st);
+    pools.clear();
+    if (st) (void)hipStreamDestroy(st);
+  }
+};
+
+namespace {
+
+// One admit / step / retire round over every pool; returns a status (errors stop the server).
+int server_round(fecl_ldpc_server* sv, bool* idle) {
+  struct Admit { Request* req; int64_t f0, n; };
+  std::vector<std::vector<Admit>> admits(sv->pools.size());
+  bool any_live = false;
+  {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    for (size_t pi = 0; pi < sv->pools.size(); ++pi) {
+      Pool& P = *sv->pools[pi];
+      int n = 0;
+      while (!P.free_slots.empty() && !P.queue.empty() && n < P.admit_cap) {
+        Request* r = P.queue.front();
+        const int64_t k = std::min<int64_t>({(int64_t)P.free_slots.size(), r->frames - r->admitted,
+                                             (int64_t)(P.admit_cap - n)});
+        int32_t* slots = P.h_admit.as<int32_t>();
+        for (int64_t i = 0; i < k; ++i) {
+          const int32_t slot = P.free_slots.back();
+          P.free_slots.pop_back();
+          slots[n + i] = slot;
+          P.owner[slot] = {r, r->admitted + i};
+        }
+        admits[pi].push_back({r, r->admitted, k});
+        r->admitted += k;
+        sv->queued_frames -= k;
+        n += (int)k;
+        if (r->admitted == r->frames) P.queue.pop_front();
+      }
+      P.live += n;
+      any_live |= P.live > 0;
+    }
+  }
+  *idle = !any_live;
+  if (!any_live) return FECL_OK;
+  // admit: stage the LLR rows (requests keep their buffers alive until completion)
+  for (size_t pi = 0; pi < sv->pools.size(); ++pi) {
+    Pool& P = *sv->pools[pi];
+    int n = 0;
+    const int64_t N = P.h->code.N;
+    for (const Admit& a : admits[pi]) {
+      std::memcpy(P.h_stage.as<float>() + (int64_t)n * N, a.req->llr + a.f0 * N, (size_t)(a.n * N) * 4);
+      n += (int)a.n;
+    }
+    if (!n) continue;
+    HIP_TRY(hipMemcpyAsync(P.d_stage.p, P.h_stage.p, (size_t)n * N * 4, hipMemcpyHostToDevice, sv->st));
+    HIP_TRY(hipMemcpyAsync(P.d_admit.p, P.h_admit.p, (size_t)n * 4, hipMemcpyHostToDevice, sv->st));
+    const uint8_t init = (uint8_t)((P.h->iterations == 1 ? PH_FINAL : PH_RUN) | PH_FRESH);
+    HIP_TRY(kernels_admit(P.lay, P.h->dc, P.s, P.d_stage.as<float>(), P.d_admit.as<int32_t>(), n, init, sv->st));
+  }
+  // step every pool that holds live frames
+  for (auto& pp : sv->pools) {
+    Pool& P = *pp;
+    if (!P.live) continue;
+    HIP_TRY(kernels_check(P.lay, P.h->dc, P.s, P.opts, P.h->code.max_row_degree, sv->st));
+    HIP_TRY(kernels_variable(P.lay, P.h->dc, P.s, P.out, P.h->code.max_var_degree, P.opts.fused_syn, sv->st));
+    HIP_TRY(kernels_syndrome_control(P.lay, P.h->dc, P.s, P.h->iterations, P.opts.fused_syn, -1,
+                                     P.h->code.max_row_degree, sv->st));
+    HIP_TRY(hipMemcpyAsync(P.h_phase.p, P.s.phase, (size_t)P.nslots, hipMemcpyDeviceToHost, sv->st));
+  }
+  HIP_TRY(hipStreamSynchronize(sv->st));
+  // retire finished frames
+  std::vector<int> retired(sv->pools.size(), 0);
+  for (size_t pi = 0; pi < sv->pools.size(); ++pi) {
+    Pool& P = *sv->pools[pi];
+    if (!P.live) continue;
+    const uint8_t* ph = P.h_phase.as<uint8_t>();
+    int32_t* list = P.h_ext_slots.as<int32_t>();
+    int n = 0;
+    for (int64_t slot = 0; slot < P.nslots; ++slot)
+      if (P.owner[slot].first && (ph[slot] & PH_MASK) == PH_DONE) list[n++] = (int32_t)slot;
+    retired[pi] = n;
+    if (!n) continue;
+    const int64_t K = P.h->code.K;
+    HIP_TRY(hipMemcpyAsync(P.d_ext_slots.p, list, (size_t)n * 4, hipMemcpyHostToDevice, sv->st));
+    HIP_TRY(kernels_extract(P.lay, P.h->dc, P.s, P.out, P.d_ext_slots.as<int32_t>(), n, P.d_ext_bits.as<uint8_t>(),
+                            P.d_ext_iters.as<int32_t>(), sv->st));
+    HIP_TRY(hipMemcpyAsync(P.h_ext_bits.p, P.d_ext_bits.p, (size_t)n * K, hipMemcpyDeviceToHost, sv->st));
+    HIP_TRY(hipMemcpyAsync(P.h_ext_iters.p, P.d_ext_iters.p, (size_t)n * 4, hipMemcpyDeviceToHost, sv->st));
+  }
+  HIP_TRY(hipStreamSynchronize(sv->st));
+  const Clock::time_point now = Clock::now();
+  bool notify = false;
+  {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    for (size_t pi = 0; pi < sv->pools.size(); ++pi) {
+      Pool& P = *sv->pools[pi];
+      const int64_t K = P.h->code.K;
+      const int32_t* list = P.h_ext_slots.as<int32_t>();
+      for (int i = 0; i < retired[pi]; ++i) {
+        const int32_t slot = list[i];
+        Request* r = P.owner[slot].first;
+        const int64_t f = P.owner[slot].second;
+        std::memcpy(r->msg + f * K, P.h_ext_bits.as<uint8_t>() + (int64_t)i * K, (size_t)K);
+        if (r->iters) r->iters[f] = P.h_ext_iters.as<int32_t>()[i];
+        P.owner[slot] = {nullptr, 0};
+        P.free_slots.push_back(slot);
+        --P.live;
+        ++sv->frames_done;
+        if (++r->done == r->frames) {
+          r->finished = true;
+          r->t_done = now;
+          notify = true;
+        }
+      }
+    }
+    ++sv->steps;
+  }
+  if (notify) sv->done_cv.notify_all();
+  return FECL_OK;
+}
+
+void server_loop(fecl_ldpc_server* sv) {
+  (void)hipSetDevice(sv->device);
+  for (;;) {
+    {
+      std::unique_lock<std::mutex> lk(sv->mu);
+      if (sv->stop) return;
+    }
+    bool idle = false;
+    const int rc = server_round(sv, &idle);
+    if (rc != FECL_OK) {
+      std::lock_guard<std::mutex> lk(sv->mu);
+      sv->error = rc;
+      sv->error_msg = fecl_last_error();
+      sv->stop = true;
+      sv->done_cv.notify_all();
+      return;
+    }
+    if (idle) {
+      std::unique_lock<std::mutex> lk(sv->mu);
+      sv->work_cv.wait_for(lk, std::chrono::milliseconds(5), [&] {
+        if (sv->stop) return true;
+        for (auto& p : sv->pools)
+          if (!p->queue.empty()) return true;
+        return false;
+      });
+    }
+  }
+}
+
+// Device bytes one slot needs in a pool of this layout (messages, channel, bookkeeping).
+int64_t slot_bytes(const fecl_ldpc* h, const Layout& lay) {
+  const Code& c = h->code;
+  return c.E * 4 / lay.pw() + c.N * 4 + (c.N + c.K) * 8 / 64 + 8;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fecl_ldpc_server_create(fecl_ldpc* const* handles, int32_t count, const int64_t* slots, int64_t memory_bytes,
+                            fecl_ldpc_server** out) {
+  if (!out || !handles || count < 1) return fail(FECL_INVALID_ARGUMENT, "invalid arguments");
+  *out = nullptr;
+  std::unique_ptr<fecl_ldpc_server> sv(new fecl_ldpc_server());
+  sv->device = handles[0]->device;
+  HIP_TRY(hipSetDevice(sv->device));
+  // Admission by device memory: pools without an explicit slot count share the budget (by
+  // default half of the free memory) in proportion to one slot's footprint.
+  if (memory_bytes <= 0) {
+    size_t free_b = 0, total_b = 0;
+    HIP_TRY(hipMemGetInfo(&free_b, &total_b));
+    memory_bytes = (int64_t)(free_b / 2);
+  }
+  int unset = 0;
+  for (int32_t i = 0; i < count; ++i) {
+    if (!handles[i] || handles[i]->device != sv->device) return fail(FECL_INVALID_ARGUMENT, "invalid handle");
+    if (!slots || slots[i] <= 0) ++unset;
+  }
+  for (int32_t i = 0; i < count; ++i) {
+    fecl_ldpc* h = handles[i];
+    std::unique_ptr<Pool> P(new Pool());
+    P->h = h;
+    P->lay = layout_of(h);
+    P->opts = make_opts(h);
+    const int64_t tw = P->lay.tw();
+    int64_t want = (slots && slots[i] > 0) ? slots[i] : (memory_bytes / unset) / slot_bytes(h, P->lay);
+    int64_t tiles = std::max<int64_t>(1, want / tw);
+    P->nslots = tiles * tw;
+    if (int rc = prepare(h, &P->ws, P->lay, P->nslots, false, &P->s, &P->out)) return rc;
+    HIP_TRY(kernels_fill_i32(P->s.iter, P->nslots, 0, nullptr));
+    HIP_TRY(hipMemset(P->s.phase, PH_FREE, (size_t)P->nslots));
+    HIP_TRY(hipMemset(P->s.tile_flags, 0, (size_t)tiles * 4));
+    HIP_TRY(hipMemset(P->s.fail, 0, (size_t)tiles * P->lay.fpl() * 8));
+    P->admit_cap = (int)std::min<int64_t>(P->nslots, 128);
+    const Code& c = h->code;
+    HIP_TRY(P->d_stage.ensure((size_t)P->admit_cap * c.N * 4));
+    HIP_TRY(P->d_admit.ensure((size_t)P->admit_cap * 4));
+    HIP_TRY(P->d_ext_slots.ensure((size_t)P->nslots * 4));
+    HIP_TRY(P->d_ext_bits.ensure((size_t)P->nslots * c.K));
+    HIP_TRY(P->d_ext_iters.ensure((size_t)P->nslots * 4));
+    HIP_TRY(P->h_stage.ensure((size_t)P->admit_cap * c.N * 4));
+    HIP_TRY(P->h_admit.ensure((size_t)P->admit_cap * 4));
+    HIP_TRY(P->h_ext_slots.ensure((size_t)P->nslots * 4));
+    HIP_TRY(P->h_ext_bits.ensure((size_t)P->nslots * c.K));
+    HIP_TRY(P->h_ext_iters.ensure((size_t)P->nslots * 4));
+    HIP_TRY(P->h_phase.ensure((size_t)P->nslots));
+    P->owner.assign((size_t)P->nslots, {nullptr, 0});
+    for (int64_t s = P->nslots - 1; s >= 0; --s) P->free_slots.push_back((int32_t)s);
+    sv->pools.push_back(std::move(P));
+  }
+  HIP_TRY(hipDeviceSynchronize());
+  HIP_TRY(hipStreamCreateWithFlags(&sv->st, hipStreamNonBlocking));
+  fecl_ldpc_server* raw = sv.get();
+  sv->worker = std::thread(server_loop, raw);
+  *out = sv.release();
+  return FECL_OK;
+}
+
+int fecl_ldpc_server_slots(const fecl_ldpc_server* sv, int32_t code, int64_t* slots) {
+  if (!sv || !slots || code < 0 || code >= (int32_t)sv->pools.size()) return fail(FECL_INVALID_ARGUMENT, "invalid arguments");
+  *slots = sv->pools[code]->nslots;
+  return FECL_OK;
+}
+
+int fecl_ldpc_server_submit(fecl_ldpc_server* sv, int32_t code, const float* llr, int64_t frames, uint8_t* msg,
+                            int32_t* iterations, uint64_t* ticket) {
+  if (!sv || !ticket || code < 0 || code >= (int32_t)sv->pools.size()) return fail(FECL_INVALID_ARGUMENT, "invalid arguments");
+  if (frames < 1 || !llr || !msg) return fail(FECL_INVALID_ARGUMENT, "Invalid size for parity");
+  std::unique_ptr<Request> r(new Request());
+  r->pool = code;
+  r->llr = llr;
+  r->frames = frames;
+  r->msg = msg;
+  r->iters = iterations;
+  r->t_submit = Clock::now();
+  {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    if (sv->stop) return fail(sv->error ? sv->error : FECL_DEVICE_ERROR, sv->error_msg.empty() ? "server stopped" : sv->error_msg);
+    r->id = sv->next_id++;
+    *ticket = r->id;
+    sv->pools[code]->queue.push_back(r.get());
+    sv->queued_frames += frames;
+    sv->requests[r->id] = std::move(r);
+  }
+  sv->work_cv.notify_one();
+  return FECL_OK;
+}
+
+int fecl_ldpc_server_wait(fecl_ldpc_server* sv, uint64_t ticket, double timeout_s, int32_t* done, double* latency_s) {
+  if (!sv || !done) return fail(FECL_INVALID_ARGUMENT, "invalid arguments");
+  std::unique_lock<std::mutex> lk(sv->mu);
+  auto it = sv->requests.find(ticket);
+  if (it == sv->requests.end()) return fail(FECL_INVALID_ARGUMENT, "unknown ticket");
+  Request* r = it->second.get();
+  auto pred = [&] { return r->finished || sv->stop; };
+  if (timeout_s < 0) sv->done_cv.wait(lk, pred);
+  else sv->done_cv.wait_for(lk, std::chrono::duration<double>(timeout_s), pred);
+  *done = r->finished ? 1 : 0;
+  if (r->finished) {
+    if (latency_s) *latency_s = std::chrono::duration<double>(r->t_done - r->t_submit).count();
+    sv->requests.erase(it);
+    return FECL_OK;
+  }
+  if (sv->error) return fail(sv->error, sv->error_msg);
+  return FECL_OK;
+}
+
+int fecl_ldpc_server_stats(fecl_ldpc_server* sv, int64_t* frames_done, int64_t* steps, int64_t* queued_frames) {
+  if (!sv) return fail(FECL_INVALID_ARGUMENT, "null server");
+  std::lock_guard<std::mutex> lk(sv->mu);
+  if (frames_done) *frames_done = sv->frames_done;
+  if (steps) *steps = sv->steps;
+  if (queued_frames) *queued_frames = sv->queued_frames;
+  return FECL_OK;
+}
+
+int fecl_ldpc_server_destroy(fecl_ldpc_server* sv) {
+  delete sv;
+  return FECL_OK;
+}
+
+}  // extern "C"

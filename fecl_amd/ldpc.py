@@ -233,6 +233,10 @@ class Ldpc:
         """Pin the lane layout: 32-bit words per lane per edge (1, 2, 4; 0 = automatic)."""
         check(lib().fecl_ldpc_set_layout(self._h, int(words)))
 
+    def setFusedSyndrome(self, enable):
+        """Syndrome in the next check kernel (True) or in its own kernel (False, default)."""
+        check(lib().fecl_ldpc_set_fused_syndrome(self._h, int(bool(enable))))
+
     def layout(self):
         """(words per lane, frames per tile) of the current device layout."""
         w, tw = ctypes.c_int32(), ctypes.c_int32()
@@ -406,3 +410,65 @@ class BatchGraph:
             except Exception:
                 pass
             self._g = None
+
+
+class Server:
+    """Continuous-batching decode server (fecl_ldpc_server_*): one device slot pool per code,
+    requests admitted as slots free up, message state resident on the device.
+
+    decoders: Ldpc handles (kept alive by the server); slots: frames per pool (None = share
+    memory_bytes, by default half of the free device memory)."""
+
+    def __init__(self, decoders, slots=None, memory_bytes=0):
+        self._decoders = list(decoders)
+        n = len(self._decoders)
+        P = ctypes.c_void_p
+        hs = (P * n)(*[d._h.value for d in self._decoders])
+        sl = (ctypes.c_int64 * n)(*(slots if slots is not None else [0] * n))
+        self._s = P()
+        self._pending = {}
+        check(lib().fecl_ldpc_server_create(hs, n, sl, int(memory_bytes), ctypes.byref(self._s)))
+
+    def slots(self, code):
+        v = ctypes.c_int64()
+        check(lib().fecl_ldpc_server_slots(self._s, int(code), ctypes.byref(v)))
+        return v.value
+
+    def submit(self, code, llr):
+        """Queue frames (host float32 [frames * N]) of code index `code`; returns a ticket."""
+        d = self._decoders[code]
+        x = np.ascontiguousarray(llr, dtype=np.float32).ravel()
+        frames = d._frames(x.size)
+        msg = np.zeros(frames * d.msgSize(), np.uint8)
+        it = np.zeros(frames, np.int32)
+        t = ctypes.c_uint64()
+        check(lib().fecl_ldpc_server_submit(self._s, int(code), _p(x), frames, _p(msg), _p(it), ctypes.byref(t)))
+        self._pending[t.value] = (x, msg, it)
+        return t.value
+
+    def wait(self, ticket, timeout=-1.0):
+        """(msg uint8 [frames*K], iterations int32 [frames], latency seconds), or None on timeout."""
+        done, lat = ctypes.c_int32(), ctypes.c_double()
+        check(lib().fecl_ldpc_server_wait(self._s, ctypes.c_uint64(ticket), float(timeout), ctypes.byref(done),
+                                          ctypes.byref(lat)))
+        if not done.value:
+            return None
+        _, msg, it = self._pending.pop(ticket)
+        return msg, it, lat.value
+
+    def stats(self):
+        a, b, c = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        check(lib().fecl_ldpc_server_stats(self._s, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return {"frames_done": a.value, "steps": b.value, "queued_frames": c.value}
+
+    def close(self):
+        s = getattr(self, "_s", None)
+        if s is not None and s.value:
+            lib().fecl_ldpc_server_destroy(s)
+            self._s = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

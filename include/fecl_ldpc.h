@@ -133,6 +133,10 @@ int fecl_ldpc_profile_read(fecl_ldpc* h, double* ms /*[3]*/, int64_t* launches /
  * words 1, 2, 4 (-1 = not eligible).  Neither may run concurrently with a decode on the
  * same handle. */
 int fecl_ldpc_set_layout(fecl_ldpc* h, int32_t words);
+/* Syndrome schedule (no counterpart in the reference; results are identical either way):
+ * 0 (default) = a separate kernel after each variable pass; 1 = folded into the next check
+ * kernel (used only when the scaling factors do not vary with the iteration). */
+int fecl_ldpc_set_fused_syndrome(fecl_ldpc* h, int32_t enable);
 int fecl_ldpc_get_layout(const fecl_ldpc* h, int32_t* words, int32_t* frames_per_tile);
 int fecl_ldpc_autotune(fecl_ldpc* h, int64_t frames, int32_t probe_iterations, int32_t* best_words,
                        double* ms);
@@ -150,6 +154,31 @@ int fecl_ldpc_graph_create(fecl_ldpc* const* handles, const int64_t* frames, int
                            fecl_ldpc_graph** out);
 int fecl_ldpc_graph_launch(fecl_ldpc_graph* g, void* stream);
 int fecl_ldpc_graph_destroy(fecl_ldpc_graph* g);
+
+/* Continuous-batching decode server (no counterpart in the reference).  Each of the `count`
+ * codes gets a pool of device frame slots: slots[i] frames, or, where slots is NULL or
+ * slots[i] <= 0, a share of memory_bytes (<= 0: half of the device's free memory) sized by one
+ * slot's footprint.  A worker thread repeats: admit queued frames into free slots, run one
+ * check/variable/syndrome step on every pool holding live frames, retire finished frames.
+ * Requests arriving while others run join at the next step; each frame's result equals a
+ * batch decode of it (frames share no state).  Handles must outlive the server and keep
+ * their options while it runs.
+ *   submit: host LLRs [frames][N] (FP32) of code `code`; msg [frames][K] (and iterations
+ *           [frames], nullable) receive the results; all three stay valid until wait()
+ *           reports done.
+ *   wait:   blocks up to timeout_s (< 0: forever); done = 1 and latency_s = completion -
+ *           submission time once finished (the ticket is then released). */
+typedef struct fecl_ldpc_server fecl_ldpc_server;
+int fecl_ldpc_server_create(fecl_ldpc* const* handles, int32_t count, const int64_t* slots,
+                            int64_t memory_bytes, fecl_ldpc_server** out);
+int fecl_ldpc_server_slots(const fecl_ldpc_server* s, int32_t code, int64_t* slots);
+int fecl_ldpc_server_submit(fecl_ldpc_server* s, int32_t code, const float* llr, int64_t frames,
+                            uint8_t* msg, int32_t* iterations, uint64_t* ticket);
+int fecl_ldpc_server_wait(fecl_ldpc_server* s, uint64_t ticket, double timeout_s, int32_t* done,
+                          double* latency_s);
+int fecl_ldpc_server_stats(fecl_ldpc_server* s, int64_t* frames_done, int64_t* steps,
+                           int64_t* queued_frames);
+int fecl_ldpc_server_destroy(fecl_ldpc_server* s);
 
 /* Releases the cached device workspaces of a handle (they are reused across calls). */
 int fecl_ldpc_trim(fecl_ldpc* h);

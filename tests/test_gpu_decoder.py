@@ -33,11 +33,13 @@ def dvbs2(n=64800, rate=(1, 2)):
     return oc, F.CheckMatrix(oc.row_ptr, oc.col, oc.N)
 
 
-def gpu_decoder(H, alg, iters, sf=1.0, fmt=F.MsgFormat.F32, words=0):
+def gpu_decoder(H, alg, iters, sf=1.0, fmt=F.MsgFormat.F32, words=0, fused=False):
     opt = F.DecoderOptions().algorithm(alg).iterations(iters).scalingFactor(sf).msgFormat(fmt)
     dec = F.Ldpc(H, opt)
     if words:
         dec.setLayout(words)
+    if fused:
+        dec.setFusedSyndrome(True)
     return dec
 
 
@@ -59,15 +61,16 @@ def assert_bit_exact(gpu, ref, frames, K, N, E):
 
 
 @pytest.mark.parametrize("alg", [F.Approximate, F.Linear])
-@pytest.mark.parametrize("iters,sf,words", [(1, 1.0, 1), (2, 1.0, 4), (5, 1.0, 2), (30, 1.0, 1), (30, 1.0, 4),
-                                            (30, 0.75, 2), (30, 0.9, 4), (5, [0.5, 0.625, 0.75, 0.875, 1.0], 1),
-                                            (5, [0.5, 0.625, 0.75, 0.875, 1.0], 4)])
-def test_gallager_fp32_bit_exact(alg, iters, sf, words):
+@pytest.mark.parametrize("iters,sf,words,fused", [(1, 1.0, 1, False), (2, 1.0, 4, True), (5, 1.0, 2, False),
+                                                  (30, 1.0, 1, True), (30, 1.0, 4, False), (30, 0.75, 2, True),
+                                                  (30, 0.9, 4, False), (5, [0.5, 0.625, 0.75, 0.875, 1.0], 1, True),
+                                                  (5, [0.5, 0.625, 0.75, 0.875, 1.0], 4, False)])
+def test_gallager_fp32_bit_exact(alg, iters, sf, words, fused):
     oc, H = gallager()
     llr = np.concatenate([gallager_llr(oc.N, oc.K, 3, 2.0, 11), gallager_llr(oc.N, oc.K, 2, -1.0, 12)])
     llr32 = llr.astype(np.float32).astype(np.float64)
     ref = oracle_run(oc, llr32, alg, iters, sf)
-    dec = gpu_decoder(H, alg, iters, {0: sf} if isinstance(sf, list) else sf, words=words)
+    dec = gpu_decoder(H, alg, iters, {0: sf} if isinstance(sf, list) else sf, words=words, fused=fused)
     out = dec.soDecode(llr32)
     assert_bit_exact(out, ref, llr.shape[0], oc.K, oc.N, oc.E)
     hard, it = dec.decode(llr32, iterations=True)
@@ -75,12 +78,13 @@ def test_gallager_fp32_bit_exact(alg, iters, sf, words):
     assert np.array_equal(it, ref["iters"])
 
 
-@pytest.mark.parametrize("frames,words", [(1, 4), (63, 1), (65, 2), (130, 4), (300, 4)])
-def test_dvbs2_minsum_fp32_ragged_frames(frames, words):
+@pytest.mark.parametrize("frames,words,fused", [(1, 4, False), (63, 1, True), (65, 2, False), (130, 4, True),
+                                                (300, 4, False)])
+def test_dvbs2_minsum_fp32_ragged_frames(frames, words, fused):
     oc, H = dvbs2()
     llr = dvbs2_llr(frames, 1.4, seed=100 + frames).astype(np.float32).astype(np.float64)
     ref = oracle_run(oc, llr, F.Approximate, 50)
-    dec = gpu_decoder(H, F.Approximate, 50, words=words)
+    dec = gpu_decoder(H, F.Approximate, 50, words=words, fused=fused)
     assert dec.layout()[0] == words
     out = dec.soDecode(llr)
     assert_bit_exact(out, ref, frames, oc.K, oc.N, oc.E)

@@ -1,0 +1,80 @@
+"""Continuous-batching server: requests of mixed codes and sizes, submitted while others run,
+each decoded exactly as it is alone (frames share no state), plus admission by memory."""
+import threading
+import time
+
+import numpy as np
+import pytest
+
+import fecl_amd as F
+from oracle import oracle as O
+from tests.golden.vectors import dvbs2_llr
+
+pytestmark = pytest.mark.gpu
+
+CODES = [(64800, (1, 2), 1.6), (64800, (3, 4), 3.3), (16200, (1, 2), 2.2)]
+
+
+def decoders(fmt=F.MsgFormat.F16, iters=50):
+    out = []
+    for n, rate, _ in CODES:
+        H = F.DvbS2.matrix(n, rate)
+        out.append(F.Ldpc(H, F.DecoderOptions().algorithm(F.Approximate).iterations(iters).msgFormat(fmt)))
+    return out
+
+
+def test_server_results_same_alone_and_in_the_mix():
+    decs = decoders()
+    rng = np.random.default_rng(5)
+    pool = {i: dvbs2_llr(200, CODES[i][2], seed=50 + i, rate=CODES[i][1], n=CODES[i][0]).astype(np.float32)
+            for i in range(len(CODES))}
+    reqs = []
+    for _ in range(40):
+        i = int(rng.integers(0, len(CODES)))
+        f = int(rng.integers(1, 40))
+        o = int(rng.integers(0, 200 - f))
+        reqs.append((i, pool[i][o:o + f]))
+    # small pools so that requests wait for slots and join mid-flight
+    tw = [d.layout()[1] for d in decs]
+    sv = F.Server(decs, slots=[tw[0], tw[1], 2 * tw[2]])
+    results = [None] * len(reqs)
+
+    def submitter():
+        for k, (i, x) in enumerate(reqs):
+            results[k] = sv.submit(i, x)
+            time.sleep(0.002)
+
+    th = threading.Thread(target=submitter)
+    th.start()
+    th.join()
+    got = [sv.wait(t, timeout=120.0) for t in results]
+    st = sv.stats()
+    sv.close()
+    assert all(g is not None for g in got), "server did not finish"
+    assert st["frames_done"] == sum(x.shape[0] for _, x in reqs)
+    for (i, x), (msg, it, lat) in zip(reqs, got):
+        alone, ait = decs[i].decode(x.astype(np.float64), iterations=True)
+        assert np.array_equal(msg, alone) and np.array_equal(it, ait)
+        assert lat > 0
+    # and against the oracle for one request per code
+    for i in range(len(CODES)):
+        k = next(k for k, (j, _) in enumerate(reqs) if j == i)
+        x = reqs[k][1]
+        n, rate, _ = CODES[i]
+        ref = O.decode(O.dvbs2(n, *rate), x.astype(np.float64), O.APPROXIMATE, 50, prec=32, msg_fmt=O.MSG_F16,
+                       want=("hard", "iters"))
+        assert np.array_equal(got[k][0].reshape(ref["hard"].shape), ref["hard"])
+        assert np.array_equal(got[k][1], ref["iters"])
+
+
+def test_server_admission_by_memory_budget():
+    decs = decoders()
+    budget = 256 << 20
+    sv = F.Server(decs, memory_bytes=budget)
+    total = 0
+    for i, d in enumerate(decs):
+        slots = sv.slots(i)
+        assert slots >= d.layout()[1] and slots % d.layout()[1] == 0
+        total += slots
+    sv.close()
+    assert total > 0
