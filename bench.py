@@ -50,6 +50,193 @@ def make_batches(F, dec, H, frames, seed, device):
     return batches
 
 
+# DVB-S2 normal rates of the uneven-batching and serving workloads, with the Eb/N0 (dB) where
+# this decoder (min-sum, FP16 messages, 50 iterations) has a frame error rate near 1e-3.
+RATE_POINTS = {(1, 4): 1.0, (1, 2): 1.4, (3, 5): 1.9, (2, 3): 2.2, (3, 4): 2.7, (5, 6): 3.4, (9, 10): 4.2}
+
+
+def codeword_pool(dec, frames, rng):
+    """Encoded frames of i.i.d. Bernoulli(0.5) info bits: (info [frames][K], codeword [frames][N])."""
+    import numpy as np
+    K, N = dec.msgSize(), dec.paritySize()
+    info = rng.integers(0, 2, (frames, K), dtype=np.uint8)
+    return info, dec.encode(info).reshape(frames, N)
+
+
+def awgn_llr_gpu(cw_dev, ebn0_dev, rate, gen):
+    """FeCl-convention FP32 LLRs of BPSK codewords over AWGN, one Eb/N0 per frame (GPU)."""
+    import torch
+    sigma2 = (1.0 / (2.0 * rate * torch.pow(10.0, ebn0_dev / 10.0))).unsqueeze(1)
+    s = 1.0 - 2.0 * cw_dev.to(torch.float32)
+    y = s + torch.randn(s.shape, generator=gen, device=cw_dev.device) * torch.sqrt(sigma2)
+    return (-2.0 * y / sigma2).to(torch.float32).contiguous()
+
+
+def run_uneven(args):
+    """Uneven batching: 7 normal rates, per-rate frame counts U[64,1024], per-frame Eb/N0 within
+    +-0.3 dB of the rate's ~1e-3 FER point; min-sum, FP16 messages; one hipGraph launch."""
+    import numpy as np
+    import torch
+    import fecl_amd as F
+    device = torch.device("cuda:0")
+    rng = np.random.default_rng(args.seed)
+    gen = torch.Generator(device=device)
+    gen.manual_seed(args.seed)
+    decs, llrs, infos, counts = [], [], [], []
+    for rate, e0 in RATE_POINTS.items():
+        n = int(rng.integers(64, 1025))
+        dec = F.Ldpc(F.DvbS2.matrix(64800, rate),
+                     F.DecoderOptions().algorithm(F.Approximate).iterations(args.iterations)
+                     .msgFormat(F.MsgFormat.F16))
+        dec.autotune(n, probe_iterations=3)
+        info, cw = codeword_pool(dec, min(n, 256), rng)
+        idx = np.arange(n) % info.shape[0]
+        ebn0 = torch.from_numpy(e0 + rng.uniform(-0.3, 0.3, n)).to(device)
+        llrs.append(awgn_llr_gpu(torch.from_numpy(cw[idx]).to(device), ebn0, dec.msgSize() / dec.paritySize(), gen))
+        infos.append(torch.from_numpy(info[idx]).to(device))
+        decs.append(dec)
+        counts.append(n)
+    g = F.BatchGraph(decs, llrs)
+    for _ in range(args.warmup):
+        g.launch()
+    torch.cuda.synchronize()
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        g.launch()
+        torch.cuda.synchronize()
+        lat.append(time.perf_counter() - ts)
+    total = time.perf_counter() - t0
+    # the same batches decoded by one eager decode() call per rate, for comparison
+    for _ in range(args.warmup):
+        for d, x in zip(decs, llrs):
+            d.decode(x)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        for d, x in zip(decs, llrs):
+            d.decode(x)
+    torch.cuda.synchronize()
+    eager = time.perf_counter() - t1
+    info_bits = sum(c * d.msgSize() for c, d in zip(counts, decs)) * args.steps
+    per_rate = {}
+    for (rate, _), d, c, m, it, inf in zip(RATE_POINTS.items(), decs, counts, g.msg, g.iterations, infos):
+        errs = int(torch.any(m.view(c, d.msgSize()) != inf, dim=1).sum())
+        per_rate[f"{rate[0]}/{rate[1]}"] = {"frames": c, "fer": round(errs / c, 5),
+                                            "mean_iterations": round(float(it.float().mean()), 2),
+                                            "lane_words": d.layout()[0]}
+    return {"metric": "decoded info Gb/s, uneven multi-rate batch in one hipGraph launch",
+            "value": round(info_bits / total / 1e9, 4), "unit": "Gb/s", "frames_per_launch": sum(counts),
+            "ms_per_launch": round(total / args.steps * 1e3, 3),
+            "eager_per_rate_calls_Gb_per_s": round(info_bits / eager / 1e9, 4),
+            "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
+                           "p99": round(float(np.percentile(lat, 99)) * 1e3, 3)},
+            "dtype": "f16", "per_rate": per_rate}
+
+
+class ServingSetup:
+    """Seven DVB-S2 normal-rate decoders (min-sum, FP16 messages) and per-rate host pools of
+    FP32 LLR frames at Eb/N0 within +-0.3 dB of each rate's ~1e-3 FER point."""
+
+    POOL = 768
+
+    def __init__(self, F, args, device):
+        import numpy as np
+        import torch
+        rng = np.random.default_rng(args.seed + 1)
+        gen = torch.Generator(device=device)
+        gen.manual_seed(args.seed + 1)
+        self.rates = list(RATE_POINTS)
+        self.decs, self.pools, self.infos = [], [], []
+        for rate in self.rates:
+            dec = F.Ldpc(F.DvbS2.matrix(64800, rate),
+                         F.DecoderOptions().algorithm(F.Approximate).iterations(args.iterations)
+                         .msgFormat(F.MsgFormat.F16))
+            info, cw = codeword_pool(dec, 256, rng)
+            idx = np.arange(self.POOL) % 256
+            ebn0 = torch.from_numpy(RATE_POINTS[rate] + rng.uniform(-0.3, 0.3, self.POOL)).to(device)
+            llr = awgn_llr_gpu(torch.from_numpy(cw[idx]).to(device), ebn0, dec.msgSize() / dec.paritySize(), gen)
+            self.pools.append(llr.cpu().numpy())
+            self.infos.append(info[idx])
+            self.decs.append(dec)
+
+
+def run_serving(F, setup, args, duration_s, offered_fps, seed):
+    """Serving: Poisson-arriving requests of U[1,256] frames, rate uniform over the 7 rates;
+    continuous batching in the library's server.  Arrivals are generated by this thread at
+    their scheduled times for duration_s; every result is then collected and checked."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    slots = [args.serve_slots] * len(setup.rates)
+    sv = F.Server(setup.decs, slots=slots)
+    lam = offered_fps / 128.5     # requests per second (mean request 128.5 frames)
+    tickets = []
+    t_start = time.perf_counter()
+    t_next = t_start
+    while True:
+        now = time.perf_counter()
+        if now - t_start >= duration_s:
+            break
+        if now < t_next:
+            time.sleep(min(t_next - now, 0.0005))
+            continue
+        t_next += rng.exponential(1.0 / lam)
+        ri = int(rng.integers(0, len(setup.rates)))
+        f = int(rng.integers(1, 257))
+        o = int(rng.integers(0, setup.POOL - f + 1))
+        tickets.append((ri, o, f, sv.submit(ri, setup.pools[ri][o:o + f])))
+    t_sub_end = time.perf_counter()
+    offered_frames = sum(f for _, _, f, _ in tickets)
+    lats, errs, frames, bits = [], 0, 0, 0
+    for ri, o, f, t in tickets:
+        r = sv.wait(t, timeout=600.0)
+        if r is None:
+            raise RuntimeError("serving request timed out")
+        msg, it, lat = r
+        lats.append(lat)
+        K = setup.decs[ri].msgSize()
+        errs += int(np.any(msg.reshape(f, K) != setup.infos[ri][o:o + f], axis=1).sum())
+        frames += f
+        bits += f * K
+    t_end = time.perf_counter()
+    st = sv.stats()
+    sv.close()
+    wall = t_end - t_start
+    return {"offered_frames_per_s": offered_fps,
+            "arrival_frames_per_s": round(offered_frames / (t_sub_end - t_start), 1),
+            "arrival_seconds": round(t_sub_end - t_start, 2), "requests": len(tickets), "frames": frames,
+            "throughput_frames_per_s": round(frames / wall, 1), "info_Gb_per_s": round(bits / wall / 1e9, 4),
+            "latency_ms": {"p50": round(float(np.percentile(lats, 50)) * 1e3, 2),
+                           "p99": round(float(np.percentile(lats, 99)) * 1e3, 2), "unit": "per request"},
+            "fer": round(errs / max(1, frames), 5), "server_rounds": st["steps"], "slots_per_rate": slots[0]}
+
+
+def serving_report(F, args, device, duration_s):
+    """The offered load (default 200k frames/s), then 70 % of the throughput it sustained, so
+    that the second latency figure is that of an unsaturated server."""
+    setup = ServingSetup(F, args, device)
+    sat = run_serving(F, setup, args, duration_s, args.offered_fps, args.seed + 2)
+    load = round(0.7 * sat["throughput_frames_per_s"], -2)
+    under = run_serving(F, setup, args, duration_s, load, args.seed + 3)
+    return {"workload": "Poisson arrivals, requests of U[1,256] frames, rate uniform over DVB-S2 normal "
+                        "{1/4,1/2,3/5,2/3,3/4,5/6,9/10}, Eb/N0 within +-0.3 dB of each rate's ~1e-3 FER point, "
+                        f"min-sum, f16 messages, {args.iterations} max iterations",
+            "at_offered": sat, "at_70pct_of_sustained": under}
+
+
+def copy_reference(device, gib=1, reps=5):
+    """A plain device-to-device copy of gib GiB (the PMC byte-counter calibration kernel)."""
+    import torch
+    n = gib << 28
+    a = torch.ones(n, dtype=torch.float32, device=device)
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    return {"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "reps": reps}
+
+
 def fp64_parity(llr_host, info_host, iters, threads, gpu_hard):
     """FER parity on the sample: FeCl's FP64 decoder (the oracle's bit-identical restatement)
     against the GPU's hard decisions on the same frames."""
@@ -58,11 +245,12 @@ def fp64_parity(llr_host, info_host, iters, threads, gpu_hard):
     code = O.dvbs2(64800, 1, 2)
     ref = O.decode(code, llr_host.astype(np.float64), O.APPROXIMATE, iters, prec=64,
                    want=("hard",), threads=threads)
-    fer64 = float(np.mean(np.any(ref["hard"] != info_host, axis=1)))
-    fer_gpu = float(np.mean(np.any(gpu_hard != info_host, axis=1)))
-    agree = float(np.mean(np.all(gpu_hard == ref["hard"], axis=1)))
-    return {"fer_fp64": fer64, "fer_gpu": fer_gpu, "frames_identical_to_fp64": agree,
-            "bit_mismatches_vs_fp64": int(np.sum(gpu_hard != ref["hard"]))}
+    ok64 = np.all(ref["hard"] == info_host, axis=1)
+    same = np.all(gpu_hard == ref["hard"], axis=1)
+    return {"frames": int(len(ok64)), "fer_fp64": float(1.0 - ok64.mean()),
+            "fer_gpu": float(np.mean(np.any(gpu_hard != info_host, axis=1))),
+            "frames_identical_to_fp64": float(same.mean()),
+            "fp64_decoded_frames_identical": float(same[ok64].mean()) if ok64.any() else None}
 
 
 def cpu_baseline(llr_host, info_host, iters, threads):
@@ -104,11 +292,16 @@ def main():
     ap.add_argument("--iterations", type=int, default=50)
     ap.add_argument("--msg", choices=["f32", "f16", "bf16"], default="f32")
     ap.add_argument("--alg", choices=["minsum", "linear", "exact"], default="minsum")
-    ap.add_argument("--cpu-frames", type=int, default=128)
+    ap.add_argument("--cpu-frames", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--words", type=int, default=-1,
                     help="lane layout: 1, 2, 4 words per lane per edge; 0 automatic; -1 autotune")
+    ap.add_argument("--config", choices=["batch", "uneven", "serving", "copy"], default="batch")
+    ap.add_argument("--serve-seconds", type=float, default=2.0)
+    ap.add_argument("--serve-slots", type=int, default=2048)
+    ap.add_argument("--offered-fps", type=float, default=200000.0)
+    ap.add_argument("--no-serving", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the decoder's frames are independent, so N GPUs would "
@@ -124,6 +317,15 @@ def main():
 
     device = torch.device("cuda:0")
     torch.cuda.set_device(device)
+    if args.config == "uneven":
+        print(json.dumps(run_uneven(args)), flush=True)
+        return
+    if args.config == "serving":
+        print(json.dumps(serving_report(F, args, device, args.serve_seconds)), flush=True)
+        return
+    if args.config == "copy":
+        print(json.dumps(copy_reference(device)), flush=True)
+        return
     alg = {"minsum": F.Approximate, "linear": F.Linear, "exact": F.Exact}[args.alg]
     fmt = {"f32": F.MsgFormat.F32, "f16": F.MsgFormat.F16, "bf16": F.MsgFormat.BF16}[args.msg]
     H = F.DvbS2.matrix(64800, 0.5)
@@ -238,14 +440,25 @@ def main():
         print(f"[bench] CPU baseline: {nf} frames on {threads} threads", file=sys.stderr, flush=True)
         cb = cpu_baseline(llr_host, info_host, args.iterations, threads)
         if args.alg == "minsum":
-            gpu_hard = out[2][0].view(args.frames, K)[:nf].cpu().numpy() if 2 in out else None
-            if gpu_hard is not None:
-                result["fer_parity_sample"] = fp64_parity(llr_host, info_host, args.iterations, threads, gpu_hard)
+            # FP64 FeCl decoder vs this run's last decode of the 1.0 and 1.4 dB batches
+            par = {}
+            for bi in (2, 4):
+                if bi not in out:
+                    continue
+                npar = min(256, args.frames)
+                b = batches[bi]
+                par[f"{b['ebn0']} dB"] = fp64_parity(
+                    b["llr"][:npar].cpu().numpy().astype(np.float64), b["info"][:npar].cpu().numpy(),
+                    args.iterations, threads, out[bi][0].view(args.frames, K)[:npar].cpu().numpy())
+            result["fer_parity_sample"] = par
         result["cpu_baseline"] = {"value": round(nf * K / cb["seconds"] / 1e9, 6), "unit": "Gb/s",
                                   "cores": threads, "kind": cb["kind"],
                                   "sample": f"{nf} frames of the Eb/N0=1.0 dB batch, FP64 min-sum, "
                                             f"{args.iterations} max iterations, {cb['seconds']:.2f} s wall",
                                   "cpu": _cpu_model()}
+    if not args.no_serving:
+        print(f"[bench] serving loop: {args.serve_seconds} s of arrivals", file=sys.stderr, flush=True)
+        result["serving"] = serving_report(F, args, device, args.serve_seconds)
     print(json.dumps(result), flush=True)
 
 
