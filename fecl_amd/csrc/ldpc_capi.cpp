@@ -268,6 +268,16 @@ int run_batch(fecl_ldpc* h, Workspace* ws, hipStream_t st, const BatchIO& io, in
   return FECL_OK;
 }
 
+int run_passes(fecl_ldpc* h, Workspace* ws, hipStream_t st, const BatchIO& io, int64_t frames, int64_t frame0,
+               bool capture) {
+  const int64_t tw = layout_of(h).tw();
+  int64_t pass = h->pass_frames > 0 ? h->pass_frames : kMaxPassFrames;
+  pass = std::max<int64_t>(tw, (pass + tw - 1) / tw * tw);
+  for (int64_t f = 0; f < frames; f += pass)
+    if (int rc = run_batch(h, ws, st, io, std::min(pass, frames - f), frame0 + f, capture)) return rc;
+  return FECL_OK;
+}
+
 int check_handle(const fecl_ldpc* h) {
   if (!h) return fail(FECL_INVALID_ARGUMENT, "null handle");
   return FECL_OK;
@@ -319,7 +329,7 @@ int host_decode(fecl_ldpc* h, const double* llr, const double* syst_in, const do
     io.res.syst = syst ? (void*)(ob + o_syst) : nullptr;
     io.res.parity = parity ? (void*)(ob + o_par) : nullptr;
     io.res.state = state ? (void*)(ob + o_state) : nullptr;
-    if (int rc = run_batch(h, ws.get(), st, io, n, 0, false)) return rc;
+    if (int rc = run_passes(h, ws.get(), st, io, n, 0, false)) return rc;
     if (msg_bits) HIP_TRY(hipMemcpyAsync(msg_bits + f0 * c.K, ob + o_bits, (size_t)n * c.K, hipMemcpyDeviceToHost, st));
     if (iterations) HIP_TRY(hipMemcpyAsync(iterations + f0, ob + o_iter, (size_t)n * 4, hipMemcpyDeviceToHost, st));
     if (msg) HIP_TRY(hipMemcpyAsync(msg + f0 * c.K, ob + o_msg, (size_t)n * c.K * 8, hipMemcpyDeviceToHost, st));
@@ -344,18 +354,14 @@ int device_decode(fecl_ldpc* h, const float* llr, const float* syst_in, const fl
   struct Back { fecl_ldpc* h; std::unique_ptr<Workspace>* ws; ~Back() { release(h, std::move(*ws)); } } back{h, &ws};
   hipStream_t st = (hipStream_t)stream;
   HIP_TRY(hipStreamWaitEvent(st, ws->done, 0));
-  const int64_t chunk = std::min<int64_t>(frames, kMaxChunkFrames);
-  for (int64_t f0 = 0; f0 < frames; f0 += chunk) {
-    const int64_t n = std::min(chunk, frames - f0);
-    BatchIO io{};
-    io.llr = llr;
-    io.syst = syst_in;
-    io.state = state_in;
-    io.dbl_in = false;
-    io.dbl_out = false;
-    io.res = BatchResults{msg_bits, iterations, msg, syst, parity, state};
-    if (int rc = run_batch(h, ws.get(), st, io, n, f0, false)) return rc;
-  }
+  BatchIO io{};
+  io.llr = llr;
+  io.syst = syst_in;
+  io.state = state_in;
+  io.dbl_in = false;
+  io.dbl_out = false;
+  io.res = BatchResults{msg_bits, iterations, msg, syst, parity, state};
+  if (int rc = run_passes(h, ws.get(), st, io, frames, 0, false)) return rc;
   HIP_TRY(hipEventRecord(ws->done, st));
   return FECL_OK;
 }
@@ -565,6 +571,14 @@ int fecl_ldpc_set_fused_syndrome(fecl_ldpc* h, int32_t enable) {
   return FECL_OK;
 }
 
+int fecl_ldpc_set_pass_frames(fecl_ldpc* h, int64_t frames) {
+  if (int rc = check_handle(h)) return rc;
+  if (frames < 0) return fail(FECL_INVALID_ARGUMENT, "frames per pass must be >= 0");
+  std::lock_guard<std::mutex> lk(h->mu);
+  h->pass_frames = frames;
+  return FECL_OK;
+}
+
 int fecl_ldpc_get_layout(const fecl_ldpc* h, int32_t* words, int32_t* frames_per_tile) {
   if (int rc = check_handle(h)) return rc;
   const Layout lay = layout_of(h);
@@ -701,7 +715,7 @@ int fecl_ldpc_graph_create(fecl_ldpc* const* handles, const int64_t* frames, int
     io.llr = llr[i];
     io.res.msg_bits = msg[i];
     io.res.iterations = iterations ? iterations[i] : nullptr;
-    rc = run_batch(handles[i], g->ws[i].get(), g->streams[i], io, frames[i], 0, /*capture=*/true);
+    rc = run_passes(handles[i], g->ws[i].get(), g->streams[i], io, frames[i], 0, /*capture=*/true);
     if (rc != FECL_OK) break;
     e = hipEventRecord(g->events[i + 1], g->streams[i]);
     if (e == hipSuccess) e = hipStreamWaitEvent(g->capture, g->events[i + 1], 0);

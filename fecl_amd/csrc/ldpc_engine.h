@@ -92,6 +92,7 @@ struct fecl_ldpc {
   // and sends few atomics, while the fused form puts one device-coherent load per row on the
   // tile's fail word.
   bool fused_syndrome = false;
+  int64_t pass_frames = 0;  // frames per pass through the iteration loop; 0 = kMaxPassFrames
   std::mutex mu;
   std::vector<std::unique_ptr<fecl::engine::Workspace>> pool;
   // kernel timing (fecl_ldpc_profile): check, variable, syndrome+control
@@ -121,9 +122,13 @@ Layout layout_of(const fecl_ldpc* h);
 DevOpts make_opts(const fecl_ldpc* h);
 // Sizes ws for `frames` frames of h's layout and returns the device views.
 int prepare(fecl_ldpc* h, Workspace* ws, const Layout& lay, int64_t frames, bool soft, DevState* sp, DevOut* op);
-// Decodes frames [frame0, frame0 + frames) of io (see ldpc_capi.cpp).
+constexpr int64_t kMaxPassFrames = 8192;
+// Decodes frames [frame0, frame0 + frames) of io in one pass (see ldpc_capi.cpp).
 int run_batch(fecl_ldpc* h, Workspace* ws, hipStream_t st, const BatchIO& io, int64_t frames, int64_t frame0,
               bool capture);
+// Decodes frames [frame0, frame0 + frames) of io in consecutive passes of h->pass_frames.
+int run_passes(fecl_ldpc* h, Workspace* ws, hipStream_t st, const BatchIO& io, int64_t frames, int64_t frame0,
+               bool capture);
 
 }  // namespace engine
 }  // namespace fecl

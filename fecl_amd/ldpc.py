@@ -237,6 +237,10 @@ class Ldpc:
         """Syndrome in the next check kernel (True) or in its own kernel (False, default)."""
         check(lib().fecl_ldpc_set_fused_syndrome(self._h, int(bool(enable))))
 
+    def setPassFrames(self, frames):
+        """Decode batches in passes of at most `frames` frames (0 = 8192); results are the same."""
+        check(lib().fecl_ldpc_set_pass_frames(self._h, int(frames)))
+
     def layout(self):
         """(words per lane, frames per tile) of the current device layout."""
         w, tw = ctypes.c_int32(), ctypes.c_int32()

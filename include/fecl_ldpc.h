@@ -138,6 +138,12 @@ int fecl_ldpc_set_layout(fecl_ldpc* h, int32_t words);
  * kernel (used only when the scaling factors do not vary with the iteration). */
 int fecl_ldpc_set_fused_syndrome(fecl_ldpc* h, int32_t enable);
 int fecl_ldpc_get_layout(const fecl_ldpc* h, int32_t* words, int32_t* frames_per_tile);
+/* Frames per pass (no counterpart in the reference; results are identical for any value).
+ * A batch is decoded in consecutive passes of at most this many frames (rounded up to whole
+ * tiles), each pass running all of its iterations before the next starts, so that a pass's
+ * messages and channel values can stay in the device's last-level cache between
+ * iterations.  0 (default) = 8192 frames per pass. */
+int fecl_ldpc_set_pass_frames(fecl_ldpc* h, int64_t frames);
 int fecl_ldpc_autotune(fecl_ldpc* h, int64_t frames, int32_t probe_iterations, int32_t* best_words,
                        double* ms);
 

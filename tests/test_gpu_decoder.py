@@ -92,14 +92,17 @@ def test_dvbs2_minsum_fp32_ragged_frames(frames, words, fused):
     assert np.array_equal(hard.reshape(frames, oc.K), ref["hard"])
 
 
-def test_dvbs2_minsum_fp32_1024_frames_device_path():
-    """The config-2 shape: 1024 frames of DVB-S2 r1/2, FP32 LLRs in HBM, min-sum, 50 iterations."""
+@pytest.mark.parametrize("pass_frames", [0, 256, 300])
+def test_dvbs2_minsum_fp32_1024_frames_device_path(pass_frames):
+    """The config-2 shape: 1024 frames of DVB-S2 r1/2, FP32 LLRs in HBM, min-sum, 50 iterations;
+    whole batch in one pass, or in passes of 256 frames and of 300 frames (two tiles)."""
     import torch
     oc, H = dvbs2()
     rng_frames = [dvbs2_llr(256, e, seed=300 + i) for i, e in enumerate((0.6, 1.0, 1.4, 2.0))]
     llr = np.concatenate(rng_frames).astype(np.float32)
     ref = oracle_run(oc, llr.astype(np.float64), F.Approximate, 50, want=("hard", "iters"), threads=16)
-    dec = gpu_decoder(H, F.Approximate, 50)
+    dec = gpu_decoder(H, F.Approximate, 50, words=4)
+    dec.setPassFrames(pass_frames)
     d_llr = torch.from_numpy(llr).cuda()
     hard, it = dec.decode(d_llr, iterations=True)
     torch.cuda.synchronize()
@@ -205,6 +208,8 @@ def test_zero_frames_and_errors():
         dec.soDecode(np.zeros(oc.N), syst=np.zeros(3))
     with pytest.raises(ValueError):
         F.Ldpc(H, F.DecoderOptions().iterations(0))
+    with pytest.raises(ValueError):
+        dec.setPassFrames(-1)
 
 
 def test_batch_graph_multirate_replays():
@@ -217,6 +222,9 @@ def test_batch_graph_multirate_replays():
         oc, H = dvbs2(n, rate)
         codes.append(oc)
         decs.append(gpu_decoder(H, F.Approximate, 50, fmt=F.MsgFormat.F16))
+        if n == 16200:  # two passes of one 128-frame tile and two frames
+            decs[-1].setLayout(1)
+            decs[-1].setPassFrames(100)
         tensors.append(torch.zeros(frames * oc.N, dtype=torch.float32, device="cuda"))
     g = F.BatchGraph(decs, tensors)
     for rep in range(3):
