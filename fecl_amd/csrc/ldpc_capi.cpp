@@ -171,7 +171,7 @@ int prepare(fecl_ldpc* h, Workspace* ws, const Layout& lay, int64_t frames, bool
   HIP_TRY(ws->phase.ensure((size_t)tiles * TW));
   HIP_TRY(ws->iter.ensure((size_t)tiles * TW * 4));
   HIP_TRY(ws->fail.ensure((size_t)tiles * FPL * 8));
-  HIP_TRY(ws->flags.ensure((size_t)tiles * 4));
+  HIP_TRY(ws->flags.ensure((size_t)tiles * 8));
   HIP_TRY(ws->active.ensure((size_t)h->iterations * 4));
   HIP_TRY(ws->out_hard.ensure((size_t)tiles * c.K * FPL * 8));
   if (soft) {
@@ -187,6 +187,7 @@ int prepare(fecl_ldpc* h, Workspace* ws, const Layout& lay, int64_t frames, bool
   s.iter = ws->iter.as<int32_t>();
   s.fail = ws->fail.as<uint64_t>();
   s.tile_flags = ws->flags.as<uint32_t>();
+  s.tile_iter = ws->flags.as<int32_t>() + tiles;
   s.active = ws->active.as<int32_t>();
   s.tiles = (int32_t)tiles;
   DevOut& out = *op;

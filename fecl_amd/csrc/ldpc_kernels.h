@@ -28,8 +28,11 @@ enum : uint8_t {
   PH_VNINIT = 8,   // state supplied: a variable-node pass must build v2c from it first
 };
 // Tile flags: any frame RUN or FINAL (check and variable kernels work), any frame RUN
-// (syndrome works), any frame waiting for its state-seeded variable pass.
-enum : uint32_t { TF_LIVE = 1, TF_RUN = 2, TF_VNINIT = 4 };
+// (syndrome works), any frame waiting for its state-seeded variable pass, and "uniform": every
+// frame of the tile is RUN with no pending flag and the same iteration count (tile_iter), so
+// kernels need not read per-frame state.  Only the control kernel sets TF_UNIFORM; every other
+// writer of a tile's flags clears it.
+enum : uint32_t { TF_LIVE = 1, TF_RUN = 2, TF_VNINIT = 4, TF_UNIFORM = 8 };
 
 struct DevCode {
   const int32_t* row_ptr;
@@ -47,6 +50,7 @@ struct DevState {
   int32_t* iter;
   uint64_t* fail;          // [tile][FPL]
   uint32_t* tile_flags;    // [tile]
+  int32_t* tile_iter;      // [tile] iteration count of a TF_UNIFORM tile
   int32_t* active;         // [iterations] frames still iterating after each step
   int32_t tiles;
 };

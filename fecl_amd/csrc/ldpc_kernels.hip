@@ -24,55 +24,50 @@ namespace fecl {
 
 // ------------------------------------------------------------------------------------------
 // Element formats: PW messages per 32-bit word.
+// 16-bit storage: saturate to the format's largest finite value (v_med3_f32), then round to
+// nearest even with gfx950's two-lane converts (v_cvt_pk_f16_f32 / v_cvt_pk_bf16_f32).
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+
+constexpr float kF16Max = 65504.0f;
+constexpr float kBF16Max = 3.38953138925153547590e38f;   // 0x1.fep127
+
+__device__ static inline uint32_t f32x2_to_f16x2(float a, float b) {
+  f32x2 x = {__builtin_amdgcn_fmed3f(a, -kF16Max, kF16Max), __builtin_amdgcn_fmed3f(b, -kF16Max, kF16Max)};
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(x, f16x2));
+}
+__device__ static inline uint32_t f32x2_to_bf16x2(float a, float b) {
+  f32x2 x = {__builtin_amdgcn_fmed3f(a, -kBF16Max, kBF16Max), __builtin_amdgcn_fmed3f(b, -kBF16Max, kBF16Max)};
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(x, bf16x2));
+}
+__device__ static inline float f16_bits_to_f32(uint32_t b) {
+  return (float)__builtin_bit_cast(_Float16, (uint16_t)b);
+}
 struct ElemF32 {
   static constexpr int PW = 1;
   __device__ static inline void unpack(uint32_t w, float* x) { x[0] = __uint_as_float(w); }
   __device__ static inline uint32_t pack(const float* x) { return __float_as_uint(x[0]); }
   __device__ static inline float quantize(float x) { return x; }
 };
-
-__device__ static inline uint32_t f32_to_f16_bits(float x) {
-  // saturate to +-65504, then round to nearest even (v_cvt_f16_f32 in the default mode)
-  if (x > 65504.0f) x = 65504.0f;
-  if (x < -65504.0f) x = -65504.0f;
-  _Float16 h = (_Float16)x;
-  return (uint32_t)__builtin_bit_cast(uint16_t, h);
-}
-__device__ static inline float f16_bits_to_f32(uint32_t b) {
-  return (float)__builtin_bit_cast(_Float16, (uint16_t)b);
-}
 struct ElemF16 {
   static constexpr int PW = 2;
   __device__ static inline void unpack(uint32_t w, float* x) {
     x[0] = f16_bits_to_f32(w & 0xffffu);
     x[1] = f16_bits_to_f32(w >> 16);
   }
-  __device__ static inline uint32_t pack(const float* x) {
-    return f32_to_f16_bits(x[0]) | (f32_to_f16_bits(x[1]) << 16);
-  }
-  __device__ static inline float quantize(float x) { return f16_bits_to_f32(f32_to_f16_bits(x)); }
+  __device__ static inline uint32_t pack(const float* x) { return f32x2_to_f16x2(x[0], x[1]); }
+  __device__ static inline float quantize(float x) { return f16_bits_to_f32(f32x2_to_f16x2(x, 0.0f)); }
 };
-
-__device__ static inline uint32_t f32_to_bf16_bits(float x) {
-  // saturate to +-0x1.fep127, then round to nearest even
-  const float bmax = 3.38953138925153547590e38f;
-  if (x > bmax) x = bmax;
-  if (x < -bmax) x = -bmax;
-  uint32_t u = __float_as_uint(x);
-  if (x != x) return 0x7fc0u;
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return u >> 16;
-}
 struct ElemBF16 {
   static constexpr int PW = 2;
   __device__ static inline void unpack(uint32_t w, float* x) {
     x[0] = __uint_as_float(w << 16);
     x[1] = __uint_as_float(w & 0xffff0000u);
   }
-  __device__ static inline uint32_t pack(const float* x) {
-    return f32_to_bf16_bits(x[0]) | (f32_to_bf16_bits(x[1]) << 16);
-  }
-  __device__ static inline float quantize(float x) { return __uint_as_float(f32_to_bf16_bits(x) << 16); }
+  __device__ static inline uint32_t pack(const float* x) { return f32x2_to_bf16x2(x[0], x[1]); }
+  __device__ static inline float quantize(float x) { return __uint_as_float(f32x2_to_bf16x2(x, 0.0f) << 16); }
 };
 
 template <int W>
@@ -161,9 +156,18 @@ struct CnLanes {
 
 // Frame state loads and the decisions drawn from them are split so that a kernel can issue
 // its row loads in between: the wait for the state then does not wait for the rows.
+// A TF_UNIFORM tile (every frame RUN, no pending flag, one iteration count) skips the
+// per-frame loads.
 template <int FPL>
-__device__ static inline void lane_state(const DevState& s, int t, int lane, uint8_t (&ph)[FPL], int (&it)[FPL]) {
+__device__ static inline void lane_state(const DevState& s, int t, uint32_t tflags, int lane, uint8_t (&ph)[FPL],
+                                         int (&it)[FPL]) {
   constexpr int TW = kWave * FPL;
+  if (tflags & TF_UNIFORM) {
+    const int ti = uniform(s.tile_iter[t]);
+#pragma unroll
+    for (int k = 0; k < FPL; ++k) { ph[k] = PH_RUN; it[k] = ti; }
+    return;
+  }
 #pragma unroll
   for (int k = 0; k < FPL; ++k) {
     const int64_t p = (int64_t)t * TW + lane * FPL + k;
@@ -232,13 +236,17 @@ __device__ static inline void cn_syndrome(const DevCode& c, const DevState& s, i
 // associative, so FeCl's forward/backward pass equals "min of the others, xor of the others'
 // signs" exactly: output j has magnitude min_{i != j} |x_i| and sign bit
 // ((d-2) & 1) ^ xor_{i != j} signbit(x_i)  (a box-sum of two operands sets the sign bit
-// 1 ^ s_a ^ s_b).  One wavefront per (check row, tile).  DMAX > 0: the row's d words are
-// loaded into registers at once (d <= DMAX); DMAX == 0: two streaming passes (any d <= 64).
+// 1 ^ s_a ^ s_b).  The row keeps the two smallest magnitudes m1 <= m2 (branch-free:
+// m2 = min(m2, max(m1, a)), m1 = min(m1, a)); "min of the others" is m2 where |x_j| == m1 and
+// m1 elsewhere (a tie makes m2 == m1, so either choice is exact).  One wavefront per
+// (check row, tile).  DMAX > 0: the row's d words are loaded into registers at once
+// (d <= DMAX); DMAX == 0: two streaming passes (any d <= 64).
 template <class F, int DMAX>
 __global__ __launch_bounds__(kBlock) void cn_minsum_kernel(DevCode c, DevState s, DevOpts o) {
   constexpr int FPL = F::FPL, TW = F::TW, RW = F::RW;
   const int t = blockIdx.y;
-  if (!(s.tile_flags[t] & TF_LIVE)) return;
+  const uint32_t tflags = uniform((int)s.tile_flags[t]);
+  if (!(tflags & TF_LIVE)) return;
   const int r = uniform(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6));
   if (r >= c.M) return;
   const int lane = threadIdx.x & 63;
@@ -248,7 +256,7 @@ __global__ __launch_bounds__(kBlock) void cn_minsum_kernel(DevCode c, DevState s
   const float* Lt = s.L + (int64_t)t * c.N * TW + lane * FPL;
   uint8_t ph[FPL];
   int it[FPL];
-  lane_state<FPL>(s, t, lane, ph, it);
+  lane_state<FPL>(s, t, tflags, lane, ph, it);
   float xs[DMAX > 0 ? DMAX : 1][FPL];
   if constexpr (DMAX > 0) {
 #pragma unroll
@@ -259,17 +267,16 @@ __global__ __launch_bounds__(kBlock) void cn_minsum_kernel(DevCode c, DevState s
   if (!cn_decide<FPL>(o, ph, it, ln)) return;
 
   float m1[FPL], m2[FPL];
-  int i1[FPL];
   uint64_t sg[FPL];
 #pragma unroll
-  for (int k = 0; k < FPL; ++k) { m1[k] = __builtin_inff(); m2[k] = __builtin_inff(); i1[k] = 0; sg[k] = 0; }
+  for (int k = 0; k < FPL; ++k) { m1[k] = __builtin_inff(); m2[k] = __builtin_inff(); sg[k] = 0; }
   auto accumulate = [&](int j, const float (&x)[FPL]) {
 #pragma unroll
     for (int k = 0; k < FPL; ++k) {
       const float a = fabsf(x[k]);
       sg[k] |= (uint64_t)(__float_as_uint(x[k]) >> 31) << j;
-      if (a < m1[k]) { m2[k] = m1[k]; m1[k] = a; i1[k] = j; }
-      else if (a < m2[k]) { m2[k] = a; }
+      m2[k] = fminf(m2[k], fmaxf(m1[k], a));
+      m1[k] = fminf(m1[k], a);
     }
   };
   auto fresh_x = [&](int j, float (&x)[FPL]) {   // frames whose v2c is still L[col(e)]
@@ -309,13 +316,15 @@ __global__ __launch_bounds__(kBlock) void cn_minsum_kernel(DevCode c, DevState s
     m2s[k] = apply_sf(o, ln.sfrow[k], d - 2, m2[k]);
   }
   if (!ln.lane_any) return;
-  auto emit = [&](int j, const float (&old)[FPL]) {
+  // x: the row's input word as accumulated (frames that take no update are never fresh, so
+  // for them x is the stored word, which they keep)
+  auto emit = [&](int j, const float (&x)[FPL]) {
     float y[FPL];
 #pragma unroll
     for (int k = 0; k < FPL; ++k) {
-      const float mag = (j == i1[k]) ? m2s[k] : m1s[k];
+      const float mag = (fabsf(x[k]) == m1[k]) ? m2s[k] : m1s[k];
       const uint32_t sb = par[k] ^ (uint32_t)((sg[k] >> j) & 1);
-      y[k] = ln.act[k] ? (sb ? -mag : mag) : old[k];   // other frames keep their word
+      y[k] = ln.act[k] ? (sb ? -mag : mag) : x[k];
     }
     F::store(mrow + (int64_t)j * RW, y);
   };
@@ -325,9 +334,99 @@ __global__ __launch_bounds__(kBlock) void cn_minsum_kernel(DevCode c, DevState s
       if (j < d) emit(j, xs[j]);
   } else {
     for (int j = 0; j < d; ++j) {
-      float old[FPL];
-      F::load(mrow + (int64_t)j * RW, old);
-      emit(j, old);
+      float x[FPL];
+      F::load(mrow + (int64_t)j * RW, x);
+      fresh_x(j, x);
+      emit(j, x);
+    }
+  }
+}
+
+// Min-sum on 16-bit messages without scaling factors: every output is one of the row's stored
+// values with a new sign, so the update runs on the raw encodings, two frames per 32-bit
+// word.  For finite FP16 and BF16 values the magnitude order is the integer order of
+// (bits & 0x7fff), and the stored values are finite (storage saturates).  Same results as
+// cn_minsum_kernel, bit for bit.
+template <class F, int DMAX>
+__global__ __launch_bounds__(kBlock) void cn_minsum16_kernel(DevCode c, DevState s, DevOpts o) {
+  static_assert(F::PW == 2 && DMAX > 0, "16-bit formats, register-staged rows");
+  constexpr int FPL = F::FPL, TW = F::TW, RW = F::RW, W = F::W;
+  const int t = blockIdx.y;
+  const uint32_t tflags = uniform((int)s.tile_flags[t]);
+  if (!(tflags & TF_LIVE)) return;
+  const int r = uniform(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6));
+  if (r >= c.M) return;
+  const int lane = threadIdx.x & 63;
+  const int e0 = uniform(c.row_ptr[r]);
+  const int d = uniform(c.row_ptr[r + 1]) - e0;
+  uint32_t* mrow = s.msg + ((int64_t)t * c.E + e0) * RW + lane * W;
+  const float* Lt = s.L + (int64_t)t * c.N * TW + lane * FPL;
+  uint8_t ph[FPL];
+  int it[FPL];
+  lane_state<FPL>(s, t, tflags, lane, ph, it);
+  uint32_t w[DMAX][W];
+#pragma unroll
+  for (int j = 0; j < DMAX; ++j)
+    if (j < d) load_words<W>(mrow + (int64_t)j * RW, w[j]);
+  CnLanes<FPL> ln;
+  if (!cn_decide<FPL>(o, ph, it, ln)) return;
+  if (ln.wave_fresh) {   // frames whose v2c is still L[col(e)], rounded to the storage format
+#pragma unroll
+    for (int j = 0; j < DMAX; ++j) {
+      if (j < d) {
+        const int v = uniform(c.col[e0 + j]);
+        float l[FPL];
+        load_floats<FPL>(Lt + (int64_t)v * TW, l);
+#pragma unroll
+        for (int i = 0; i < W; ++i) {
+          const uint32_t q = F::Elem::pack(&l[2 * i]);
+          const uint32_t fm = (ln.fresh[2 * i] ? 0x0000ffffu : 0u) | (ln.fresh[2 * i + 1] ? 0xffff0000u : 0u);
+          w[j][i] = (q & fm) | (w[j][i] & ~fm);
+        }
+      }
+    }
+  }
+  if (o.fused_syn && ln.wave_seen) cn_syndrome<FPL>(c, s, t, e0, d, lane);
+  u16x2 m1[W], m2[W];
+  uint32_t sx[W];
+#pragma unroll
+  for (int i = 0; i < W; ++i) { m1[i] = u16x2{0x7fff, 0x7fff}; m2[i] = m1[i]; sx[i] = 0; }
+#pragma unroll
+  for (int j = 0; j < DMAX; ++j) {
+    if (j < d) {
+#pragma unroll
+      for (int i = 0; i < W; ++i) {
+        const u16x2 a = __builtin_bit_cast(u16x2, w[j][i] & 0x7fff7fffu);
+        m2[i] = __builtin_elementwise_min(m2[i], __builtin_elementwise_max(m1[i], a));
+        m1[i] = __builtin_elementwise_min(m1[i], a);
+        sx[i] ^= w[j][i];
+      }
+    }
+  }
+  if (!ln.lane_any) return;
+  const uint32_t flip = ((d - 2) & 1) ? 0x80008000u : 0u;
+  uint32_t am[W];
+  u16x2 dm[W];
+#pragma unroll
+  for (int i = 0; i < W; ++i) {
+    am[i] = (ln.act[2 * i] ? 0x0000ffffu : 0u) | (ln.act[2 * i + 1] ? 0xffff0000u : 0u);
+    dm[i] = m2[i] - m1[i];
+  }
+  const u16x2 one = {1, 1};
+#pragma unroll
+  for (int j = 0; j < DMAX; ++j) {
+    if (j < d) {
+      uint32_t y[W];
+#pragma unroll
+      for (int i = 0; i < W; ++i) {
+        const u16x2 a = __builtin_bit_cast(u16x2, w[j][i] & 0x7fff7fffu);
+        // |x_j| == m1 -> m2, else m1   (a >= m1, so a - m1 does not wrap)
+        const u16x2 is_min = __builtin_elementwise_sub_sat(one, a - m1[i]);
+        const uint32_t mag = __builtin_bit_cast(uint32_t, m1[i] + is_min * dm[i]);
+        const uint32_t out = mag | ((sx[i] ^ flip ^ w[j][i]) & 0x80008000u);
+        y[i] = (out & am[i]) | (w[j][i] & ~am[i]);
+      }
+      store_words<W>(mrow + (int64_t)j * RW, y);
     }
   }
 }
@@ -371,7 +470,8 @@ template <class F, int DMAX, int ALG>
 __global__ __launch_bounds__(kBlock) void cn_box_kernel(DevCode c, DevState s, DevOpts o) {
   constexpr int FPL = F::FPL, TW = F::TW, RW = F::RW;
   const int t = blockIdx.y;
-  if (!(s.tile_flags[t] & TF_LIVE)) return;
+  const uint32_t tflags = uniform((int)s.tile_flags[t]);
+  if (!(tflags & TF_LIVE)) return;
   const int r = uniform(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6));
   if (r >= c.M) return;
   const int lane = threadIdx.x & 63;
@@ -381,7 +481,7 @@ __global__ __launch_bounds__(kBlock) void cn_box_kernel(DevCode c, DevState s, D
   const float* Lt = s.L + (int64_t)t * c.N * TW + lane * FPL;
   uint8_t ph[FPL];
   int it[FPL];
-  lane_state<FPL>(s, t, lane, ph, it);
+  lane_state<FPL>(s, t, tflags, lane, ph, it);
   float x[DMAX][FPL];
 #pragma unroll
   for (int j = 0; j < DMAX; ++j)
@@ -478,7 +578,8 @@ __global__ __launch_bounds__(kBlock) void vn_kernel(DevCode c, DevState s, DevOu
   // so this pass starts on the tiles whose messages the check pass wrote last (still in the
   // 256 MB Infinity Cache), and the next check pass starts where this one ended.
   const int t = s.tiles - 1 - (int)blockIdx.y;
-  if (!(s.tile_flags[t] & (init_pass ? TF_VNINIT : TF_LIVE))) return;
+  const uint32_t tflags = uniform((int)s.tile_flags[t]);
+  if (!(tflags & (init_pass ? TF_VNINIT : TF_LIVE))) return;
   const int v = uniform(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6));
   if (v >= c.N) return;
   const int lane = threadIdx.x & 63;
@@ -492,7 +593,7 @@ __global__ __launch_bounds__(kBlock) void vn_kernel(DevCode c, DevState s, DevOu
     if (j < dv) eid[j] = uniform(c.var_edges[s0 + j]);
   uint8_t ph[FPL];
   int itv[FPL];
-  lane_state<FPL>(s, t, lane, ph, itv);
+  lane_state<FPL>(s, t, tflags, lane, ph, itv);
   uint64_t fw[FPL];
 #pragma unroll
   for (int k = 0; k < FPL; ++k) fw[k] = s.fail[t * FPL + k];
@@ -652,18 +753,21 @@ __global__ __launch_bounds__(kWave) void control_kernel(DevState s, int iteratio
   constexpr int TW = kWave * FPL;
   const int t = blockIdx.x;
   const int lane = threadIdx.x;
-  bool any_live = false, any_run = false;
+  bool any_live = false, any_run = false, all_run = true;
   int alive = 0;
+  int iters[FPL];
 #pragma unroll
   for (int k = 0; k < FPL; ++k) {
     const uint64_t fw = s.fail[t * FPL + k];
     const int64_t p = (int64_t)t * TW + lane * FPL + k;
     uint8_t ph = s.phase[p];
     const int base = ph & PH_MASK;
+    iters[k] = -1;
     if (base == PH_RUN) {
       const int it0 = s.iter[p];
       const int it = it0 + 1;
       s.iter[p] = it;
+      iters[k] = it;
       const bool ok = ((fw >> lane) & 1ull) == 0;
       if (fused_syn)
         ph = (it0 >= 1 && ok) ? PH_DONE : (it >= iterations - 1 ? PH_FINAL : PH_RUN);
@@ -678,15 +782,23 @@ __global__ __launch_bounds__(kWave) void control_kernel(DevState s, int iteratio
     s.phase[p] = ph;
     any_live |= ph == PH_RUN || ph == PH_FINAL;
     any_run |= ph == PH_RUN;
+    all_run &= ph == PH_RUN;
     alive += (ph == PH_RUN || ph == PH_FINAL) ? 1 : 0;
   }
+  // uniform: every frame still RUN (phases leave this kernel without pending flags) at one
+  // iteration count
+  const int it_ref = __builtin_amdgcn_readfirstlane(iters[0]);
+#pragma unroll
+  for (int k = 0; k < FPL; ++k) all_run &= iters[k] == it_ref;
+  const bool uni = __ballot(!all_run) == 0;
   const uint64_t blive = __ballot(any_live), brun = __ballot(any_run);
 #pragma unroll
   for (int o = 32; o >= 1; o >>= 1) alive += __shfl_xor(alive, o, 64);
   if (lane == 0) {
 #pragma unroll
     for (int k = 0; k < FPL; ++k) s.fail[t * FPL + k] = 0;
-    s.tile_flags[t] = (blive ? TF_LIVE : 0u) | (brun ? TF_RUN : 0u);
+    s.tile_flags[t] = (blive ? TF_LIVE : 0u) | (brun ? TF_RUN : 0u) | (uni ? TF_UNIFORM : 0u);
+    s.tile_iter[t] = it_ref;
     if (alive && step >= 0) atomicAdd(&s.active[step], alive);
   }
 }
@@ -855,6 +967,7 @@ __global__ __launch_bounds__(kBlock) void admit_kernel(DevCode c, DevState s, co
     s.phase[p] = init_phase;
     s.iter[p] = 0;
     const uint8_t base = init_phase & PH_MASK;
+    atomicAnd(&s.tile_flags[t], ~TF_UNIFORM);
     atomicOr(&s.tile_flags[t], (base == PH_RUN || base == PH_FINAL ? TF_LIVE : 0u) | (base == PH_RUN ? TF_RUN : 0u));
   }
 }
@@ -937,6 +1050,26 @@ static hipError_t launch_cn(const DevCode& c, const DevState& s, const DevOpts& 
   dim3 grid((c.M + kWavesPerBlock - 1) / kWavesPerBlock, s.tiles);
   if (o.algorithm == 2) {
     bool done = false;
+    if constexpr (F::PW == 2) {
+      if (o.sf_mode == 0) {   // raw 16-bit encodings (no scaling factor to apply)
+        constexpr int W = F::W;
+        if (max_dc <= 8) {
+          hipLaunchKernelGGL((cn_minsum16_kernel<F, 8>), grid, dim3(kBlock), 0, st, c, s, o);
+          done = true;
+        } else if (max_dc <= 16) {
+          if constexpr (16 * W <= 64) {
+            hipLaunchKernelGGL((cn_minsum16_kernel<F, 16>), grid, dim3(kBlock), 0, st, c, s, o);
+            done = true;
+          }
+        } else if (max_dc <= 32) {
+          if constexpr (32 * W <= 64) {
+            hipLaunchKernelGGL((cn_minsum16_kernel<F, 32>), grid, dim3(kBlock), 0, st, c, s, o);
+            done = true;
+          }
+        }
+        if (done) return hipGetLastError();
+      }
+    }
     if (max_dc <= 8) {
       hipLaunchKernelGGL((cn_minsum_kernel<F, 8>), grid, dim3(kBlock), 0, st, c, s, o);
       done = true;
