@@ -431,6 +431,16 @@ int fecl_ldpc_create(const int32_t* row_ptr, const int32_t* col, int64_t M, int6
   h->dc.col = h->d_col.as<int32_t>();
   h->dc.var_ptr = h->d_var_ptr.as<int32_t>();
   h->dc.var_edges = h->d_var_edges.as<int32_t>();
+  {
+    const int vs = var_table_stride(c.max_var_degree);
+    std::vector<int32_t> tab((size_t)c.N * vs, -1);
+    for (int64_t v = 0; v < c.N; ++v)
+      for (int32_t j = c.var_ptr[v]; j < c.var_ptr[v + 1]; ++j) tab[(size_t)v * vs + (j - c.var_ptr[v])] = c.var_edges[j];
+    HIP_TRY(h->d_var_tab.ensure(tab.size() * 4));
+    HIP_TRY(hipMemcpy(h->d_var_tab.p, tab.data(), tab.size() * 4, hipMemcpyHostToDevice));
+    h->dc.var_tab = h->d_var_tab.as<int32_t>();
+    h->dc.vstride = vs;
+  }
   h->dc.M = (int32_t)c.M;
   h->dc.N = (int32_t)c.N;
   h->dc.K = (int32_t)c.K;

@@ -76,7 +76,7 @@ struct Workspace {
 struct fecl_ldpc {
   fecl::Code code;
   int device = 0;
-  fecl::engine::DevBuf d_row_ptr, d_col, d_var_ptr, d_var_edges, d_sf, d_sf32;
+  fecl::engine::DevBuf d_row_ptr, d_col, d_var_ptr, d_var_edges, d_var_tab, d_sf, d_sf32;
   fecl::DevCode dc{};
   // options
   int32_t algorithm = FECL_APPROXIMATE;
@@ -101,7 +101,8 @@ struct fecl_ldpc {
   int64_t prof_launches[3] = {0, 0, 0};
   ~fecl_ldpc() {
     pool.clear();
-    for (fecl::engine::DevBuf* b : {&d_row_ptr, &d_col, &d_var_ptr, &d_var_edges, &d_sf, &d_sf32}) b->release();
+    for (fecl::engine::DevBuf* b : {&d_row_ptr, &d_col, &d_var_ptr, &d_var_edges, &d_var_tab, &d_sf, &d_sf32})
+      b->release();
   }
 };
 

@@ -39,6 +39,11 @@ struct DevCode {
   const int32_t* col;
   const int32_t* var_ptr;
   const int32_t* var_edges;
+  // var_tab[v * vstride + j]: j-th edge of variable v in ascending edge order, -1 past its
+  // degree; vstride is the variable kernel's degree bound (4, 8, 16 or 32), so one scalar
+  // load fetches a variable's whole edge list
+  const int32_t* var_tab;
+  int32_t vstride;
   int32_t M, N, K, E;
 };
 

@@ -245,13 +245,15 @@ template <class F, int DMAX>
 __global__ __launch_bounds__(kBlock) void cn_minsum_kernel(DevCode c, DevState s, DevOpts o) {
   constexpr int FPL = F::FPL, TW = F::TW, RW = F::RW;
   const int t = blockIdx.y;
-  const uint32_t tflags = uniform((int)s.tile_flags[t]);
-  if (!(tflags & TF_LIVE)) return;
   const int r = uniform(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6));
   if (r >= c.M) return;
   const int lane = threadIdx.x & 63;
+  // the row's extent and the tile's flags are requested together (the exit test reads both,
+  // which keeps the compiler from deferring the extent load past it)
   const int e0 = uniform(c.row_ptr[r]);
   const int d = uniform(c.row_ptr[r + 1]) - e0;
+  const uint32_t tflags = uniform((int)s.tile_flags[t]);
+  if (!(tflags & TF_LIVE) || d < 0) return;
   uint32_t* mrow = s.msg + ((int64_t)t * c.E + e0) * RW + lane * F::W;
   const float* Lt = s.L + (int64_t)t * c.N * TW + lane * FPL;
   uint8_t ph[FPL];
@@ -352,13 +354,15 @@ __global__ __launch_bounds__(kBlock) void cn_minsum16_kernel(DevCode c, DevState
   static_assert(F::PW == 2 && DMAX > 0, "16-bit formats, register-staged rows");
   constexpr int FPL = F::FPL, TW = F::TW, RW = F::RW, W = F::W;
   const int t = blockIdx.y;
-  const uint32_t tflags = uniform((int)s.tile_flags[t]);
-  if (!(tflags & TF_LIVE)) return;
   const int r = uniform(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6));
   if (r >= c.M) return;
   const int lane = threadIdx.x & 63;
+  // the row's extent and the tile's flags are requested together (the exit test reads both,
+  // which keeps the compiler from deferring the extent load past it)
   const int e0 = uniform(c.row_ptr[r]);
   const int d = uniform(c.row_ptr[r + 1]) - e0;
+  const uint32_t tflags = uniform((int)s.tile_flags[t]);
+  if (!(tflags & TF_LIVE) || d < 0) return;
   uint32_t* mrow = s.msg + ((int64_t)t * c.E + e0) * RW + lane * W;
   const float* Lt = s.L + (int64_t)t * c.N * TW + lane * FPL;
   uint8_t ph[FPL];
@@ -470,13 +474,15 @@ template <class F, int DMAX, int ALG>
 __global__ __launch_bounds__(kBlock) void cn_box_kernel(DevCode c, DevState s, DevOpts o) {
   constexpr int FPL = F::FPL, TW = F::TW, RW = F::RW;
   const int t = blockIdx.y;
-  const uint32_t tflags = uniform((int)s.tile_flags[t]);
-  if (!(tflags & TF_LIVE)) return;
   const int r = uniform(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6));
   if (r >= c.M) return;
   const int lane = threadIdx.x & 63;
+  // the row's extent and the tile's flags are requested together (the exit test reads both,
+  // which keeps the compiler from deferring the extent load past it)
   const int e0 = uniform(c.row_ptr[r]);
   const int d = uniform(c.row_ptr[r + 1]) - e0;
+  const uint32_t tflags = uniform((int)s.tile_flags[t]);
+  if (!(tflags & TF_LIVE) || d < 0) return;
   uint32_t* mrow = s.msg + ((int64_t)t * c.E + e0) * RW + lane * F::W;
   const float* Lt = s.L + (int64_t)t * c.N * TW + lane * FPL;
   uint8_t ph[FPL];
@@ -578,27 +584,28 @@ __global__ __launch_bounds__(kBlock) void vn_kernel(DevCode c, DevState s, DevOu
   // so this pass starts on the tiles whose messages the check pass wrote last (still in the
   // 256 MB Infinity Cache), and the next check pass starts where this one ended.
   const int t = s.tiles - 1 - (int)blockIdx.y;
-  const uint32_t tflags = uniform((int)s.tile_flags[t]);
-  if (!(tflags & (init_pass ? TF_VNINIT : TF_LIVE))) return;
   const int v = uniform(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6));
   if (v >= c.N) return;
   const int lane = threadIdx.x & 63;
-  // graph indices, frame state and the variable's rows are all requested before any of them
-  // is waited on
-  const int s0 = uniform(c.var_ptr[v]);
-  const int dv = uniform(c.var_ptr[v + 1]) - s0;
+  // the tile's flags and the variable's edge list (one scalar load of VMAX ids) are requested
+  // together (the exit test reads both, which keeps the compiler from deferring the list load
+  // past it); channel values, frame state and rows follow at once
   int eid[VMAX];
 #pragma unroll
-  for (int j = 0; j < VMAX; ++j)
-    if (j < dv) eid[j] = uniform(c.var_edges[s0 + j]);
+  for (int j = 0; j < VMAX; ++j) eid[j] = c.var_tab[v * VMAX + j];
+  const uint32_t tflags = uniform((int)s.tile_flags[t]);
+  if (!(tflags & (init_pass ? TF_VNINIT : TF_LIVE)) || eid[0] < -1) return;
+  float Lv[FPL];
+  load_floats<FPL>(s.L + ((int64_t)t * c.N + v) * TW + lane * FPL, Lv);
+  int dv = 0;
+#pragma unroll
+  for (int j = 0; j < VMAX; ++j) dv += eid[j] >= 0 ? 1 : 0;
   uint8_t ph[FPL];
   int itv[FPL];
   lane_state<FPL>(s, t, tflags, lane, ph, itv);
   uint64_t fw[FPL];
 #pragma unroll
   for (int k = 0; k < FPL; ++k) fw[k] = s.fail[t * FPL + k];
-  float Lv[FPL];
-  load_floats<FPL>(s.L + ((int64_t)t * c.N + v) * TW + lane * FPL, Lv);
   uint32_t* mt = s.msg + (int64_t)t * c.E * RW + lane * F::W;
   float cv[VMAX][FPL];
 #pragma unroll
@@ -1011,6 +1018,7 @@ static hipError_t launch_vn(const DevCode& c, const DevState& s, const DevOut& o
                             hipStream_t st) {
   constexpr int FPL = F::FPL;
   dim3 grid((c.N + kWavesPerBlock - 1) / kWavesPerBlock, s.tiles);
+  if (c.vstride != var_table_stride(max_dv)) return hipErrorInvalidValue;
   if (max_dv <= 4) hipLaunchKernelGGL((vn_kernel<F, 4>), grid, dim3(kBlock), 0, st, c, s, o, fused, init);
   else if (max_dv <= 8) hipLaunchKernelGGL((vn_kernel<F, 8>), grid, dim3(kBlock), 0, st, c, s, o, fused, init);
   else if (max_dv <= 16) {

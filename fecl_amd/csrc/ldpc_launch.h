@@ -30,6 +30,9 @@ struct Layout {
 // Whether the kernels for this layout stay within the register budget for the code's degrees.
 bool layout_fits(const Layout& lay, int algorithm, int max_dc, int max_dv);
 
+// Degree bound of the variable kernel for a code (the stride of DevCode::var_tab).
+inline int var_table_stride(int max_dv) { return max_dv <= 4 ? 4 : max_dv <= 8 ? 8 : max_dv <= 16 ? 16 : 32; }
+
 hipError_t kernels_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t st);
 hipError_t kernels_ingest(const Layout& lay, const DevCode& c, const DevState& s, const void* llr, const void* syst,
                           bool dbl, int64_t frames, int64_t frame0, uint8_t init_phase, hipStream_t st);
