@@ -425,6 +425,7 @@ def main():
     ap.add_argument("--offered-fps", type=float, default=200000.0)
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--pass-frames", type=int, default=0, help="frames per decode pass (0 = library default)")
+    ap.add_argument("--vpw", type=int, default=0, help="variables per wavefront (0 automatic; with --words >= 0)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the decoder's frames are independent, so N GPUs would "
@@ -463,9 +464,11 @@ def main():
     tune = None
     if args.words >= 0:
         dec.setLayout(args.words)
+        dec.setVarsPerWave(args.vpw)
     else:
         best, times = dec.autotune(args.frames, probe_iterations=5)
-        tune = {"picked_words": best, "probe_ms": {str(k): round(v, 3) for k, v in times.items()}}
+        tune = {"picked_words": best, "picked_vars_per_wave": dec.varsPerWave(),
+                "probe_ms": {f"w{k[0]}v{k[1]}": round(v, 3) for k, v in times.items()}}
         print(f"[bench] autotune: {tune}", file=sys.stderr, flush=True)
     dec.setPassFrames(args.pass_frames)
     words, frames_per_tile = dec.layout()
@@ -550,7 +553,8 @@ def main():
                                f"Eb/N0 {list(EBN0)} dB",
                    "frames_per_step": args.frames, "iterations": args.iterations, "algorithm": args.alg,
                    "msg_format": args.msg, "code": "dvbs2 64800 1/2",
-                   "lane_words": words, "frames_per_tile": frames_per_tile, "autotune": tune,
+                   "lane_words": words, "frames_per_tile": frames_per_tile, "vars_per_wave": dec.varsPerWave(),
+                   "autotune": tune,
                    "pass_frames": args.pass_frames},
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
                        "p99": round(float(np.percentile(lat, 99)) * 1e3, 3), "unit": "per batch decode"},

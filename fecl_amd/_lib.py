@@ -32,6 +32,8 @@ SIGNATURES = {
     "fecl_ldpc_get_layout": [P, P, P],
     "fecl_ldpc_set_fused_syndrome": [P, I32],
     "fecl_ldpc_set_pass_frames": [P, I64],
+    "fecl_ldpc_set_vars_per_wave": [P, I32],
+    "fecl_ldpc_get_vars_per_wave": [P, P],
     "fecl_ldpc_autotune": [P, I64, I32, P, P],
     "fecl_ldpc_graph_create": [P, P, I32, P, P, P, P],
     "fecl_ldpc_graph_launch": [P, P],

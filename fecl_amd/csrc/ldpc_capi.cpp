@@ -38,28 +38,29 @@ int fail(int status, const std::string& msg) {
 // Frames decoded per batch: bounded so a workspace stays a few GiB at most.
 constexpr int64_t kMaxChunkFrames = 8192;
 
-// Widest lane layout the kernels allow for this code and options (unless one was pinned).
+// Widest lane layout the kernels allow for this code and options (unless one was pinned), and
+// the variables per wavefront (pinned, else 1) where they fit that layout.
 void choose_layout(fecl_ldpc* h) {
   const int cand[3] = {4, 2, 1};
+  h->words = 0;
   if (h->words_pinned) {
     Layout lay;
     lay.fmt = h->msg_format;
     lay.words = h->words_pinned;
-    if (layout_fits(lay, h->algorithm, h->code.max_row_degree, h->code.max_var_degree)) {
-      h->words = h->words_pinned;
-      return;
-    }
+    if (layout_fits(lay, h->algorithm, h->code.max_row_degree, h->code.max_var_degree)) h->words = h->words_pinned;
   }
-  for (int w : cand) {
+  for (int i = 0; i < 3 && !h->words; ++i) {
     Layout lay;
     lay.fmt = h->msg_format;
-    lay.words = w;
-    if (layout_fits(lay, h->algorithm, h->code.max_row_degree, h->code.max_var_degree)) {
-      h->words = w;
-      return;
-    }
+    lay.words = cand[i];
+    if (layout_fits(lay, h->algorithm, h->code.max_row_degree, h->code.max_var_degree)) h->words = cand[i];
   }
-  h->words = 1;
+  if (!h->words) h->words = 1;
+  Layout lay;
+  lay.fmt = h->msg_format;
+  lay.words = h->words;
+  lay.vpw = h->vpw_pinned ? h->vpw_pinned : 1;
+  h->vpw = vpw_fits(lay, h->code.max_var_degree) ? lay.vpw : 1;
 }
 
 int set_options(fecl_ldpc* h, const fecl_decoder_options* o) {
@@ -157,6 +158,7 @@ Layout layout_of(const fecl_ldpc* h) {
   Layout lay;
   lay.fmt = h->msg_format;
   lay.words = h->words;
+  lay.vpw = h->vpw;
   return lay;
 }
 
@@ -433,7 +435,8 @@ int fecl_ldpc_create(const int32_t* row_ptr, const int32_t* col, int64_t M, int6
   h->dc.var_edges = h->d_var_edges.as<int32_t>();
   {
     const int vs = var_table_stride(c.max_var_degree);
-    std::vector<int32_t> tab((size_t)c.N * vs, -1);
+    // padded by a few variables so a wavefront serving several variables reads past none
+    std::vector<int32_t> tab((size_t)(c.N + 4) * vs, -1);
     for (int64_t v = 0; v < c.N; ++v)
       for (int32_t j = c.var_ptr[v]; j < c.var_ptr[v + 1]; ++j) tab[(size_t)v * vs + (j - c.var_ptr[v])] = c.var_edges[j];
     HIP_TRY(h->d_var_tab.ensure(tab.size() * 4));
@@ -582,6 +585,27 @@ int fecl_ldpc_set_fused_syndrome(fecl_ldpc* h, int32_t enable) {
   return FECL_OK;
 }
 
+int fecl_ldpc_set_vars_per_wave(fecl_ldpc* h, int32_t vars) {
+  if (int rc = check_handle(h)) return rc;
+  if (vars < 0 || vars > 2) return fail(FECL_INVALID_ARGUMENT, "variables per wavefront must be 0, 1 or 2");
+  if (vars == 2) {
+    Layout lay = layout_of(h);
+    lay.vpw = 2;
+    if (!vpw_fits(lay, h->code.max_var_degree))
+      return fail(FECL_NOT_SUPPORTED, "two variables per wavefront exceed the register budget for this layout");
+  }
+  std::lock_guard<std::mutex> lk(h->mu);
+  h->vpw_pinned = vars;
+  choose_layout(h);
+  return FECL_OK;
+}
+
+int fecl_ldpc_get_vars_per_wave(const fecl_ldpc* h, int32_t* vars) {
+  if (int rc = check_handle(h)) return rc;
+  if (vars) *vars = h->vpw;
+  return FECL_OK;
+}
+
 int fecl_ldpc_set_pass_frames(fecl_ldpc* h, int64_t frames) {
   if (int rc = check_handle(h)) return rc;
   if (frames < 0) return fail(FECL_INVALID_ARGUMENT, "frames per pass must be >= 0");
@@ -614,8 +638,7 @@ int fecl_ldpc_autotune(fecl_ldpc* h, int64_t frames, int32_t probe_iterations, i
   HIP_TRY(llr.ensure(host.size() * 4));
   HIP_TRY(bits.ensure((size_t)frames * c.K));
   HIP_TRY(hipMemcpy(llr.p, host.data(), host.size() * 4, hipMemcpyHostToDevice));
-  const int32_t saved_iters = h->iterations, saved_pin = h->words_pinned;
-  const int32_t saved_words = h->words;
+  const int32_t saved_iters = h->iterations, saved_pin = h->words_pinned, saved_vpin = h->vpw_pinned;
   const bool saved_prof = h->profiling;
   h->iterations = probe_iterations;
   h->profiling = false;
@@ -625,16 +648,20 @@ int fecl_ldpc_autotune(fecl_ldpc* h, int64_t frames, int32_t probe_iterations, i
   HIP_TRY(hipEventCreate(&e0));
   HIP_TRY(hipEventCreate(&e1));
   double best = 1e30;
-  int32_t best_w = saved_words;
+  int32_t best_w = 0, best_v = 0;
   const int cand[3] = {1, 2, 4};
   int rc = FECL_OK;
-  for (int i = 0; i < 3 && rc == FECL_OK; ++i) {
+  for (int i = 0; i < 6 && rc == FECL_OK; ++i) {
     Layout lay;
     lay.fmt = h->msg_format;
-    lay.words = cand[i];
+    lay.words = cand[i / 2];
+    lay.vpw = 1 + i % 2;
     if (ms) ms[i] = -1.0;
-    if (!layout_fits(lay, h->algorithm, c.max_row_degree, c.max_var_degree)) continue;
-    h->words = cand[i];
+    if (!layout_fits(lay, h->algorithm, c.max_row_degree, c.max_var_degree) || !vpw_fits(lay, c.max_var_degree))
+      continue;
+    h->words_pinned = lay.words;
+    h->vpw_pinned = lay.vpw;
+    choose_layout(h);
     float t = 0.0f;
     for (int rep = 0; rep < 2 && rc == FECL_OK; ++rep) {  // first run sizes the workspace
       hipEventRecord(e0, st);
@@ -645,19 +672,21 @@ int fecl_ldpc_autotune(fecl_ldpc* h, int64_t frames, int32_t probe_iterations, i
       hipEventElapsedTime(&t, e0, e1);
     }
     if (ms) ms[i] = t;
-    if (rc == FECL_OK && t < best) { best = t; best_w = cand[i]; }
+    if (rc == FECL_OK && t < best) { best = t; best_w = lay.words; best_v = lay.vpw; }
   }
   h->iterations = saved_iters;
   h->profiling = saved_prof;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   (void)hipStreamDestroy(st);
-  if (rc != FECL_OK) {
-    h->words = saved_words;
+  if (rc != FECL_OK || !best_w) {
     h->words_pinned = saved_pin;
+    h->vpw_pinned = saved_vpin;
+    choose_layout(h);
     return rc;
   }
   h->words_pinned = best_w;
+  h->vpw_pinned = best_v;
   choose_layout(h);
   if (best_words) *best_words = h->words;
   return FECL_OK;

@@ -87,6 +87,8 @@ struct fecl_ldpc {
   int32_t sf_mode = 0;
   int32_t words = 1;        // device layout: 32-bit words per lane per edge (1, 2 or 4)
   int32_t words_pinned = 0; // set by fecl_ldpc_set_layout / autotune; 0 = automatic
+  int32_t vpw = 1;          // variables per wavefront in the variable kernel (1 or 2)
+  int32_t vpw_pinned = 0;   // set by fecl_ldpc_set_vars_per_wave / autotune; 0 = automatic
   // Syndrome evaluated inside the next check kernel (one scaling row only) instead of a
   // separate kernel after the variable pass.  Off: the separate kernel reduces per workgroup
   // and sends few atomics, while the fused form puts one device-coherent load per row on the

@@ -571,46 +571,59 @@ __global__ __launch_bounds__(kBlock) void cn_box_kernel(DevCode c, DevState s, D
 // Variable-node update in FeCl's order (BpDecoderImpl.cpp:162-179):
 //   v2c_j = (0 + c_1 + ... + c_{j-1}) + (L + c_dv + ... + c_{j+1}),  APP = L + c_dv + ... + c_1
 // with edges of the variable in ascending edge order, then hard = APP >= 0 per frame.
-// One wavefront per (variable, tile).  The same pass forms the outputs of frames whose final
-// check update ran in this step (BpDecoderImpl.cpp:61-69, :109-129): with the final c2v,
+// One wavefront per (VPW consecutive variables, tile): the tile's frame state is read once and
+// every variable's rows are requested before the first is used.  The same pass forms the
+// outputs of frames whose final check update ran in this step (BpDecoderImpl.cpp:61-69,
+// :109-129): with the final c2v,
 //   decode   : app = L + c_1 + ... + c_dv (ascending), bit = app >= 0
 //   soDecode : ext = 0 + c_1 + ... + c_dv, msg = L + ext
 // and leaves their messages (the final state) untouched.  init_pass builds v2c from a
 // supplied state (BpDecoderImpl.cpp:81-88) for frames flagged PH_VNINIT.
-template <class F, int VMAX>
+template <class F, int VMAX, int VPW>
 __global__ __launch_bounds__(kBlock) void vn_kernel(DevCode c, DevState s, DevOut out, int fused_syn, int init_pass) {
-  constexpr int FPL = F::FPL, TW = F::TW, RW = F::RW;
+  constexpr int FPL = F::FPL, TW = F::TW, RW = F::RW, W = F::W;
   // Tiles in the opposite order to the check kernel: workgroups are dispatched in grid order,
   // so this pass starts on the tiles whose messages the check pass wrote last (still in the
   // 256 MB Infinity Cache), and the next check pass starts where this one ended.
   const int t = s.tiles - 1 - (int)blockIdx.y;
-  const int v = uniform(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6));
-  if (v >= c.N) return;
+  const int v0 = uniform((blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6)) * VPW);
+  if (v0 >= c.N) return;
   const int lane = threadIdx.x & 63;
-  // the tile's flags and the variable's edge list (one scalar load of VMAX ids) are requested
-  // together (the exit test reads both, which keeps the compiler from deferring the list load
-  // past it); channel values, frame state and rows follow at once
-  int eid[VMAX];
+  // the tile's flags and the variables' edge lists (VPW * VMAX ids; the table is padded to a
+  // whole number of waves) are requested together (the exit test reads both, which keeps the
+  // compiler from deferring the list load past it); channel values, frame state and rows
+  // follow at once
+  int eid[VPW][VMAX];
 #pragma unroll
-  for (int j = 0; j < VMAX; ++j) eid[j] = c.var_tab[v * VMAX + j];
+  for (int i = 0; i < VPW; ++i)
+#pragma unroll
+    for (int j = 0; j < VMAX; ++j) eid[i][j] = c.var_tab[(v0 + i) * VMAX + j];
   const uint32_t tflags = uniform((int)s.tile_flags[t]);
-  if (!(tflags & (init_pass ? TF_VNINIT : TF_LIVE)) || eid[0] < -1) return;
-  float Lv[FPL];
-  load_floats<FPL>(s.L + ((int64_t)t * c.N + v) * TW + lane * FPL, Lv);
-  int dv = 0;
+  if (!(tflags & (init_pass ? TF_VNINIT : TF_LIVE)) || eid[0][0] < -1) return;
+  float Lv[VPW][FPL];
 #pragma unroll
-  for (int j = 0; j < VMAX; ++j) dv += eid[j] >= 0 ? 1 : 0;
+  for (int i = 0; i < VPW; ++i)
+    if (v0 + i < c.N) load_floats<FPL>(s.L + ((int64_t)t * c.N + v0 + i) * TW + lane * FPL, Lv[i]);
+  int dv[VPW];
+#pragma unroll
+  for (int i = 0; i < VPW; ++i) {
+    dv[i] = 0;
+#pragma unroll
+    for (int j = 0; j < VMAX; ++j) dv[i] += eid[i][j] >= 0 ? 1 : 0;
+  }
   uint8_t ph[FPL];
   int itv[FPL];
   lane_state<FPL>(s, t, tflags, lane, ph, itv);
   uint64_t fw[FPL];
 #pragma unroll
   for (int k = 0; k < FPL; ++k) fw[k] = s.fail[t * FPL + k];
-  uint32_t* mt = s.msg + (int64_t)t * c.E * RW + lane * F::W;
-  float cv[VMAX][FPL];
+  uint32_t* mt = s.msg + (int64_t)t * c.E * RW + lane * W;
+  uint32_t wv[VPW][VMAX][W];
 #pragma unroll
-  for (int j = 0; j < VMAX; ++j)
-    if (j < dv) F::load(mt + (int64_t)eid[j] * RW, cv[j]);
+  for (int i = 0; i < VPW; ++i)
+#pragma unroll
+    for (int j = 0; j < VMAX; ++j)
+      if (j < dv[i]) load_words<W>(mt + (int64_t)eid[i][j] * RW, wv[i][j]);
   bool run[FPL], fin[FPL];
   bool lane_run = false, lane_fin = false;
 #pragma unroll
@@ -627,71 +640,84 @@ __global__ __launch_bounds__(kBlock) void vn_kernel(DevCode c, DevState s, DevOu
   const bool wave_run = __ballot(lane_run) != 0;
   const bool wave_fin = __ballot(lane_fin) != 0;
   if (!wave_run && !wave_fin) return;
-  if (wave_fin) {
-    float app[FPL], ext[FPL];
 #pragma unroll
-    for (int k = 0; k < FPL; ++k) { app[k] = Lv[k]; ext[k] = 0.0f; }
+  for (int i = 0; i < VPW; ++i) {
+    const int v = v0 + i;
+    if (v >= c.N) break;
+    float cv[VMAX][FPL];
 #pragma unroll
     for (int j = 0; j < VMAX; ++j) {
-      if (j < dv) {
+      if (j < dv[i]) {
 #pragma unroll
-        for (int k = 0; k < FPL; ++k) { app[k] = app[k] + cv[j][k]; ext[k] = ext[k] + cv[j][k]; }
+        for (int q = 0; q < W; ++q) F::Elem::unpack(wv[i][j][q], &cv[j][q * F::PW]);
       }
     }
-    if (v < c.K) {
+    if (wave_fin) {
+      float app[FPL], ext[FPL];
 #pragma unroll
-      for (int k = 0; k < FPL; ++k) {
-        const uint64_t bits = __ballot(app[k] >= 0.0f);
-        const uint64_t fm = __ballot(fin[k]);
-        if (lane == 0 && fm) {
-          uint64_t* w = &out.hard[((int64_t)t * c.K + v) * FPL + k];
-          *w = (*w & ~fm) | (bits & fm);
+      for (int k = 0; k < FPL; ++k) { app[k] = Lv[i][k]; ext[k] = 0.0f; }
+#pragma unroll
+      for (int j = 0; j < VMAX; ++j) {
+        if (j < dv[i]) {
+#pragma unroll
+          for (int k = 0; k < FPL; ++k) { app[k] = app[k] + cv[j][k]; ext[k] = ext[k] + cv[j][k]; }
         }
       }
-    }
-    if (out.ext) {
+      if (v < c.K) {
 #pragma unroll
-      for (int k = 0; k < FPL; ++k)
-        if (fin[k]) out.ext[((int64_t)t * c.N + v) * TW + lane * FPL + k] = ext[k];
-    }
-    if (out.app && v < c.K) {
-#pragma unroll
-      for (int k = 0; k < FPL; ++k)
-        if (fin[k]) out.app[((int64_t)t * c.K + v) * TW + lane * FPL + k] = Lv[k] + ext[k];
-    }
-  }
-  if (!wave_run) return;
-  float pre[VMAX][FPL];
-  float acc[FPL];
-#pragma unroll
-  for (int k = 0; k < FPL; ++k) acc[k] = 0.0f;
-#pragma unroll
-  for (int j = 0; j < VMAX; ++j) {
-    if (j < dv) {
-#pragma unroll
-      for (int k = 0; k < FPL; ++k) { pre[j][k] = acc[k]; acc[k] = acc[k] + cv[j][k]; }
-    }
-  }
-#pragma unroll
-  for (int k = 0; k < FPL; ++k) acc[k] = Lv[k];
-#pragma unroll
-  for (int j = VMAX - 1; j >= 0; --j) {
-    if (j < dv) {
-      float y[FPL];
-#pragma unroll
-      for (int k = 0; k < FPL; ++k) {
-        y[k] = pre[j][k] + acc[k];
-        acc[k] = acc[k] + cv[j][k];
-        if (!run[k]) y[k] = cv[j][k];   // other frames keep their word (exact round trip)
+        for (int k = 0; k < FPL; ++k) {
+          const uint64_t bits = __ballot(app[k] >= 0.0f);
+          const uint64_t fm = __ballot(fin[k]);
+          if (lane == 0 && fm) {
+            uint64_t* w = &out.hard[((int64_t)t * c.K + v) * FPL + k];
+            *w = (*w & ~fm) | (bits & fm);
+          }
+        }
       }
-      if (lane_run) F::store(mt + (int64_t)eid[j] * RW, y);
-    }
-  }
-  if (init_pass) return;
+      if (out.ext) {
 #pragma unroll
-  for (int k = 0; k < FPL; ++k) {
-    const uint64_t m = __ballot(acc[k] >= 0.0f);
-    if (lane == 0) s.hard[((int64_t)t * c.N + v) * FPL + k] = m;
+        for (int k = 0; k < FPL; ++k)
+          if (fin[k]) out.ext[((int64_t)t * c.N + v) * TW + lane * FPL + k] = ext[k];
+      }
+      if (out.app && v < c.K) {
+#pragma unroll
+        for (int k = 0; k < FPL; ++k)
+          if (fin[k]) out.app[((int64_t)t * c.K + v) * TW + lane * FPL + k] = Lv[i][k] + ext[k];
+      }
+    }
+    if (!wave_run) continue;
+    float pre[VMAX][FPL];
+    float acc[FPL];
+#pragma unroll
+    for (int k = 0; k < FPL; ++k) acc[k] = 0.0f;
+#pragma unroll
+    for (int j = 0; j < VMAX; ++j) {
+      if (j < dv[i]) {
+#pragma unroll
+        for (int k = 0; k < FPL; ++k) { pre[j][k] = acc[k]; acc[k] = acc[k] + cv[j][k]; }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < FPL; ++k) acc[k] = Lv[i][k];
+#pragma unroll
+    for (int j = VMAX - 1; j >= 0; --j) {
+      if (j < dv[i]) {
+        float y[FPL];
+#pragma unroll
+        for (int k = 0; k < FPL; ++k) {
+          y[k] = pre[j][k] + acc[k];
+          acc[k] = acc[k] + cv[j][k];
+          if (!run[k]) y[k] = cv[j][k];   // other frames keep their word (exact round trip)
+        }
+        if (lane_run) F::store(mt + (int64_t)eid[i][j] * RW, y);
+      }
+    }
+    if (init_pass) continue;
+#pragma unroll
+    for (int k = 0; k < FPL; ++k) {
+      const uint64_t m = __ballot(acc[k] >= 0.0f);
+      if (lane == 0) s.hard[((int64_t)t * c.N + v) * FPL + k] = m;
+    }
   }
 }
 
@@ -1013,24 +1039,42 @@ __global__ void fill_i32_kernel(int32_t* p, int64_t n, int32_t v) {
 // values fit the register budget (degree bound x frames per lane <= 64); the host picks a
 // layout that satisfies it (layout_fits()).
 
-template <class F>
-static hipError_t launch_vn(const DevCode& c, const DevState& s, const DevOut& o, int max_dv, int fused, int init,
-                            hipStream_t st) {
-  constexpr int FPL = F::FPL;
-  dim3 grid((c.N + kWavesPerBlock - 1) / kWavesPerBlock, s.tiles);
+template <class F, int VPW>
+static hipError_t launch_vn_w(const DevCode& c, const DevState& s, const DevOut& o, int max_dv, int fused, int init,
+                              hipStream_t st) {
+  constexpr int FPL = F::FPL, W = F::W;
+  const int per_block = kWavesPerBlock * VPW;
+  dim3 grid((c.N + per_block - 1) / per_block, s.tiles);
   if (c.vstride != var_table_stride(max_dv)) return hipErrorInvalidValue;
-  if (max_dv <= 4) hipLaunchKernelGGL((vn_kernel<F, 4>), grid, dim3(kBlock), 0, st, c, s, o, fused, init);
-  else if (max_dv <= 8) hipLaunchKernelGGL((vn_kernel<F, 8>), grid, dim3(kBlock), 0, st, c, s, o, fused, init);
-  else if (max_dv <= 16) {
-    if constexpr (16 * FPL <= 64) hipLaunchKernelGGL((vn_kernel<F, 16>), grid, dim3(kBlock), 0, st, c, s, o, fused, init);
+  // several variables per wave only where their packed rows fit (VPW * VMAX * W <= 32 words)
+  if (max_dv <= 4) {
+    if constexpr (VPW == 1 || VPW * 4 * W <= 32)
+      hipLaunchKernelGGL((vn_kernel<F, 4, VPW>), grid, dim3(kBlock), 0, st, c, s, o, fused, init);
+    else return hipErrorInvalidValue;
+  } else if (max_dv <= 8) {
+    if constexpr (VPW == 1 || VPW * 8 * W <= 32)
+      hipLaunchKernelGGL((vn_kernel<F, 8, VPW>), grid, dim3(kBlock), 0, st, c, s, o, fused, init);
+    else return hipErrorInvalidValue;
+  } else if (max_dv <= 16) {
+    if constexpr (16 * FPL <= 64 && (VPW == 1 || VPW * 16 * W <= 32))
+      hipLaunchKernelGGL((vn_kernel<F, 16, VPW>), grid, dim3(kBlock), 0, st, c, s, o, fused, init);
     else return hipErrorInvalidValue;
   } else if (max_dv <= 32) {
-    if constexpr (32 * FPL <= 64) hipLaunchKernelGGL((vn_kernel<F, 32>), grid, dim3(kBlock), 0, st, c, s, o, fused, init);
+    if constexpr (32 * FPL <= 64 && VPW == 1)
+      hipLaunchKernelGGL((vn_kernel<F, 32, VPW>), grid, dim3(kBlock), 0, st, c, s, o, fused, init);
     else return hipErrorInvalidValue;
   } else {
     return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+template <class F>
+static hipError_t launch_vn(const DevCode& c, const DevState& s, const DevOut& o, int max_dv, int fused, int init,
+                            int vpw, hipStream_t st) {
+  if (vpw == 2) return launch_vn_w<F, 2>(c, s, o, max_dv, fused, init, st);
+  if (vpw == 1) return launch_vn_w<F, 1>(c, s, o, max_dv, fused, init, st);
+  return hipErrorInvalidValue;
 }
 
 template <class F, int ALG>
@@ -1202,6 +1246,11 @@ bool layout_fits(const Layout& lay, int algorithm, int max_dc, int max_dv) {
   return max_dc <= 32 && degree_bound(max_dc) * fpl <= 64;
 }
 
+bool vpw_fits(const Layout& lay, int max_dv) {
+  if (lay.vpw == 1) return true;
+  return lay.vpw == 2 && max_dv <= 16 && 2 * degree_bound(max_dv) * lay.words <= 32;
+}
+
 hipError_t kernels_ingest(const Layout& lay, const DevCode& c, const DevState& s, const void* llr, const void* syst,
                           bool dbl, int64_t frames, int64_t frame0, uint8_t init_phase, hipStream_t st) {
   FECL_DISPATCH(lay, launch_ingest<F>(c, s, llr, syst, dbl, frames, frame0, init_phase, st));
@@ -1211,7 +1260,7 @@ hipError_t kernels_state_in(const Layout& lay, const DevCode& c, const DevState&
   FECL_DISPATCH(lay, launch_state_in<F>(c, s, state, dbl, frames, frame0, st));
 }
 hipError_t kernels_vn_init(const Layout& lay, const DevCode& c, const DevState& s, int max_dv, hipStream_t st) {
-  FECL_DISPATCH(lay, launch_vn<F>(c, s, DevOut{}, max_dv, 0, 1, st));
+  FECL_DISPATCH(lay, launch_vn<F>(c, s, DevOut{}, max_dv, 0, 1, lay.vpw, st));
 }
 hipError_t kernels_check(const Layout& lay, const DevCode& c, const DevState& s, const DevOpts& o, int max_dc,
                          hipStream_t st) {
@@ -1219,7 +1268,7 @@ hipError_t kernels_check(const Layout& lay, const DevCode& c, const DevState& s,
 }
 hipError_t kernels_variable(const Layout& lay, const DevCode& c, const DevState& s, const DevOut& out, int max_dv,
                             int fused, hipStream_t st) {
-  FECL_DISPATCH(lay, launch_vn<F>(c, s, out, max_dv, fused, 0, st));
+  FECL_DISPATCH(lay, launch_vn<F>(c, s, out, max_dv, fused, 0, lay.vpw, st));
 }
 hipError_t kernels_syndrome_control(const Layout& lay, const DevCode& c, const DevState& s, int iterations,
                                     int fused, int step, int max_dc, hipStream_t st) {

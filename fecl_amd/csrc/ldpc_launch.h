@@ -21,6 +21,7 @@ struct BatchResults {
 struct Layout {
   int fmt = 0;     // fecl_msg_format
   int words = 1;   // 1, 2 or 4
+  int vpw = 1;     // variables per wavefront in the variable kernel: 1 or 2
   int pw() const { return fmt == 0 ? 1 : 2; }        // messages per word
   int fpl() const { return words * pw(); }           // frames per lane
   int tw() const { return 64 * fpl(); }              // frames per tile
@@ -29,6 +30,9 @@ struct Layout {
 
 // Whether the kernels for this layout stay within the register budget for the code's degrees.
 bool layout_fits(const Layout& lay, int algorithm, int max_dc, int max_dv);
+
+// Whether vpw variables per wavefront fit the variable kernel's register budget.
+bool vpw_fits(const Layout& lay, int max_dv);
 
 // Degree bound of the variable kernel for a code (the stride of DevCode::var_tab).
 inline int var_table_stride(int max_dv) { return max_dv <= 4 ? 4 : max_dv <= 8 ? 8 : max_dv <= 16 ? 16 : 32; }

@@ -247,13 +247,23 @@ class Ldpc:
         check(lib().fecl_ldpc_get_layout(self._h, ctypes.byref(w), ctypes.byref(tw)))
         return w.value, tw.value
 
+    def setVarsPerWave(self, n):
+        """Variables one wavefront of the variable kernel serves (1, 2; 0 = automatic)."""
+        check(lib().fecl_ldpc_set_vars_per_wave(self._h, int(n)))
+
+    def varsPerWave(self):
+        v = ctypes.c_int32()
+        check(lib().fecl_ldpc_get_vars_per_wave(self._h, ctypes.byref(v)))
+        return v.value
+
     def autotune(self, frames, probe_iterations=10):
-        """Times each eligible layout on a frames-frame batch and pins the fastest.
-        Returns (words, {words: ms})."""
+        """Times each eligible (words, variables per wave) pair on a frames-frame batch and pins
+        the fastest.  Returns (words, {(words, vars_per_wave): ms})."""
         best = ctypes.c_int32()
-        ms = (ctypes.c_double * 3)()
+        ms = (ctypes.c_double * 6)()
         check(lib().fecl_ldpc_autotune(self._h, int(frames), int(probe_iterations), ctypes.byref(best), ms))
-        return best.value, {w: ms[i] for i, w in enumerate((1, 2, 4)) if ms[i] >= 0}
+        keys = [(w, v) for w in (1, 2, 4) for v in (1, 2)]
+        return best.value, {k: ms[i] for i, k in enumerate(keys) if ms[i] >= 0}
 
     # ---- encode / check (host)
     def encode(self, msg):

@@ -126,13 +126,19 @@ int fecl_ldpc_profile(fecl_ldpc* h, int32_t enable);
 int fecl_ldpc_profile_read(fecl_ldpc* h, double* ms /*[3]*/, int64_t* launches /*[3]*/);
 
 /* Device layout (no counterpart in the reference).  Frames are decoded in tiles of
- * 64 * words * (1 for FP32, 2 for 16-bit) frames; each GPU lane moves `words` 32-bit words per
- * edge.  set_layout pins words (1, 2, 4; 0 = widest that fits the code's degrees).
- * autotune times `probe_iterations` iterations of a `frames`-frame decode for every layout
- * that fits, pins the fastest and returns it; ms[3] (nullable) receives the times for
- * words 1, 2, 4 (-1 = not eligible).  Neither may run concurrently with a decode on the
- * same handle. */
+ * 64 * words * (1 for FP32, 2 for 16-bit) frames (the frames one workgroup serves); each GPU
+ * lane moves `words` 32-bit words per edge.  set_layout pins words (1, 2, 4; 0 = widest that
+ * fits the code's degrees).  set_vars_per_wave pins how many variables one wavefront of the
+ * variable kernel serves (1, 2; 0 = automatic, 1); 2 is refused (FECL_NOT_SUPPORTED) where
+ * it exceeds the register budget of the current layout, and falls back to 1 if a later
+ * layout change makes it not fit.  autotune times `probe_iterations` iterations of a
+ * `frames`-frame decode for every (words, variables per wave) pair that fits, pins the
+ * fastest and returns its words; ms[6] (nullable) receives the times in the order
+ * (1,1) (1,2) (2,1) (2,2) (4,1) (4,2) (-1 = not eligible).  None of these may run
+ * concurrently with a decode on the same handle. */
 int fecl_ldpc_set_layout(fecl_ldpc* h, int32_t words);
+int fecl_ldpc_set_vars_per_wave(fecl_ldpc* h, int32_t vars);
+int fecl_ldpc_get_vars_per_wave(const fecl_ldpc* h, int32_t* vars);
 /* Syndrome schedule (no counterpart in the reference; results are identical either way):
  * 0 (default) = a separate kernel after each variable pass; 1 = folded into the next check
  * kernel (used only when the scaling factors do not vary with the iteration). */
@@ -145,7 +151,7 @@ int fecl_ldpc_get_layout(const fecl_ldpc* h, int32_t* words, int32_t* frames_per
  * iterations.  0 (default) = 8192 frames per pass. */
 int fecl_ldpc_set_pass_frames(fecl_ldpc* h, int64_t frames);
 int fecl_ldpc_autotune(fecl_ldpc* h, int64_t frames, int32_t probe_iterations, int32_t* best_words,
-                       double* ms);
+                       double* ms /*[6]*/);
 
 /* Multi-code batch in one launch (no counterpart in the reference: FeCl decodes one code per
  * call).  Captures, for each i < count, a full decode of frames[i] frames of code handles[i]

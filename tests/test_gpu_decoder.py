@@ -92,6 +92,31 @@ def test_dvbs2_minsum_fp32_ragged_frames(frames, words, fused):
     assert np.array_equal(hard.reshape(frames, oc.K), ref["hard"])
 
 
+@pytest.mark.parametrize("fmt,ofmt,words", [(F.MsgFormat.F32, O.MSG_F32, 1), (F.MsgFormat.F32, O.MSG_F32, 2),
+                                             (F.MsgFormat.F16, O.MSG_F16, 1), (F.MsgFormat.F16, O.MSG_F16, 2)])
+def test_two_variables_per_wave(fmt, ofmt, words):
+    """The variable kernel serving two variables per wavefront gives the same bits, including
+    a code with an odd variable count (the Gallager code without its last column), whose last
+    wavefront serves one variable."""
+    oc, H = dvbs2(16200, (1, 2))
+    llr = dvbs2_llr(70, 1.6, seed=17, rate=(1, 2), n=16200).astype(np.float32).astype(np.float64)
+    ref = oracle_run(oc, llr, F.Approximate, 30, fmt=ofmt)
+    dec = gpu_decoder(H, F.Approximate, 30, fmt=fmt, words=words)
+    dec.setVarsPerWave(2)
+    assert dec.varsPerWave() == 2
+    assert_bit_exact(dec.soDecode(llr), ref, 70, oc.K, oc.N, oc.E)
+    g, _ = gallager()
+    keep = g.col < g.N - 1
+    row_ptr = np.concatenate([[0], np.cumsum([int(keep[a:b].sum()) for a, b in zip(g.row_ptr[:-1], g.row_ptr[1:])])])
+    oc2 = O.Code(row_ptr, g.col[keep], g.N - 1, g.N - 1 - g.M)
+    H2 = F.CheckMatrix(oc2.row_ptr, oc2.col, oc2.N)
+    llr2 = gallager_llr(oc2.N, oc2.K, 9, 2.0, seed=5)
+    ref2 = oracle_run(oc2, llr2, F.Linear, 20, fmt=ofmt)
+    dec2 = gpu_decoder(H2, F.Linear, 20, fmt=fmt, words=words)
+    dec2.setVarsPerWave(2)
+    assert_bit_exact(dec2.soDecode(llr2), ref2, 9, oc2.K, oc2.N, oc2.E)
+
+
 @pytest.mark.parametrize("pass_frames", [0, 256, 300])
 def test_dvbs2_minsum_fp32_1024_frames_device_path(pass_frames):
     """The config-2 shape: 1024 frames of DVB-S2 r1/2, FP32 LLRs in HBM, min-sum, 50 iterations;
@@ -179,7 +204,7 @@ def test_autotune_picks_an_eligible_layout():
     oc, H = dvbs2()
     dec = gpu_decoder(H, F.Approximate, 50)
     best, times = dec.autotune(256, probe_iterations=3)
-    assert best in times and dec.layout()[0] == best
+    assert (best, dec.varsPerWave()) in times and dec.layout()[0] == best
     llr = dvbs2_llr(2, 1.4, seed=3).astype(np.float32).astype(np.float64)
     ref = oracle_run(oc, llr, F.Approximate, 50)
     hard, it = dec.decode(llr, iterations=True)
