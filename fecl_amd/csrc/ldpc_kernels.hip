@@ -344,15 +344,52 @@ __global__ __launch_bounds__(kBlock) void cn_minsum_kernel(DevCode c, DevState s
   }
 }
 
-// Min-sum on 16-bit messages without scaling factors: every output is one of the row's stored
-// values with a new sign, so the update runs on the raw encodings, two frames per 32-bit
-// word.  For finite FP16 and BF16 values the magnitude order is the integer order of
-// (bits & 0x7fff), and the stored values are finite (storage saturates).  Same results as
-// cn_minsum_kernel, bit for bit.
+// Min-sum without scaling factors: every output is one of the row's stored values with a new
+// sign, so the update runs on the raw encodings with integer min/max (for FP16/BF16, two frames
+// per 32-bit word).  For non-NaN values the magnitude order is the integer order of the bits
+// without the sign bit, and min-sum never forms a NaN.  Same results as cn_minsum_kernel, bit
+// for bit.
+template <class F>
+struct RawMinSum;
+template <int W>
+struct RawMinSum<Fmt<ElemF32, W>> {
+  using M = uint32_t;
+  static constexpr uint32_t kMag = 0x7fffffffu, kSign = 0x80000000u;
+  __device__ static inline M mag(uint32_t w) { return w & kMag; }
+  __device__ static inline M init() { return kMag; }
+  __device__ static inline void fold(M& m1, M& m2, M a) { m2 = min(m2, max(m1, a)); m1 = min(m1, a); }
+  // |x_j| == m1 -> m2, else m1
+  __device__ static inline uint32_t pick(M a, M m1, M m2) { return a == m1 ? m2 : m1; }
+  __device__ static inline uint32_t lanes(const bool* f) { return f[0] ? 0xffffffffu : 0u; }
+};
+template <class E, int W>
+struct RawMinSum16 {
+  using M = u16x2;
+  static constexpr uint32_t kMag = 0x7fff7fffu, kSign = 0x80008000u;
+  __device__ static inline M mag(uint32_t w) { return __builtin_bit_cast(u16x2, w & kMag); }
+  __device__ static inline M init() { return u16x2{0x7fff, 0x7fff}; }
+  __device__ static inline void fold(M& m1, M& m2, M a) {
+    m2 = __builtin_elementwise_min(m2, __builtin_elementwise_max(m1, a));
+    m1 = __builtin_elementwise_min(m1, a);
+  }
+  // a >= m1, so a - m1 does not wrap; 1 - (a - m1) saturates to 1 where a == m1, else 0
+  __device__ static inline uint32_t pick(M a, M m1, M m2) {
+    const u16x2 one = {1, 1};
+    return __builtin_bit_cast(uint32_t, m1 + __builtin_elementwise_sub_sat(one, a - m1) * (m2 - m1));
+  }
+  __device__ static inline uint32_t lanes(const bool* f) { return (f[0] ? 0x0000ffffu : 0u) | (f[1] ? 0xffff0000u : 0u); }
+};
+template <int W>
+struct RawMinSum<Fmt<ElemF16, W>> : RawMinSum16<ElemF16, W> {};
+template <int W>
+struct RawMinSum<Fmt<ElemBF16, W>> : RawMinSum16<ElemBF16, W> {};
+
 template <class F, int DMAX>
-__global__ __launch_bounds__(kBlock) void cn_minsum16_kernel(DevCode c, DevState s, DevOpts o) {
-  static_assert(F::PW == 2 && DMAX > 0, "16-bit formats, register-staged rows");
-  constexpr int FPL = F::FPL, TW = F::TW, RW = F::RW, W = F::W;
+__global__ __launch_bounds__(kBlock) void cn_minsum_raw_kernel(DevCode c, DevState s, DevOpts o) {
+  static_assert(DMAX > 0, "register-staged rows");
+  using R = RawMinSum<F>;
+  using M = typename R::M;
+  constexpr int FPL = F::FPL, TW = F::TW, RW = F::RW, W = F::W, PW = F::PW;
   const int t = blockIdx.y;
   const int r = uniform(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6));
   if (r >= c.M) return;
@@ -383,52 +420,41 @@ __global__ __launch_bounds__(kBlock) void cn_minsum16_kernel(DevCode c, DevState
         load_floats<FPL>(Lt + (int64_t)v * TW, l);
 #pragma unroll
         for (int i = 0; i < W; ++i) {
-          const uint32_t q = F::Elem::pack(&l[2 * i]);
-          const uint32_t fm = (ln.fresh[2 * i] ? 0x0000ffffu : 0u) | (ln.fresh[2 * i + 1] ? 0xffff0000u : 0u);
+          const uint32_t q = F::Elem::pack(&l[PW * i]);
+          const uint32_t fm = R::lanes(&ln.fresh[PW * i]);
           w[j][i] = (q & fm) | (w[j][i] & ~fm);
         }
       }
     }
   }
   if (o.fused_syn && ln.wave_seen) cn_syndrome<FPL>(c, s, t, e0, d, lane);
-  u16x2 m1[W], m2[W];
+  M m1[W], m2[W];
   uint32_t sx[W];
 #pragma unroll
-  for (int i = 0; i < W; ++i) { m1[i] = u16x2{0x7fff, 0x7fff}; m2[i] = m1[i]; sx[i] = 0; }
+  for (int i = 0; i < W; ++i) { m1[i] = R::init(); m2[i] = m1[i]; sx[i] = 0; }
 #pragma unroll
   for (int j = 0; j < DMAX; ++j) {
     if (j < d) {
 #pragma unroll
       for (int i = 0; i < W; ++i) {
-        const u16x2 a = __builtin_bit_cast(u16x2, w[j][i] & 0x7fff7fffu);
-        m2[i] = __builtin_elementwise_min(m2[i], __builtin_elementwise_max(m1[i], a));
-        m1[i] = __builtin_elementwise_min(m1[i], a);
+        R::fold(m1[i], m2[i], R::mag(w[j][i]));
         sx[i] ^= w[j][i];
       }
     }
   }
   if (!ln.lane_any) return;
-  const uint32_t flip = ((d - 2) & 1) ? 0x80008000u : 0u;
+  const uint32_t flip = ((d - 2) & 1) ? R::kSign : 0u;
   uint32_t am[W];
-  u16x2 dm[W];
 #pragma unroll
-  for (int i = 0; i < W; ++i) {
-    am[i] = (ln.act[2 * i] ? 0x0000ffffu : 0u) | (ln.act[2 * i + 1] ? 0xffff0000u : 0u);
-    dm[i] = m2[i] - m1[i];
-  }
-  const u16x2 one = {1, 1};
+  for (int i = 0; i < W; ++i) am[i] = R::lanes(&ln.act[PW * i]);
 #pragma unroll
   for (int j = 0; j < DMAX; ++j) {
     if (j < d) {
       uint32_t y[W];
 #pragma unroll
       for (int i = 0; i < W; ++i) {
-        const u16x2 a = __builtin_bit_cast(u16x2, w[j][i] & 0x7fff7fffu);
-        // |x_j| == m1 -> m2, else m1   (a >= m1, so a - m1 does not wrap)
-        const u16x2 is_min = __builtin_elementwise_sub_sat(one, a - m1[i]);
-        const uint32_t mag = __builtin_bit_cast(uint32_t, m1[i] + is_min * dm[i]);
-        const uint32_t out = mag | ((sx[i] ^ flip ^ w[j][i]) & 0x80008000u);
-        y[i] = (out & am[i]) | (w[j][i] & ~am[i]);
+        const uint32_t out = R::pick(R::mag(w[j][i]), m1[i], m2[i]) | ((sx[i] ^ flip ^ w[j][i]) & R::kSign);
+        y[i] = (out & am[i]) | (w[j][i] & ~am[i]);   // frames without an update keep their word
       }
       store_words<W>(mrow + (int64_t)j * RW, y);
     }
@@ -1102,25 +1128,23 @@ static hipError_t launch_cn(const DevCode& c, const DevState& s, const DevOpts& 
   dim3 grid((c.M + kWavesPerBlock - 1) / kWavesPerBlock, s.tiles);
   if (o.algorithm == 2) {
     bool done = false;
-    if constexpr (F::PW == 2) {
-      if (o.sf_mode == 0) {   // raw 16-bit encodings (no scaling factor to apply)
-        constexpr int W = F::W;
-        if (max_dc <= 8) {
-          hipLaunchKernelGGL((cn_minsum16_kernel<F, 8>), grid, dim3(kBlock), 0, st, c, s, o);
+    if (o.sf_mode == 0) {   // raw encodings (no scaling factor to apply)
+      constexpr int W = F::W;
+      if (max_dc <= 8) {
+        hipLaunchKernelGGL((cn_minsum_raw_kernel<F, 8>), grid, dim3(kBlock), 0, st, c, s, o);
+        done = true;
+      } else if (max_dc <= 16) {
+        if constexpr (16 * W <= 64) {
+          hipLaunchKernelGGL((cn_minsum_raw_kernel<F, 16>), grid, dim3(kBlock), 0, st, c, s, o);
           done = true;
-        } else if (max_dc <= 16) {
-          if constexpr (16 * W <= 64) {
-            hipLaunchKernelGGL((cn_minsum16_kernel<F, 16>), grid, dim3(kBlock), 0, st, c, s, o);
-            done = true;
-          }
-        } else if (max_dc <= 32) {
-          if constexpr (32 * W <= 64) {
-            hipLaunchKernelGGL((cn_minsum16_kernel<F, 32>), grid, dim3(kBlock), 0, st, c, s, o);
-            done = true;
-          }
         }
-        if (done) return hipGetLastError();
+      } else if (max_dc <= 32) {
+        if constexpr (32 * W <= 64) {
+          hipLaunchKernelGGL((cn_minsum_raw_kernel<F, 32>), grid, dim3(kBlock), 0, st, c, s, o);
+          done = true;
+        }
       }
+      if (done) return hipGetLastError();
     }
     if (max_dc <= 8) {
       hipLaunchKernelGGL((cn_minsum_kernel<F, 8>), grid, dim3(kBlock), 0, st, c, s, o);
