@@ -191,7 +191,7 @@ class ServingSetup:
         gen = torch.Generator(device=device)
         gen.manual_seed(args.seed + 1)
         self.rates = list(RATE_POINTS)
-        self.decs, self.pools, self.infos = [], [], []
+        self.decs, self.pools, self.infos, self.pinned = [], [], [], []
         for rate in self.rates:
             dec = F.Ldpc(F.DvbS2.matrix(64800, rate),
                          F.DecoderOptions().algorithm(F.Approximate).iterations(args.iterations)
@@ -200,7 +200,10 @@ class ServingSetup:
             idx = np.arange(self.POOL) % 256
             ebn0 = torch.from_numpy(RATE_POINTS[rate] + rng.uniform(-0.3, 0.3, self.POOL)).to(device)
             llr = awgn_llr_gpu(torch.from_numpy(cw[idx]).to(device), ebn0, dec.msgSize() / dec.paritySize(), gen)
-            self.pools.append(llr.cpu().numpy())
+            # page-locked host memory: the server copies admitted frames straight from it
+            host = llr.cpu().pin_memory()
+            self.pinned.append(host)
+            self.pools.append(host.numpy())
             self.infos.append(info[idx])
             self.decs.append(dec)
 
@@ -357,6 +360,42 @@ def copy_reference(device, gib=1, reps=5):
     return {"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "reps": reps}
 
 
+def time_form(F, H, alg, fmt, iterations, batches, frames, K, steps, device):
+    """Gb/s, FER and kernel GB/s of one algorithm / message format on the bench batches
+    (autotuned layout, one warm-up decode, then `steps` timed decodes)."""
+    import torch
+    from fecl_amd._lib import lib, check
+    import ctypes
+    dec = F.Ldpc(H, F.DecoderOptions().algorithm(alg).iterations(iterations).msgFormat(fmt))
+    dec.autotune(frames, probe_iterations=3)
+    stream = torch.cuda.current_stream(device).cuda_stream
+    dec.decode(batches[0]["llr"], stream=stream)
+    torch.cuda.synchronize()
+    check(lib().fecl_ldpc_profile(dec._h, 1))
+    errs, its = 0, 0
+    t0 = time.perf_counter()
+    outs = []
+    for i in range(steps):
+        b = batches[i % len(batches)]
+        outs.append((b, dec.decode(b["llr"], iterations=True, stream=stream)))
+    torch.cuda.synchronize()
+    total = time.perf_counter() - t0
+    ms = (ctypes.c_double * 3)()
+    launches = (ctypes.c_int64 * 3)()
+    check(lib().fecl_ldpc_profile_read(dec._h, ms, launches))
+    check(lib().fecl_ldpc_profile(dec._h, 0))
+    for b, (hard, it) in outs:
+        errs += int(torch.any(hard.view(frames, K) != b["info"], dim=1).sum())
+        its += int(it.to(torch.int64).sum())
+    E, N = dec.stateSize(), dec.paritySize()
+    w = 4 if fmt == F.MsgFormat.F32 else 2
+    vn_updates = its - steps * frames
+    return {"Gb_per_s": round(steps * frames * K / total / 1e9, 4), "fer": round(errs / (steps * frames), 5),
+            "lane_words": dec.layout()[0], "steps": steps,
+            "cn_GBs": round(its * 2 * E * w / (ms[0] / 1e3) / 1e9, 1) if ms[0] > 0 else None,
+            "vn_GBs": round(vn_updates * (2 * E * w + 4 * N) / (ms[1] / 1e3) / 1e9, 1) if ms[1] > 0 else None}
+
+
 def fp64_parity(llr_host, info_host, iters, threads, gpu_hard):
     """FER parity on the sample: FeCl's FP64 decoder (the oracle's bit-identical restatement)
     against the GPU's hard decisions on the same frames."""
@@ -424,6 +463,7 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=2048)
     ap.add_argument("--offered-fps", type=float, default=200000.0)
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-forms", action="store_true", help="skip the Linear / FP16 / BF16 lines")
     ap.add_argument("--pass-frames", type=int, default=0, help="frames per decode pass (0 = library default)")
     ap.add_argument("--vpw", type=int, default=0, help="variables per wavefront (0 automatic; with --words >= 0)")
     args = ap.parse_args()
@@ -567,6 +607,18 @@ def main():
                      "launch_ms": round(dom_launch_ms, 4), "algorithmic_bytes_per_launch": int(dom_bytes_per),
                      "cn_GBs": round(cn_gbs, 1), "vn_GBs": round(vn_gbs, 1)},
     }
+
+    if not args.no_forms:
+        # the same batches through the other forms of the path (fewer timed decodes each)
+        forms = {}
+        for name, a2, f2 in (("linear_f32", F.Linear, F.MsgFormat.F32), ("minsum_f16", F.Approximate, F.MsgFormat.F16),
+                             ("minsum_bf16", F.Approximate, F.MsgFormat.BF16)):
+            if a2 == alg and f2 == fmt:
+                continue
+            forms[name] = time_form(F, H, a2, f2, args.iterations, batches, args.frames, K, max(2, args.steps // 3),
+                                    device)
+            print(f"[bench] {name}: {forms[name]}", file=sys.stderr, flush=True)
+        result["other_forms"] = forms
 
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)

@@ -1270,6 +1270,12 @@ bool layout_fits(const Layout& lay, int algorithm, int max_dc, int max_dv) {
   return max_dc <= 32 && degree_bound(max_dc) * fpl <= 64;
 }
 
+bool cn_register_staged(const Layout& lay, int algorithm, int sf_mode, int max_dc) {
+  const int db = degree_bound(max_dc);
+  if (algorithm == 2 && sf_mode == 0) return db * lay.words <= 64;   // raw-encoding min-sum
+  return db * lay.fpl() <= 64;
+}
+
 bool vpw_fits(const Layout& lay, int max_dv) {
   if (lay.vpw == 1) return true;
   return lay.vpw == 2 && max_dv <= 16 && 2 * degree_bound(max_dv) * lay.words <= 32;

@@ -31,6 +31,10 @@ struct Layout {
 // Whether the kernels for this layout stay within the register budget for the code's degrees.
 bool layout_fits(const Layout& lay, int algorithm, int max_dc, int max_dv);
 
+// Whether the check kernel for this layout holds a row in registers (rather than streaming it
+// twice from memory, the fallback for long rows).
+bool cn_register_staged(const Layout& lay, int algorithm, int sf_mode, int max_dc);
+
 // Whether vpw variables per wavefront fit the variable kernel's register budget.
 bool vpw_fits(const Layout& lay, int max_dv);
 

@@ -17,6 +17,7 @@
 #include <deque>
 #include <memory>
 #include <mutex>
+#include <queue>
 #include <thread>
 #include <unordered_map>
 #include <vector>
@@ -32,6 +33,7 @@ struct Request {
   uint64_t id = 0;
   int pool = 0;
   const float* llr = nullptr;   // host [frames][N]
+  bool pinned = false;          // llr is page-locked: admission copies it straight to the device
   int64_t frames = 0;
   uint8_t* msg = nullptr;       // host [frames][K]
   int32_t* iters = nullptr;     // host [frames] or null
@@ -67,7 +69,9 @@ struct Pool {
   DevOut out{};
   int64_t nslots = 0;
   int admit_cap = 0;
-  std::vector<int32_t> free_slots;
+  // free slots, lowest first: live frames stay packed into few tiles, and tiles without a
+  // live frame cost the kernels one flag load
+  std::priority_queue<int32_t, std::vector<int32_t>, std::greater<int32_t>> free_slots;
   std::vector<std::pair<Request*, int64_t>> owner;  // slot -> (request, frame)
   int64_t live = 0;
   std::deque<Request*> queue;
@@ -129,8 +133,8 @@ int server_round(fecl_ldpc_server* sv, bool* idle) {
                                              (int64_t)(P.admit_cap - n)});
         int32_t* slots = P.h_admit.as<int32_t>();
         for (int64_t i = 0; i < k; ++i) {
-          const int32_t slot = P.free_slots.back();
-          P.free_slots.pop_back();
+          const int32_t slot = P.free_slots.top();
+          P.free_slots.pop();
           slots[n + i] = slot;
           P.owner[slot] = {r, r->admitted + i};
         }
@@ -151,12 +155,30 @@ int server_round(fecl_ldpc_server* sv, bool* idle) {
     Pool& P = *sv->pools[pi];
     int n = 0;
     const int64_t N = P.h->code.N;
+    int64_t staged = 0;   // frames in the pinned staging buffer, copied in one transfer
     for (const Admit& a : admits[pi]) {
-      std::memcpy(P.h_stage.as<float>() + (int64_t)n * N, a.req->llr + a.f0 * N, (size_t)(a.n * N) * 4);
+      if (a.req->pinned) {
+        HIP_TRY(hipMemcpyAsync(P.d_stage.as<float>() + (int64_t)n * N, a.req->llr + a.f0 * N, (size_t)(a.n * N) * 4,
+                               hipMemcpyHostToDevice, sv->st));
+      } else {
+        std::memcpy(P.h_stage.as<float>() + staged * N, a.req->llr + a.f0 * N, (size_t)(a.n * N) * 4);
+        staged += a.n;
+      }
       n += (int)a.n;
     }
     if (!n) continue;
-    HIP_TRY(hipMemcpyAsync(P.d_stage.p, P.h_stage.p, (size_t)n * N * 4, hipMemcpyHostToDevice, sv->st));
+    // pageable frames were packed into h_stage in order; place each run at its offset
+    if (staged) {
+      int64_t off = 0, src = 0;
+      for (const Admit& a : admits[pi]) {
+        if (!a.req->pinned) {
+          HIP_TRY(hipMemcpyAsync(P.d_stage.as<float>() + off * N, P.h_stage.as<float>() + src * N, (size_t)(a.n * N) * 4,
+                                 hipMemcpyHostToDevice, sv->st));
+          src += a.n;
+        }
+        off += a.n;
+      }
+    }
     HIP_TRY(hipMemcpyAsync(P.d_admit.p, P.h_admit.p, (size_t)n * 4, hipMemcpyHostToDevice, sv->st));
     const uint8_t init = (uint8_t)((P.h->iterations == 1 ? PH_FINAL : PH_RUN) | PH_FRESH);
     HIP_TRY(kernels_admit(P.lay, P.h->dc, P.s, P.d_stage.as<float>(), P.d_admit.as<int32_t>(), n, init, sv->st));
@@ -207,7 +229,7 @@ int server_round(fecl_ldpc_server* sv, bool* idle) {
         std::memcpy(r->msg + f * K, P.h_ext_bits.as<uint8_t>() + (int64_t)i * K, (size_t)K);
         if (r->iters) r->iters[f] = P.h_ext_iters.as<int32_t>()[i];
         P.owner[slot] = {nullptr, 0};
-        P.free_slots.push_back(slot);
+        P.free_slots.push(slot);
         --P.live;
         ++sv->frames_done;
         if (++r->done == r->frames) {
@@ -310,7 +332,7 @@ int fecl_ldpc_server_create(fecl_ldpc* const* handles, int32_t count, const int6
     HIP_TRY(P->h_ext_iters.ensure((size_t)P->nslots * 4));
     HIP_TRY(P->h_phase.ensure((size_t)P->nslots));
     P->owner.assign((size_t)P->nslots, {nullptr, 0});
-    for (int64_t s = P->nslots - 1; s >= 0; --s) P->free_slots.push_back((int32_t)s);
+    for (int64_t s = 0; s < P->nslots; ++s) P->free_slots.push((int32_t)s);
     sv->pools.push_back(std::move(P));
   }
   HIP_TRY(hipDeviceSynchronize());
@@ -337,6 +359,11 @@ int fecl_ldpc_server_submit(fecl_ldpc_server* sv, int32_t code, const float* llr
   r->frames = frames;
   r->msg = msg;
   r->iters = iterations;
+  {
+    hipPointerAttribute_t attr{};
+    r->pinned = hipPointerGetAttributes(&attr, llr) == hipSuccess && attr.type == hipMemoryTypeHost;
+    (void)hipGetLastError();   // a pageable pointer is not an error here
+  }
   r->t_submit = Clock::now();
   {
     std::lock_guard<std::mutex> lk(sv->mu);

@@ -23,11 +23,21 @@ def decoders(fmt=F.MsgFormat.F16, iters=50):
     return out
 
 
-def test_server_results_same_alone_and_in_the_mix():
+@pytest.mark.parametrize("pinned", [False, True])
+def test_server_results_same_alone_and_in_the_mix(pinned):
+    """pinned: the LLRs live in page-locked host memory, which the server copies to the device
+    directly instead of through its staging buffer."""
+    import torch
     decs = decoders()
     rng = np.random.default_rng(5)
     pool = {i: dvbs2_llr(200, CODES[i][2], seed=50 + i, rate=CODES[i][1], n=CODES[i][0]).astype(np.float32)
             for i in range(len(CODES))}
+    keep = []
+    if pinned:
+        for i in pool:
+            t = torch.from_numpy(pool[i]).pin_memory()
+            keep.append(t)
+            pool[i] = t.numpy()
     reqs = []
     for _ in range(40):
         i = int(rng.integers(0, len(CODES)))
