@@ -21,6 +21,7 @@ SIGNATURES = {
     "fecl_ldpc_get_options": [P, P, P, I64],
     "fecl_ldpc_sizes": [P, P, P, P, P],
     "fecl_ldpc_encode": [P, P, I64, P],
+    "fecl_ldpc_encode_device": [P, P, I64, P, P],
     "fecl_ldpc_check": [P, P, I64, P],
     "fecl_ldpc_decode": [P, P, I64, P, P],
     "fecl_ldpc_sodecode": [P, P, P, P, I64, P, P, P, P, P],

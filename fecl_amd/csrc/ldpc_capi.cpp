@@ -509,6 +509,17 @@ int fecl_ldpc_encode(const fecl_ldpc* h, const uint8_t* msg, int64_t frames, uin
   return FECL_OK;
 }
 
+int fecl_ldpc_encode_device(fecl_ldpc* h, const uint8_t* msg, int64_t frames, uint8_t* parity, void* stream) {
+  if (int rc = check_handle(h)) return rc;
+  if (frames < 0 || (frames > 0 && (!msg || !parity))) return fail(FECL_INVALID_ARGUMENT, "Invalid size for message");
+  if (!h->code.staircase) return fail(FECL_NOT_SUPPORTED, "encoder supports staircase (DVB-S2 type) codes only");
+  if (h->code.N > 65536) return fail(FECL_NOT_SUPPORTED, "device encoder holds a frame in 64 KiB of LDS");
+  if (frames == 0) return FECL_OK;
+  HIP_TRY(hipSetDevice(h->device));
+  HIP_TRY(kernels_encode(h->dc, msg, parity, frames, (hipStream_t)stream));
+  return FECL_OK;
+}
+
 int fecl_ldpc_check(const fecl_ldpc* h, const uint8_t* parity, int64_t frames, int32_t* ok) {
   if (int rc = check_handle(h)) return rc;
   if (!ok || frames < 0 || (frames > 0 && !parity)) return fail(FECL_INVALID_ARGUMENT, "Invalid size for parity");

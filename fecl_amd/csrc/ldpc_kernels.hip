@@ -1013,16 +1013,30 @@ __global__ __launch_bounds__(kBlock) void state_out_kernel(DevCode c, DevState s
 
 // Serving: frames admitted into free slots.  stage holds their LLRs frame-major [n][N];
 // slot i of the list receives frame i (L rows written at the slot's position in its tile).
+// A block moves 64 frames x 64 variables through LDS: reads run along the variables of a
+// frame, writes along the slots of a variable, which are adjacent when the slots are.
 template <int TW>
 __global__ __launch_bounds__(kBlock) void admit_kernel(DevCode c, DevState s, const float* stage,
                                                        const int32_t* slots, int n, uint8_t init_phase) {
-  const int i = blockIdx.y;
-  if (i >= n) return;
-  const int p = slots[i];
-  const int t = p / TW, pos = p % TW;
-  for (int v = blockIdx.x * kBlock + threadIdx.x; v < c.N; v += gridDim.x * kBlock)
-    s.L[((int64_t)t * c.N + v) * TW + pos] = stage[(int64_t)i * c.N + v];
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  __shared__ float tile[64][65];
+  const int v0 = blockIdx.x * 64, f0 = blockIdx.y * 64;
+  for (int i = threadIdx.x; i < 64 * 64; i += kBlock) {
+    const int fl = i / 64, vl = i % 64;
+    const int f = f0 + fl, v = v0 + vl;
+    tile[fl][vl] = (f < n && v < c.N) ? stage[(int64_t)f * c.N + v] : 0.0f;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < 64 * 64; i += kBlock) {
+    const int vl = i / 64, fl = i % 64;
+    const int f = f0 + fl, v = v0 + vl;
+    if (f < n && v < c.N) {
+      const int p = slots[f];
+      s.L[((int64_t)(p / TW) * c.N + v) * TW + p % TW] = tile[fl][vl];
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x < 64 && f0 + (int)threadIdx.x < n) {
+    const int p = slots[f0 + threadIdx.x];
+    const int t = p / TW;
     s.phase[p] = init_phase;
     s.iter[p] = 0;
     const uint8_t base = init_phase & PH_MASK;
@@ -1058,6 +1072,60 @@ __global__ void iters_out_kernel(const int32_t* iter, int64_t frames, int32_t* o
 __global__ void fill_i32_kernel(int32_t* p, int64_t n, int32_t v) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
+}
+
+// ------------------------------------------------------------------------------------------
+// Systematic encoder for staircase codes (H = [A | dual diagonal], DVB-S2): codeword =
+// [msg | p] with p_r = p_{r-1} xor (xor of the message bits in row r), the accumulator form
+// of FeCl's generator transform for these codes (src/detail/Ldpc.cpp:180-209).  One workgroup
+// per frame: the frame's bits sit in LDS (N bytes), each thread folds a run of consecutive
+// rows, a workgroup xor-scan of the runs' parities gives every run its starting p, and a
+// second pass writes the parity bits; input and output move coalesced.
+__global__ __launch_bounds__(kBlock) void encode_kernel(DevCode c, const uint8_t* msg, uint8_t* cw) {
+  extern __shared__ uint8_t bits[];   // [0, K) message, [K, N) parity
+  __shared__ uint32_t carry[kWavesPerBlock];
+  const int64_t f = blockIdx.x;
+  const uint8_t* m = msg + f * c.K;
+  for (int i = threadIdx.x; i < c.K; i += kBlock) bits[i] = m[i] & 1;
+  __syncthreads();
+  const int run = (c.M + kBlock - 1) / kBlock;
+  const int r0 = threadIdx.x * run, r1 = min(c.M, r0 + run);
+  auto row_parity = [&](int r) {
+    uint32_t x = 0;
+    for (int e = c.row_ptr[r]; e < c.row_ptr[r + 1]; ++e) {
+      const int v = c.col[e];
+      if (v < c.K) x ^= bits[v];
+    }
+    return x;
+  };
+  uint32_t tot = 0;
+  for (int r = r0; r < r1; ++r) tot ^= row_parity(r);
+  // exclusive xor-scan of the runs: within the wave by shuffles, then across the waves
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  uint32_t inc = tot;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(inc, o, 64);
+    if (lane >= o) inc ^= y;
+  }
+  if (lane == 63) carry[wv] = inc;
+  __syncthreads();
+  uint32_t x = inc ^ tot;   // exclusive within the wave
+  for (int w = 0; w < wv; ++w) x ^= carry[w];
+  for (int r = r0; r < r1; ++r) {
+    x ^= row_parity(r);
+    bits[c.K + r] = (uint8_t)x;
+  }
+  __syncthreads();
+  uint8_t* out = cw + f * c.N;
+  for (int i = threadIdx.x; i < c.N; i += kBlock) out[i] = bits[i];
+}
+
+hipError_t kernels_encode(const DevCode& c, const uint8_t* msg, uint8_t* cw, int64_t frames, hipStream_t st) {
+  if (frames <= 0) return hipSuccess;
+  if (c.N > 65536) return hipErrorInvalidValue;   // one frame's bits in LDS
+  hipLaunchKernelGGL(encode_kernel, dim3((unsigned)frames), dim3(kBlock), (size_t)c.N, st, c, msg, cw);
+  return hipGetLastError();
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1313,7 +1381,7 @@ hipError_t kernels_syndrome_control(const Layout& lay, const DevCode& c, const D
 hipError_t kernels_admit(const Layout& lay, const DevCode& c, const DevState& s, const float* stage,
                          const int32_t* slots, int n, uint8_t init_phase, hipStream_t st) {
   if (n <= 0) return hipSuccess;
-  dim3 grid((unsigned)std::min<int64_t>((c.N + kBlock - 1) / kBlock, 64), (unsigned)n);
+  dim3 grid((unsigned)((c.N + 63) / 64), (unsigned)((n + 63) / 64));
   switch (lay.tw()) {
     case 64: hipLaunchKernelGGL((admit_kernel<64>), grid, dim3(kBlock), 0, st, c, s, stage, slots, n, init_phase); break;
     case 128: hipLaunchKernelGGL((admit_kernel<128>), grid, dim3(kBlock), 0, st, c, s, stage, slots, n, init_phase); break;

@@ -42,6 +42,8 @@ bool vpw_fits(const Layout& lay, int max_dv);
 inline int var_table_stride(int max_dv) { return max_dv <= 4 ? 4 : max_dv <= 8 ? 8 : max_dv <= 16 ? 16 : 32; }
 
 hipError_t kernels_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t st);
+// Systematic staircase encoder: msg [frames][K] (0/1 bytes) -> codeword [frames][N].
+hipError_t kernels_encode(const DevCode& c, const uint8_t* msg, uint8_t* cw, int64_t frames, hipStream_t st);
 hipError_t kernels_ingest(const Layout& lay, const DevCode& c, const DevState& s, const void* llr, const void* syst,
                           bool dbl, int64_t frames, int64_t frame0, uint8_t init_phase, hipStream_t st);
 hipError_t kernels_state_in(const Layout& lay, const DevCode& c, const DevState& s, const void* state, bool dbl,

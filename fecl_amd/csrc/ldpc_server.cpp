@@ -2,12 +2,13 @@
 //
 // FeCl has no serving layer: Codec::decode blocks on one batch (src/Codec.h:210-238).  Here
 // each code owns a pool of device frame slots whose message state stays resident across
-// iterations.  A worker thread repeats one step for every pool:
+// iterations.  A worker thread runs rounds, each one step for every pool:
 //   admit   queued frames into free slots (their LLRs are copied into the slots' tile rows)
 //   step    check-node, variable-node (+ outputs of finishing frames), syndrome/control
 //   retire  frames that reached DONE: read their hard bits and iteration counts, free the slots
-// so requests that arrive while others are on the GPU join at the next step boundary, and a
-// frame's arithmetic is exactly that of a batch decode (no cross-frame state).
+// with the device kept one round ahead of the host's bookkeeping.  Requests that arrive while
+// others are on the GPU join at a round boundary, and a frame's arithmetic is exactly that of
+// a batch decode (no cross-frame state).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -75,8 +76,11 @@ struct Pool {
   std::vector<std::pair<Request*, int64_t>> owner;  // slot -> (request, frame)
   int64_t live = 0;
   std::deque<Request*> queue;
+  std::vector<int64_t> admit_round;                 // slot -> round its current frame was admitted in
   DevBuf d_stage, d_admit, d_ext_slots, d_ext_bits, d_ext_iters;
-  PinnedBuf h_stage, h_admit, h_ext_slots, h_ext_bits, h_ext_iters, h_phase;
+  // host buffers, two of each: the worker fills or reads one while the device uses the other
+  PinnedBuf h_stage[2], h_admit[2], h_ext_slots[2], h_ext_bits[2], h_ext_iters[2], h_phase[2];
+  std::vector<std::pair<Request*, int64_t>> ext_owner[2];   // retired entries awaiting their results
   ~Pool() {
     for (DevBuf* b : {&d_stage, &d_admit, &d_ext_slots, &d_ext_bits, &d_ext_iters}) b->release();
   }
@@ -117,73 +121,90 @@ struct fecl_ldpc_server {
 
 namespace {
 
-// One admit / step / retire round over every pool; returns a status (errors stop the server).
-int server_round(fecl_ldpc_server* sv, bool* idle) {
-  struct Admit { Request* req; int64_t f0, n; };
-  std::vector<std::vector<Admit>> admits(sv->pools.size());
-  bool any_live = false;
-  {
-    std::lock_guard<std::mutex> lk(sv->mu);
-    for (size_t pi = 0; pi < sv->pools.size(); ++pi) {
-      Pool& P = *sv->pools[pi];
-      int n = 0;
-      while (!P.free_slots.empty() && !P.queue.empty() && n < P.admit_cap) {
-        Request* r = P.queue.front();
-        const int64_t k = std::min<int64_t>({(int64_t)P.free_slots.size(), r->frames - r->admitted,
-                                             (int64_t)(P.admit_cap - n)});
-        int32_t* slots = P.h_admit.as<int32_t>();
-        for (int64_t i = 0; i < k; ++i) {
-          const int32_t slot = P.free_slots.top();
-          P.free_slots.pop();
-          slots[n + i] = slot;
-          P.owner[slot] = {r, r->admitted + i};
-        }
-        admits[pi].push_back({r, r->admitted, k});
-        r->admitted += k;
-        sv->queued_frames -= k;
-        n += (int)k;
-        if (r->admitted == r->frames) P.queue.pop_front();
-      }
-      P.live += n;
-      any_live |= P.live > 0;
-    }
+// The worker keeps the device one round ahead of the host.  Round e = admissions chosen from
+// the free slots, one step of every pool holding frames, and a copy of every slot's phase.
+// While the device runs round e, the host reads the phases of round e - 1 and retires the
+// frames that were DONE by then: an extract of their bits and iteration counts is queued
+// behind round e, and their slots may take new frames from round e + 1 on (stream order runs
+// the extract first).  The results of that extract are handed out in the next iteration.
+// Host buffers come in pairs indexed by the round's parity, so the host never touches a
+// buffer the device may still be using.
+struct Events {
+  hipEvent_t round[2] = {nullptr, nullptr}, ext[2] = {nullptr, nullptr};
+  ~Events() {
+    for (hipEvent_t e : {round[0], round[1], ext[0], ext[1]})
+      if (e) (void)hipEventDestroy(e);
   }
-  *idle = !any_live;
-  if (!any_live) return FECL_OK;
-  // admit: stage the LLR rows (requests keep their buffers alive until completion)
+};
+
+struct Admit { Request* req; int64_t f0, n; };
+
+// Host side of the admissions of round e: queued frames into free slots (lowest first).
+// Returns whether any pool holds frames (and so takes a step in round e).
+bool choose_admissions(fecl_ldpc_server* sv, int64_t e, std::vector<std::vector<Admit>>* admits) {
+  bool any = false;
+  std::lock_guard<std::mutex> lk(sv->mu);
   for (size_t pi = 0; pi < sv->pools.size(); ++pi) {
     Pool& P = *sv->pools[pi];
+    (*admits)[pi].clear();
     int n = 0;
+    int32_t* slots = P.h_admit[e & 1].as<int32_t>();
+    while (!P.free_slots.empty() && !P.queue.empty() && n < P.admit_cap) {
+      Request* q = P.queue.front();
+      const int64_t k = std::min<int64_t>({(int64_t)P.free_slots.size(), q->frames - q->admitted,
+                                           (int64_t)(P.admit_cap - n)});
+      for (int64_t i = 0; i < k; ++i) {
+        const int32_t slot = P.free_slots.top();
+        P.free_slots.pop();
+        slots[n + i] = slot;
+        P.owner[slot] = {q, q->admitted + i};
+        P.admit_round[slot] = e;
+      }
+      (*admits)[pi].push_back({q, q->admitted, k});
+      q->admitted += k;
+      sv->queued_frames -= k;
+      n += (int)k;
+      if (q->admitted == q->frames) P.queue.pop_front();
+    }
+    P.live += n;
+    any |= P.live > 0;
+  }
+  return any;
+}
+
+// Device side of round e: admission copies and kernels, one step per pool, phase copies.
+int enqueue_round(fecl_ldpc_server* sv, Events* ev, int64_t e, const std::vector<std::vector<Admit>>& admits) {
+  const int b = (int)(e & 1);
+  for (size_t pi = 0; pi < sv->pools.size(); ++pi) {
+    Pool& P = *sv->pools[pi];
     const int64_t N = P.h->code.N;
-    int64_t staged = 0;   // frames in the pinned staging buffer, copied in one transfer
+    int64_t n = 0, staged = 0;
     for (const Admit& a : admits[pi]) {
-      if (a.req->pinned) {
-        HIP_TRY(hipMemcpyAsync(P.d_stage.as<float>() + (int64_t)n * N, a.req->llr + a.f0 * N, (size_t)(a.n * N) * 4,
+      if (a.req->pinned) {   // page-locked request memory: copied straight to the device
+        HIP_TRY(hipMemcpyAsync(P.d_stage.as<float>() + n * N, a.req->llr + a.f0 * N, (size_t)(a.n * N) * 4,
                                hipMemcpyHostToDevice, sv->st));
       } else {
-        std::memcpy(P.h_stage.as<float>() + staged * N, a.req->llr + a.f0 * N, (size_t)(a.n * N) * 4);
+        std::memcpy(P.h_stage[b].as<float>() + staged * N, a.req->llr + a.f0 * N, (size_t)(a.n * N) * 4);
         staged += a.n;
       }
-      n += (int)a.n;
+      n += a.n;
     }
     if (!n) continue;
-    // pageable frames were packed into h_stage in order; place each run at its offset
-    if (staged) {
+    if (staged) {   // pageable frames were packed into the staging buffer in order
       int64_t off = 0, src = 0;
       for (const Admit& a : admits[pi]) {
         if (!a.req->pinned) {
-          HIP_TRY(hipMemcpyAsync(P.d_stage.as<float>() + off * N, P.h_stage.as<float>() + src * N, (size_t)(a.n * N) * 4,
-                                 hipMemcpyHostToDevice, sv->st));
+          HIP_TRY(hipMemcpyAsync(P.d_stage.as<float>() + off * N, P.h_stage[b].as<float>() + src * N,
+                                 (size_t)(a.n * N) * 4, hipMemcpyHostToDevice, sv->st));
           src += a.n;
         }
         off += a.n;
       }
     }
-    HIP_TRY(hipMemcpyAsync(P.d_admit.p, P.h_admit.p, (size_t)n * 4, hipMemcpyHostToDevice, sv->st));
+    HIP_TRY(hipMemcpyAsync(P.d_admit.p, P.h_admit[b].p, (size_t)n * 4, hipMemcpyHostToDevice, sv->st));
     const uint8_t init = (uint8_t)((P.h->iterations == 1 ? PH_FINAL : PH_RUN) | PH_FRESH);
-    HIP_TRY(kernels_admit(P.lay, P.h->dc, P.s, P.d_stage.as<float>(), P.d_admit.as<int32_t>(), n, init, sv->st));
+    HIP_TRY(kernels_admit(P.lay, P.h->dc, P.s, P.d_stage.as<float>(), P.d_admit.as<int32_t>(), (int)n, init, sv->st));
   }
-  // step every pool that holds live frames
   for (auto& pp : sv->pools) {
     Pool& P = *pp;
     if (!P.live) continue;
@@ -191,46 +212,73 @@ int server_round(fecl_ldpc_server* sv, bool* idle) {
     HIP_TRY(kernels_variable(P.lay, P.h->dc, P.s, P.out, P.h->code.max_var_degree, P.opts.fused_syn, sv->st));
     HIP_TRY(kernels_syndrome_control(P.lay, P.h->dc, P.s, P.h->iterations, P.opts.fused_syn, -1,
                                      P.h->code.max_row_degree, sv->st));
-    HIP_TRY(hipMemcpyAsync(P.h_phase.p, P.s.phase, (size_t)P.nslots, hipMemcpyDeviceToHost, sv->st));
   }
-  HIP_TRY(hipStreamSynchronize(sv->st));
-  // retire finished frames
-  std::vector<int> retired(sv->pools.size(), 0);
-  for (size_t pi = 0; pi < sv->pools.size(); ++pi) {
-    Pool& P = *sv->pools[pi];
-    if (!P.live) continue;
-    const uint8_t* ph = P.h_phase.as<uint8_t>();
-    int32_t* list = P.h_ext_slots.as<int32_t>();
-    int n = 0;
-    for (int64_t slot = 0; slot < P.nslots; ++slot)
-      if (P.owner[slot].first && (ph[slot] & PH_MASK) == PH_DONE) list[n++] = (int32_t)slot;
-    retired[pi] = n;
+  for (auto& pp : sv->pools) {   // every pool, so that each round's phases are complete
+    Pool& P = *pp;
+    HIP_TRY(hipMemcpyAsync(P.h_phase[b].p, P.s.phase, (size_t)P.nslots, hipMemcpyDeviceToHost, sv->st));
+  }
+  HIP_TRY(hipEventRecord(ev->round[b], sv->st));
+  std::lock_guard<std::mutex> lk(sv->mu);
+  ++sv->steps;
+  return FECL_OK;
+}
+
+// Round q has completed: retire the frames its phases show DONE and queue their extract.
+int retire_round(fecl_ldpc_server* sv, Events* ev, int64_t q, bool* queued) {
+  const int b = (int)(q & 1);
+  bool any = false;
+  *queued = false;
+  {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    for (auto& pp : sv->pools) {
+      Pool& P = *pp;
+      P.ext_owner[b].clear();
+      const uint8_t* ph = P.h_phase[b].as<uint8_t>();
+      int32_t* list = P.h_ext_slots[b].as<int32_t>();
+      for (int64_t slot = 0; slot < P.nslots; ++slot) {
+        // a frame admitted after round q is not the one these phases describe
+        if (!P.owner[slot].first || P.admit_round[slot] > q || (ph[slot] & PH_MASK) != PH_DONE) continue;
+        list[P.ext_owner[b].size()] = (int32_t)slot;
+        P.ext_owner[b].push_back(P.owner[slot]);
+        P.owner[slot] = {nullptr, 0};
+        P.free_slots.push((int32_t)slot);
+        --P.live;
+      }
+      any |= !P.ext_owner[b].empty();
+    }
+  }
+  if (!any) return FECL_OK;
+  for (auto& pp : sv->pools) {
+    Pool& P = *pp;
+    const int n = (int)P.ext_owner[b].size();
     if (!n) continue;
     const int64_t K = P.h->code.K;
-    HIP_TRY(hipMemcpyAsync(P.d_ext_slots.p, list, (size_t)n * 4, hipMemcpyHostToDevice, sv->st));
+    HIP_TRY(hipMemcpyAsync(P.d_ext_slots.p, P.h_ext_slots[b].p, (size_t)n * 4, hipMemcpyHostToDevice, sv->st));
     HIP_TRY(kernels_extract(P.lay, P.h->dc, P.s, P.out, P.d_ext_slots.as<int32_t>(), n, P.d_ext_bits.as<uint8_t>(),
                             P.d_ext_iters.as<int32_t>(), sv->st));
-    HIP_TRY(hipMemcpyAsync(P.h_ext_bits.p, P.d_ext_bits.p, (size_t)n * K, hipMemcpyDeviceToHost, sv->st));
-    HIP_TRY(hipMemcpyAsync(P.h_ext_iters.p, P.d_ext_iters.p, (size_t)n * 4, hipMemcpyDeviceToHost, sv->st));
+    HIP_TRY(hipMemcpyAsync(P.h_ext_bits[b].p, P.d_ext_bits.p, (size_t)n * K, hipMemcpyDeviceToHost, sv->st));
+    HIP_TRY(hipMemcpyAsync(P.h_ext_iters[b].p, P.d_ext_iters.p, (size_t)n * 4, hipMemcpyDeviceToHost, sv->st));
   }
-  HIP_TRY(hipStreamSynchronize(sv->st));
+  HIP_TRY(hipEventRecord(ev->ext[b], sv->st));
+  *queued = true;
+  return FECL_OK;
+}
+
+// The extract queued for round q has finished: copy its results into the requests.
+void hand_out(fecl_ldpc_server* sv, int64_t q) {
+  const int b = (int)(q & 1);
   const Clock::time_point now = Clock::now();
   bool notify = false;
   {
     std::lock_guard<std::mutex> lk(sv->mu);
-    for (size_t pi = 0; pi < sv->pools.size(); ++pi) {
-      Pool& P = *sv->pools[pi];
+    for (auto& pp : sv->pools) {
+      Pool& P = *pp;
       const int64_t K = P.h->code.K;
-      const int32_t* list = P.h_ext_slots.as<int32_t>();
-      for (int i = 0; i < retired[pi]; ++i) {
-        const int32_t slot = list[i];
-        Request* r = P.owner[slot].first;
-        const int64_t f = P.owner[slot].second;
-        std::memcpy(r->msg + f * K, P.h_ext_bits.as<uint8_t>() + (int64_t)i * K, (size_t)K);
-        if (r->iters) r->iters[f] = P.h_ext_iters.as<int32_t>()[i];
-        P.owner[slot] = {nullptr, 0};
-        P.free_slots.push(slot);
-        --P.live;
+      for (size_t i = 0; i < P.ext_owner[b].size(); ++i) {
+        Request* r = P.ext_owner[b][i].first;
+        const int64_t f = P.ext_owner[b][i].second;
+        std::memcpy(r->msg + f * K, P.h_ext_bits[b].as<uint8_t>() + (int64_t)i * K, (size_t)K);
+        if (r->iters) r->iters[f] = P.h_ext_iters[b].as<int32_t>()[i];
         ++sv->frames_done;
         if (++r->done == r->frames) {
           r->finished = true;
@@ -238,40 +286,71 @@ int server_round(fecl_ldpc_server* sv, bool* idle) {
           notify = true;
         }
       }
+      P.ext_owner[b].clear();
     }
-    ++sv->steps;
   }
   if (notify) sv->done_cv.notify_all();
-  return FECL_OK;
+}
+
+int server_run(fecl_ldpc_server* sv) {
+  Events ev;
+  for (int i = 0; i < 2; ++i) {
+    HIP_TRY(hipEventCreateWithFlags(&ev.round[i], hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&ev.ext[i], hipEventDisableTiming));
+  }
+  std::vector<std::vector<Admit>> admits(sv->pools.size());
+  int64_t enqueued = 0, processed = 0;   // rounds enqueued / rounds whose phases were read
+  int64_t ext_round = -1;                // round whose extract results are still to be handed out
+  for (;;) {
+    {
+      std::lock_guard<std::mutex> lk(sv->mu);
+      if (sv->stop) return FECL_OK;
+    }
+    const bool live = choose_admissions(sv, enqueued, &admits);
+    if (live) {
+      if (int rc = enqueue_round(sv, &ev, enqueued, admits)) return rc;
+      ++enqueued;
+    }
+    // read every completed round but the newest (which keeps the device busy meanwhile)
+    while (processed < enqueued - (live ? 1 : 0)) {
+      HIP_TRY(hipEventSynchronize(ev.round[processed & 1]));
+      if (ext_round >= 0) {
+        HIP_TRY(hipEventSynchronize(ev.ext[ext_round & 1]));
+        hand_out(sv, ext_round);
+        ext_round = -1;
+      }
+      bool queued = false;
+      if (int rc = retire_round(sv, &ev, processed, &queued)) return rc;
+      if (queued) ext_round = processed;
+      ++processed;
+    }
+    if (live) continue;
+    if (ext_round >= 0) {   // nothing to step: finish the last extract, its slots are free now
+      HIP_TRY(hipEventSynchronize(ev.ext[ext_round & 1]));
+      hand_out(sv, ext_round);
+      ext_round = -1;
+      continue;
+    }
+    std::unique_lock<std::mutex> lk(sv->mu);
+    sv->work_cv.wait_for(lk, std::chrono::milliseconds(5), [&] {
+      if (sv->stop) return true;
+      for (auto& p : sv->pools)
+        if (!p->queue.empty()) return true;
+      return false;
+    });
+  }
 }
 
 void server_loop(fecl_ldpc_server* sv) {
   (void)hipSetDevice(sv->device);
-  for (;;) {
-    {
-      std::unique_lock<std::mutex> lk(sv->mu);
-      if (sv->stop) return;
-    }
-    bool idle = false;
-    const int rc = server_round(sv, &idle);
-    if (rc != FECL_OK) {
-      std::lock_guard<std::mutex> lk(sv->mu);
-      sv->error = rc;
-      sv->error_msg = fecl_last_error();
-      sv->stop = true;
-      sv->done_cv.notify_all();
-      return;
-    }
-    if (idle) {
-      std::unique_lock<std::mutex> lk(sv->mu);
-      sv->work_cv.wait_for(lk, std::chrono::milliseconds(5), [&] {
-        if (sv->stop) return true;
-        for (auto& p : sv->pools)
-          if (!p->queue.empty()) return true;
-        return false;
-      });
-    }
+  const int rc = server_run(sv);
+  if (rc != FECL_OK) {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    sv->error = rc;
+    sv->error_msg = fecl_last_error();
+    sv->stop = true;
   }
+  sv->done_cv.notify_all();
 }
 
 // Device bytes one slot needs in a pool of this layout (messages, channel, bookkeeping).
@@ -325,13 +404,16 @@ int fecl_ldpc_server_create(fecl_ldpc* const* handles, int32_t count, const int6
     HIP_TRY(P->d_ext_slots.ensure((size_t)P->nslots * 4));
     HIP_TRY(P->d_ext_bits.ensure((size_t)P->nslots * c.K));
     HIP_TRY(P->d_ext_iters.ensure((size_t)P->nslots * 4));
-    HIP_TRY(P->h_stage.ensure((size_t)P->admit_cap * c.N * 4));
-    HIP_TRY(P->h_admit.ensure((size_t)P->admit_cap * 4));
-    HIP_TRY(P->h_ext_slots.ensure((size_t)P->nslots * 4));
-    HIP_TRY(P->h_ext_bits.ensure((size_t)P->nslots * c.K));
-    HIP_TRY(P->h_ext_iters.ensure((size_t)P->nslots * 4));
-    HIP_TRY(P->h_phase.ensure((size_t)P->nslots));
+    for (int b = 0; b < 2; ++b) {
+      HIP_TRY(P->h_stage[b].ensure((size_t)P->admit_cap * c.N * 4));
+      HIP_TRY(P->h_admit[b].ensure((size_t)P->admit_cap * 4));
+      HIP_TRY(P->h_ext_slots[b].ensure((size_t)P->nslots * 4));
+      HIP_TRY(P->h_ext_bits[b].ensure((size_t)P->nslots * c.K));
+      HIP_TRY(P->h_ext_iters[b].ensure((size_t)P->nslots * 4));
+      HIP_TRY(P->h_phase[b].ensure((size_t)P->nslots));
+    }
     P->owner.assign((size_t)P->nslots, {nullptr, 0});
+    P->admit_round.assign((size_t)P->nslots, 0);
     for (int64_t s = 0; s < P->nslots; ++s) P->free_slots.push((int32_t)s);
     sv->pools.push_back(std::move(P));
   }

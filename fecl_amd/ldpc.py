@@ -266,7 +266,22 @@ class Ldpc:
         return best.value, {k: ms[i] for i, k in enumerate(keys) if ms[i] >= 0}
 
     # ---- encode / check (host)
-    def encode(self, msg):
+    def encode(self, msg, stream=None):
+        """Codec::encode: message bits [F*K] (0/1) -> codeword [F*N].  A CUDA uint8 tensor is
+        encoded on the device (on the current stream) and the codeword returned as one."""
+        if _is_cuda_tensor(msg):
+            import torch
+            x = msg.contiguous().view(-1)
+            if x.dtype != torch.uint8:
+                raise ValueError("message bits must be uint8")
+            frames = x.numel() // self._msg
+            if x.numel() != frames * self._msg:
+                raise ValueError("Invalid size for message")
+            out = torch.empty(frames * self._parity, dtype=torch.uint8, device=x.device)
+            st = stream if stream is not None else torch.cuda.current_stream(x.device).cuda_stream
+            check(lib().fecl_ldpc_encode_device(self._h, ctypes.c_void_p(x.data_ptr()), frames,
+                                                ctypes.c_void_p(out.data_ptr()), ctypes.c_void_p(st)))
+            return out
         msg = np.ascontiguousarray(msg, dtype=np.uint8).ravel()
         frames = msg.size // self._msg
         if msg.size != frames * self._msg:

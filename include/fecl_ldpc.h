@@ -90,6 +90,10 @@ int fecl_ldpc_sizes(const fecl_ldpc* h, int64_t* msg, int64_t* syst, int64_t* pa
  * staircase codes such as DVB-S2 (host computation). */
 int fecl_ldpc_encode(const fecl_ldpc* h, const uint8_t* msg, int64_t frames, uint8_t* parity);
 int fecl_ldpc_check(const fecl_ldpc* h, const uint8_t* parity, int64_t frames, int32_t* ok);
+/* Codec::encode on the device (same results as fecl_ldpc_encode): msg [frames][K] and parity
+ * [frames][N] are device buffers of 0/1 bytes; enqueued on `stream` (a hipStream_t, NULL =
+ * the null stream).  Staircase codes with N <= 65536. */
+int fecl_ldpc_encode_device(fecl_ldpc* h, const uint8_t* msg, int64_t frames, uint8_t* parity, void* stream);
 
 /* Codec::decode -> Ldpc::decodeBlocks (src/Codec.h:210-238, src/Ldpc.cpp:55-59), host buffers:
  * llr [frames][N] double in, msg [frames][K] (0/1 bytes) out.  iterations (nullable)
