@@ -611,7 +611,8 @@ def main():
     if not args.no_forms:
         # the same batches through the other forms of the path (fewer timed decodes each)
         forms = {}
-        for name, a2, f2 in (("linear_f32", F.Linear, F.MsgFormat.F32), ("minsum_f16", F.Approximate, F.MsgFormat.F16),
+        for name, a2, f2 in (("linear_f32", F.Linear, F.MsgFormat.F32), ("exact_f32", F.Exact, F.MsgFormat.F32),
+                             ("minsum_f16", F.Approximate, F.MsgFormat.F16),
                              ("minsum_bf16", F.Approximate, F.MsgFormat.BF16)):
             if a2 == alg and f2 == fmt:
                 continue

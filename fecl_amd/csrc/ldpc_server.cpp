@@ -77,12 +77,13 @@ struct Pool {
   int64_t live = 0;
   std::deque<Request*> queue;
   std::vector<int64_t> admit_round;                 // slot -> round its current frame was admitted in
-  DevBuf d_stage, d_admit, d_ext_slots, d_ext_bits, d_ext_iters;
+  DevBuf d_stage[2], d_admit[2], d_ext_slots, d_ext_bits, d_ext_iters;
   // host buffers, two of each: the worker fills or reads one while the device uses the other
   PinnedBuf h_stage[2], h_admit[2], h_ext_slots[2], h_ext_bits[2], h_ext_iters[2], h_phase[2];
   std::vector<std::pair<Request*, int64_t>> ext_owner[2];   // retired entries awaiting their results
   ~Pool() {
-    for (DevBuf* b : {&d_stage, &d_admit, &d_ext_slots, &d_ext_bits, &d_ext_iters}) b->release();
+    for (DevBuf* b : {&d_stage[0], &d_stage[1], &d_admit[0], &d_admit[1], &d_ext_slots, &d_ext_bits, &d_ext_iters})
+      b->release();
   }
 };
 
@@ -102,7 +103,8 @@ struct fecl_ldpc_server {
   int error = FECL_OK;
   std::string error_msg;
   int64_t frames_done = 0, steps = 0, queued_frames = 0;
-  hipStream_t st = nullptr;
+  hipStream_t st = nullptr;      // admissions' kernels, steps, retirements
+  hipStream_t copy = nullptr;    // admissions' host-to-device copies, overlapping the steps
   int device = 0;
   std::thread worker;
   ~fecl_ldpc_server() {
@@ -114,8 +116,10 @@ struct fecl_ldpc_server {
     if (worker.joinable()) worker.join();
     (void)hipSetDevice(device);
     if (st) (void)hipStreamSynchronize(st);
+    if (copy) (void)hipStreamSynchronize(copy);
     pools.clear();
     if (st) (void)hipStreamDestroy(st);
+    if (copy) (void)hipStreamDestroy(copy);
   }
 };
 
@@ -131,8 +135,10 @@ namespace {
 // buffer the device may still be using.
 struct Events {
   hipEvent_t round[2] = {nullptr, nullptr}, ext[2] = {nullptr, nullptr};
+  hipEvent_t copied[2] = {nullptr, nullptr};    // admissions' copies of round e (copy stream)
+  hipEvent_t admitted[2] = {nullptr, nullptr};  // admission kernels of round e (compute stream)
   ~Events() {
-    for (hipEvent_t e : {round[0], round[1], ext[0], ext[1]})
+    for (hipEvent_t e : {round[0], round[1], ext[0], ext[1], copied[0], copied[1], admitted[0], admitted[1]})
       if (e) (void)hipEventDestroy(e);
   }
 };
@@ -172,38 +178,55 @@ bool choose_admissions(fecl_ldpc_server* sv, int64_t e, std::vector<std::vector<
   return any;
 }
 
-// Device side of round e: admission copies and kernels, one step per pool, phase copies.
+// Device side of round e: admission copies (on the copy stream, so they overlap the steps of
+// round e - 1 still running) and kernels, one step per pool, phase copies.
 int enqueue_round(fecl_ldpc_server* sv, Events* ev, int64_t e, const std::vector<std::vector<Admit>>& admits) {
   const int b = (int)(e & 1);
-  for (size_t pi = 0; pi < sv->pools.size(); ++pi) {
-    Pool& P = *sv->pools[pi];
-    const int64_t N = P.h->code.N;
-    int64_t n = 0, staged = 0;
-    for (const Admit& a : admits[pi]) {
-      if (a.req->pinned) {   // page-locked request memory: copied straight to the device
-        HIP_TRY(hipMemcpyAsync(P.d_stage.as<float>() + n * N, a.req->llr + a.f0 * N, (size_t)(a.n * N) * 4,
-                               hipMemcpyHostToDevice, sv->st));
-      } else {
-        std::memcpy(P.h_stage[b].as<float>() + staged * N, a.req->llr + a.f0 * N, (size_t)(a.n * N) * 4);
-        staged += a.n;
-      }
-      n += a.n;
-    }
-    if (!n) continue;
-    if (staged) {   // pageable frames were packed into the staging buffer in order
-      int64_t off = 0, src = 0;
+  bool any_admit = false;
+  for (const auto& a : admits) any_admit |= !a.empty();
+  if (any_admit) {
+    // staging buffers b were last read by the admission kernels of round e - 2
+    if (e >= 2) HIP_TRY(hipStreamWaitEvent(sv->copy, ev->admitted[b], 0));
+    for (size_t pi = 0; pi < sv->pools.size(); ++pi) {
+      Pool& P = *sv->pools[pi];
+      const int64_t N = P.h->code.N;
+      int64_t n = 0, staged = 0;
       for (const Admit& a : admits[pi]) {
-        if (!a.req->pinned) {
-          HIP_TRY(hipMemcpyAsync(P.d_stage.as<float>() + off * N, P.h_stage[b].as<float>() + src * N,
-                                 (size_t)(a.n * N) * 4, hipMemcpyHostToDevice, sv->st));
-          src += a.n;
+        if (a.req->pinned) {   // page-locked request memory: copied straight to the device
+          HIP_TRY(hipMemcpyAsync(P.d_stage[b].as<float>() + n * N, a.req->llr + a.f0 * N, (size_t)(a.n * N) * 4,
+                                 hipMemcpyHostToDevice, sv->copy));
+        } else {
+          std::memcpy(P.h_stage[b].as<float>() + staged * N, a.req->llr + a.f0 * N, (size_t)(a.n * N) * 4);
+          staged += a.n;
         }
-        off += a.n;
+        n += a.n;
       }
+      if (!n) continue;
+      if (staged) {   // pageable frames were packed into the staging buffer in order
+        int64_t off = 0, src = 0;
+        for (const Admit& a : admits[pi]) {
+          if (!a.req->pinned) {
+            HIP_TRY(hipMemcpyAsync(P.d_stage[b].as<float>() + off * N, P.h_stage[b].as<float>() + src * N,
+                                   (size_t)(a.n * N) * 4, hipMemcpyHostToDevice, sv->copy));
+            src += a.n;
+          }
+          off += a.n;
+        }
+      }
+      HIP_TRY(hipMemcpyAsync(P.d_admit[b].p, P.h_admit[b].p, (size_t)n * 4, hipMemcpyHostToDevice, sv->copy));
     }
-    HIP_TRY(hipMemcpyAsync(P.d_admit.p, P.h_admit[b].p, (size_t)n * 4, hipMemcpyHostToDevice, sv->st));
-    const uint8_t init = (uint8_t)((P.h->iterations == 1 ? PH_FINAL : PH_RUN) | PH_FRESH);
-    HIP_TRY(kernels_admit(P.lay, P.h->dc, P.s, P.d_stage.as<float>(), P.d_admit.as<int32_t>(), (int)n, init, sv->st));
+    HIP_TRY(hipEventRecord(ev->copied[b], sv->copy));
+    HIP_TRY(hipStreamWaitEvent(sv->st, ev->copied[b], 0));
+    for (size_t pi = 0; pi < sv->pools.size(); ++pi) {
+      Pool& P = *sv->pools[pi];
+      int64_t n = 0;
+      for (const Admit& a : admits[pi]) n += a.n;
+      if (!n) continue;
+      const uint8_t init = (uint8_t)((P.h->iterations == 1 ? PH_FINAL : PH_RUN) | PH_FRESH);
+      HIP_TRY(kernels_admit(P.lay, P.h->dc, P.s, P.d_stage[b].as<float>(), P.d_admit[b].as<int32_t>(), (int)n, init,
+                            sv->st));
+    }
+    HIP_TRY(hipEventRecord(ev->admitted[b], sv->st));
   }
   for (auto& pp : sv->pools) {
     Pool& P = *pp;
@@ -297,6 +320,8 @@ int server_run(fecl_ldpc_server* sv) {
   for (int i = 0; i < 2; ++i) {
     HIP_TRY(hipEventCreateWithFlags(&ev.round[i], hipEventDisableTiming));
     HIP_TRY(hipEventCreateWithFlags(&ev.ext[i], hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&ev.copied[i], hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&ev.admitted[i], hipEventDisableTiming));
   }
   std::vector<std::vector<Admit>> admits(sv->pools.size());
   int64_t enqueued = 0, processed = 0;   // rounds enqueued / rounds whose phases were read
@@ -399,8 +424,10 @@ int fecl_ldpc_server_create(fecl_ldpc* const* handles, int32_t count, const int6
     HIP_TRY(hipMemset(P->s.fail, 0, (size_t)tiles * P->lay.fpl() * 8));
     P->admit_cap = (int)std::min<int64_t>(P->nslots, 128);
     const Code& c = h->code;
-    HIP_TRY(P->d_stage.ensure((size_t)P->admit_cap * c.N * 4));
-    HIP_TRY(P->d_admit.ensure((size_t)P->admit_cap * 4));
+    for (int b = 0; b < 2; ++b) {
+      HIP_TRY(P->d_stage[b].ensure((size_t)P->admit_cap * c.N * 4));
+      HIP_TRY(P->d_admit[b].ensure((size_t)P->admit_cap * 4));
+    }
     HIP_TRY(P->d_ext_slots.ensure((size_t)P->nslots * 4));
     HIP_TRY(P->d_ext_bits.ensure((size_t)P->nslots * c.K));
     HIP_TRY(P->d_ext_iters.ensure((size_t)P->nslots * 4));
@@ -419,6 +446,7 @@ int fecl_ldpc_server_create(fecl_ldpc* const* handles, int32_t count, const int6
   }
   HIP_TRY(hipDeviceSynchronize());
   HIP_TRY(hipStreamCreateWithFlags(&sv->st, hipStreamNonBlocking));
+  HIP_TRY(hipStreamCreateWithFlags(&sv->copy, hipStreamNonBlocking));
   fecl_ldpc_server* raw = sv.get();
   sv->worker = std::thread(server_loop, raw);
   *out = sv.release();
