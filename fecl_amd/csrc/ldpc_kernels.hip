@@ -467,26 +467,25 @@ __global__ __launch_bounds__(kBlock) void cn_minsum_raw_kernel(DevCode c, DevSta
 //   Linear: LinearBoxSum, min - f(|a+b|) + f(|a-b|) with FeCl's 8-knot table f.
 //   Exact : the same form with f(z) = log(1 + e^-z), i.e. the exact box-plus
 //           -2 atanh(tanh(-a/2) tanh(-b/2)) in a form that stays finite in FP32.
-__device__ static inline float lin_f(const DevOpts& o, float z) {
-  float x = z * 2.0f;
-  if (x >= 7.0f) return 0.0f;
-  const int i = (int)x;
-  float y = o.lin_y[0], sl = o.lin_s[0];
-#pragma unroll
-  for (int q = 1; q < 7; ++q) { if (i >= q) { y = o.lin_y[q]; sl = o.lin_s[q]; } }
+// The Linear table's knots y[q] and slopes s[q] sit in LDS (filled once per workgroup), so a
+// lookup is one ds_read_b64 instead of a chain of selects.
+__device__ static inline float lin_f(const float2* tab, float z) {
+  const float x = z * 2.0f;
+  const int i = min((int)x, 7);   // x >= 7 gives 0 below; the clamp keeps the index in the table
+  const float2 ys = tab[i];
   const float dx = x - (float)i;
-  const float prod = sl * dx;
-  return prod + y;
+  const float prod = ys.y * dx;
+  return x >= 7.0f ? 0.0f : prod + ys.x;
 }
 __device__ static inline float exact_f(float z) { return __logf(1.0f + __expf(-z)); }
 
 template <int ALG>
-__device__ static inline float box_sum(const DevOpts& o, float a, float b) {
+__device__ static inline float box_sum(const float2* tab, float a, float b) {
   const float fa = fabsf(a), fb = fabsf(b);
   const float m = (fb < fa) ? fb : fa;
   const float s1 = fabsf(a + b), s2 = fabsf(a - b);
   float f1, f2;
-  if (ALG == 1) { f1 = lin_f(o, s1); f2 = lin_f(o, s2); }
+  if (ALG == 1) { f1 = lin_f(tab, s1); f2 = lin_f(tab, s2); }
   else { f1 = exact_f(s1); f2 = exact_f(s2); }
   const bool diff = (__float_as_uint(a) ^ __float_as_uint(b)) >> 31;
   const float t = (diff ? m : -m) - f1;
@@ -499,6 +498,17 @@ __device__ static inline float box_sum(const DevOpts& o, float a, float b) {
 template <class F, int DMAX, int ALG>
 __global__ __launch_bounds__(kBlock) void cn_box_kernel(DevCode c, DevState s, DevOpts o) {
   constexpr int FPL = F::FPL, TW = F::TW, RW = F::RW;
+  __shared__ float2 tab[8];
+  if (ALG == 1) {   // before any wave of the workgroup can leave
+    if (threadIdx.x < 8) {
+      float y = o.lin_y[0], sl = o.lin_s[0];
+#pragma unroll
+      for (int q = 1; q < 8; ++q)
+        if ((int)threadIdx.x == q) { y = o.lin_y[q]; sl = o.lin_s[q]; }
+      tab[threadIdx.x] = make_float2(y, sl);
+    }
+    __syncthreads();
+  }
   const int t = blockIdx.y;
   const int r = uniform(blockIdx.x * kWavesPerBlock + (threadIdx.x >> 6));
   if (r >= c.M) return;
@@ -544,7 +554,7 @@ __global__ __launch_bounds__(kBlock) void cn_box_kernel(DevCode c, DevState s, D
   for (int j = 1; j < DMAX - 1; ++j) {
     if (j < d - 1) {
 #pragma unroll
-      for (int k = 0; k < FPL; ++k) { pre[j][k] = P[k]; P[k] = box_sum<ALG>(o, P[k], x[j][k]); }
+      for (int k = 0; k < FPL; ++k) { pre[j][k] = P[k]; P[k] = box_sum<ALG>(tab, P[k], x[j][k]); }
     }
   }
   auto store = [&](int j, float (&y)[FPL]) {
@@ -579,8 +589,8 @@ __global__ __launch_bounds__(kBlock) void cn_box_kernel(DevCode c, DevState s, D
       float y[FPL];
 #pragma unroll
       for (int k = 0; k < FPL; ++k) {
-        y[k] = box_sum<ALG>(o, pre[j][k], S[k]);
-        S[k] = box_sum<ALG>(o, S[k], x[j][k]);
+        y[k] = box_sum<ALG>(tab, pre[j][k], S[k]);
+        S[k] = box_sum<ALG>(tab, S[k], x[j][k]);
       }
       store(j, y);
     }
