@@ -70,27 +70,31 @@ struct ElemBF16 {
   __device__ static inline float quantize(float x) { return __uint_as_float(f32x2_to_bf16x2(x, 0.0f) << 16); }
 };
 
+// Message rows are read once and written once per kernel: non-temporal loads and stores keep
+// them from displacing the graph indices and frame state in the caches.
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 template <int W>
 __device__ static inline void load_words(const uint32_t* p, uint32_t (&w)[W]) {
   if constexpr (W == 1) {
-    w[0] = p[0];
+    w[0] = __builtin_nontemporal_load(p);
   } else if constexpr (W == 2) {
-    const uint2 v = *reinterpret_cast<const uint2*>(p);
+    const u32x2 v = __builtin_nontemporal_load(reinterpret_cast<const u32x2*>(p));
     w[0] = v.x; w[1] = v.y;
   } else {
     static_assert(W == 4, "W is 1, 2 or 4");
-    const uint4 v = *reinterpret_cast<const uint4*>(p);
+    const u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(p));
     w[0] = v.x; w[1] = v.y; w[2] = v.z; w[3] = v.w;
   }
 }
 template <int W>
 __device__ static inline void store_words(uint32_t* p, const uint32_t (&w)[W]) {
   if constexpr (W == 1) {
-    p[0] = w[0];
+    __builtin_nontemporal_store(w[0], p);
   } else if constexpr (W == 2) {
-    *reinterpret_cast<uint2*>(p) = make_uint2(w[0], w[1]);
+    __builtin_nontemporal_store(u32x2{w[0], w[1]}, reinterpret_cast<u32x2*>(p));
   } else {
-    *reinterpret_cast<uint4*>(p) = make_uint4(w[0], w[1], w[2], w[3]);
+    __builtin_nontemporal_store(u32x4{w[0], w[1], w[2], w[3]}, reinterpret_cast<u32x4*>(p));
   }
 }
 template <int N>

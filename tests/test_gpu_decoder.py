@@ -156,6 +156,24 @@ def test_all_normal_rates_minsum_f16(rate):
     assert_bit_exact(out, ref, 3, oc.K, oc.N, oc.E)
 
 
+@pytest.mark.parametrize("rate,words,sf,fmt,ofmt", [
+    ((9, 10), 4, 1.0, F.MsgFormat.F16, O.MSG_F16),     # rows of 30 over 8 frames a lane: streamed check kernel
+    ((9, 10), 4, 0.75, F.MsgFormat.F16, O.MSG_F16),    # streamed, with a scaling factor
+    ((5, 6), 2, 0.75, F.MsgFormat.BF16, O.MSG_BF16),   # rows of 22 over 4 frames a lane: streamed, scaled
+    ((2, 3), 1, 0.75, F.MsgFormat.BF16, O.MSG_BF16),   # register-staged float form (rows of 10), scaled
+    ((9, 10), 1, 1.0, F.MsgFormat.F32, O.MSG_F32),     # raw FP32 form, rows of 30
+    ((1, 4), 4, 0.875, F.MsgFormat.F32, O.MSG_F32)])   # scaled float form, variable degree 12
+def test_check_kernel_forms_on_high_degree_codes(rate, words, sf, fmt, ofmt):
+    """Each min-sum check-kernel form (raw encodings, float with scaling, streamed rows) on DVB-S2
+    codes whose rows exceed the register-staged bound for the layout."""
+    oc, H = dvbs2(64800, rate)
+    llr = dvbs2_llr(4, 2.5 + 2 * rate[0] / rate[1], seed=21, rate=rate).astype(np.float32).astype(np.float64)
+    ref = oracle_run(oc, llr, F.Approximate, 20, sf=sf, fmt=ofmt)
+    dec = gpu_decoder(H, F.Approximate, 20, sf=sf, fmt=fmt, words=words)
+    assert dec.layout()[0] == words
+    assert_bit_exact(dec.soDecode(llr), ref, 4, oc.K, oc.N, oc.E)
+
+
 @pytest.mark.parametrize("code", ["gallager", "dvbs2"])
 def test_exact_fp32_against_fp64(code):
     """Exact: FeCl evaluates BoxSum as -log((1+p)/(1-p)) with p a product of tanh(-x/2)
