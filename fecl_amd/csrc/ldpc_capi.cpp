@@ -13,6 +13,7 @@
 #include <array>
 #include <cmath>
 #include <cstring>
+#include <future>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -35,8 +36,6 @@ int fail(int status, const std::string& msg) {
 }
 
 
-// Frames decoded per batch: bounded so a workspace stays a few GiB at most.
-constexpr int64_t kMaxChunkFrames = 8192;
 
 // Lane layout (unless one was pinned): the first that the kernels allow for this code and
 // options, in the order measured fastest on the DVB-S2 workloads (FP32: 4, 2, 1 words per
@@ -294,7 +293,11 @@ int check_handle(const fecl_ldpc* h) {
   return FECL_OK;
 }
 
-// Host-buffer decode/soDecode: stage each chunk through device memory.
+// Host-buffer decode/soDecode (FeCl's hooks hand over host FP64 buffers): the frames go through
+// device staging in chunks, with the inputs of chunk k + 1 copied (on a second stream, into the
+// other staging buffer) while chunk k decodes.
+constexpr int64_t kHostChunkFrames = 512;
+
 int host_decode(fecl_ldpc* h, const double* llr, const double* syst_in, const double* state_in, int64_t frames,
                 uint8_t* msg_bits, double* msg, double* syst, double* parity, double* state, int32_t* iterations) {
   if (int rc = check_handle(h)) return rc;
@@ -307,12 +310,21 @@ int host_decode(fecl_ldpc* h, const double* llr, const double* syst_in, const do
   if (int rc = acquire(h, &ws)) return rc;
   struct Back { fecl_ldpc* h; std::unique_ptr<Workspace>* ws; ~Back() { release(h, std::move(*ws)); } } back{h, &ws};
   if (!ws->own_stream) HIP_TRY(hipStreamCreateWithFlags(&ws->own_stream, hipStreamNonBlocking));
-  hipStream_t st = ws->own_stream;
-  const int64_t chunk = std::min<int64_t>(frames, kMaxChunkFrames);
-  // staging layout (device): inputs, then outputs
-  HIP_TRY(ws->stage_in.ensure((size_t)chunk * c.N * 8));
-  if (syst_in) HIP_TRY(ws->stage_syst.ensure((size_t)chunk * c.K * 8));
-  if (state_in) HIP_TRY(ws->stage_state.ensure((size_t)chunk * c.E * 8));
+  if (!ws->copy_stream) {
+    HIP_TRY(hipStreamCreateWithFlags(&ws->copy_stream, hipStreamNonBlocking));
+    for (int b = 0; b < 2; ++b) {
+      HIP_TRY(hipEventCreateWithFlags(&ws->copied[b], hipEventDisableTiming));
+      HIP_TRY(hipEventCreateWithFlags(&ws->used[b], hipEventDisableTiming));
+    }
+  }
+  hipStream_t st = ws->own_stream, cs = ws->copy_stream;
+  const int64_t tw = layout_of(h).tw();
+  const int64_t chunk = std::min<int64_t>(frames, std::max<int64_t>(tw, kHostChunkFrames / tw * tw));
+  for (int b = 0; b < 2; ++b) {
+    HIP_TRY(ws->stage_in[b].ensure((size_t)chunk * c.N * 8));
+    if (syst_in) HIP_TRY(ws->stage_syst[b].ensure((size_t)chunk * c.K * 8));
+    if (state_in) HIP_TRY(ws->stage_state[b].ensure((size_t)chunk * c.E * 8));
+  }
   size_t out_bytes = 0;
   auto take = [&](bool want, size_t n) { size_t off = out_bytes; if (want) out_bytes += (n + 255) / 256 * 256; return off; };
   const size_t o_bits = take(msg_bits != nullptr, (size_t)chunk * c.K);
@@ -323,15 +335,43 @@ int host_decode(fecl_ldpc* h, const double* llr, const double* syst_in, const do
   const size_t o_state = take(state != nullptr, (size_t)chunk * c.E * 8);
   HIP_TRY(ws->stage_out.ensure(std::max<size_t>(out_bytes, 256)));
   char* ob = ws->stage_out.as<char>();
-  for (int64_t f0 = 0; f0 < frames; f0 += chunk) {
-    const int64_t n = std::min(chunk, frames - f0);
-    HIP_TRY(hipMemcpyAsync(ws->stage_in.p, llr + f0 * c.N, (size_t)n * c.N * 8, hipMemcpyHostToDevice, st));
-    if (syst_in) HIP_TRY(hipMemcpyAsync(ws->stage_syst.p, syst_in + f0 * c.K, (size_t)n * c.K * 8, hipMemcpyHostToDevice, st));
-    if (state_in) HIP_TRY(hipMemcpyAsync(ws->stage_state.p, state_in + f0 * c.E, (size_t)n * c.E * 8, hipMemcpyHostToDevice, st));
+  const int64_t nchunks = (frames + chunk - 1) / chunk;
+  // inputs of chunk k into staging buffer k & 1, once the decode of chunk k - 2 has read it
+  auto copy_in = [&](int64_t k) -> int {
+    const int b = (int)(k & 1);
+    const int64_t f0 = k * chunk, n = std::min(chunk, frames - f0);
+    if (k >= 2) HIP_TRY(hipStreamWaitEvent(cs, ws->used[b], 0));
+    HIP_TRY(hipMemcpyAsync(ws->stage_in[b].p, llr + f0 * c.N, (size_t)n * c.N * 8, hipMemcpyHostToDevice, cs));
+    if (syst_in)
+      HIP_TRY(hipMemcpyAsync(ws->stage_syst[b].p, syst_in + f0 * c.K, (size_t)n * c.K * 8, hipMemcpyHostToDevice, cs));
+    if (state_in)
+      HIP_TRY(hipMemcpyAsync(ws->stage_state[b].p, state_in + f0 * c.E, (size_t)n * c.E * 8, hipMemcpyHostToDevice, cs));
+    HIP_TRY(hipEventRecord(ws->copied[b], cs));
+    return FECL_OK;
+  };
+  // A copy from pageable memory holds the calling thread for its duration, and the decode of a
+  // chunk holds it too (it polls the device), so the next chunk's copy runs on a helper thread.
+  // (errors are recorded per thread, so the helper hands its message back with the status)
+  using Copied = std::pair<int, std::string>;
+  auto copy_async = [&](int64_t k) {
+    return std::async(std::launch::async, [&, k]() -> Copied {
+      if (hipSetDevice(h->device) != hipSuccess) return {FECL_DEVICE_ERROR, "hipSetDevice failed"};
+      const int rc = copy_in(k);
+      return {rc, rc ? g_err : std::string()};
+    });
+  };
+  std::future<Copied> next = copy_async(0);
+  for (int64_t k = 0; k < nchunks; ++k) {
+    const int b = (int)(k & 1);
+    const int64_t f0 = k * chunk, n = std::min(chunk, frames - f0);
+    const Copied got = next.get();
+    if (got.first) return fail(got.first, got.second);
+    if (k + 1 < nchunks) next = copy_async(k + 1);   // overlaps this chunk's decode
+    HIP_TRY(hipStreamWaitEvent(st, ws->copied[b], 0));
     BatchIO io{};
-    io.llr = ws->stage_in.p;
-    io.syst = syst_in ? ws->stage_syst.p : nullptr;
-    io.state = state_in ? ws->stage_state.p : nullptr;
+    io.llr = ws->stage_in[b].p;
+    io.syst = syst_in ? ws->stage_syst[b].p : nullptr;
+    io.state = state_in ? ws->stage_state[b].p : nullptr;
     io.dbl_in = true;
     io.dbl_out = true;
     io.res.msg_bits = msg_bits ? (uint8_t*)(ob + o_bits) : nullptr;
@@ -340,7 +380,11 @@ int host_decode(fecl_ldpc* h, const double* llr, const double* syst_in, const do
     io.res.syst = syst ? (void*)(ob + o_syst) : nullptr;
     io.res.parity = parity ? (void*)(ob + o_par) : nullptr;
     io.res.state = state ? (void*)(ob + o_state) : nullptr;
-    if (int rc = run_passes(h, ws.get(), st, io, n, 0, false)) return rc;
+    if (int rc = run_passes(h, ws.get(), st, io, n, 0, false)) {
+      if (next.valid()) next.wait();
+      return rc;
+    }
+    HIP_TRY(hipEventRecord(ws->used[b], st));
     if (msg_bits) HIP_TRY(hipMemcpyAsync(msg_bits + f0 * c.K, ob + o_bits, (size_t)n * c.K, hipMemcpyDeviceToHost, st));
     if (iterations) HIP_TRY(hipMemcpyAsync(iterations + f0, ob + o_iter, (size_t)n * 4, hipMemcpyDeviceToHost, st));
     if (msg) HIP_TRY(hipMemcpyAsync(msg + f0 * c.K, ob + o_msg, (size_t)n * c.K * 8, hipMemcpyDeviceToHost, st));
@@ -349,6 +393,7 @@ int host_decode(fecl_ldpc* h, const double* llr, const double* syst_in, const do
     if (state) HIP_TRY(hipMemcpyAsync(state + f0 * c.E, ob + o_state, (size_t)n * c.E * 8, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
   }
+  HIP_TRY(hipStreamSynchronize(cs));
   return FECL_OK;
 }
 

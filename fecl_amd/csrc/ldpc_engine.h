@@ -49,16 +49,19 @@ struct DevBuf {
 
 struct Workspace {
   DevBuf msg, L, hard, phase, iter, fail, flags, active, out_hard, out_ext, out_app;
-  DevBuf stage_in, stage_syst, stage_state, stage_out;
+  DevBuf stage_in[2], stage_syst[2], stage_state[2], stage_out;   // host path: inputs double-buffered
   int32_t* host_active = nullptr;
   int host_active_cap = 0;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   hipEvent_t done = nullptr;   // recorded after the last use, waited on by the next user's stream
   std::vector<std::array<hipEvent_t, 4>> pev;   // per-step timing events (profiling only)
   hipStream_t own_stream = nullptr;
+  hipStream_t copy_stream = nullptr;                        // host path: input copies
+  hipEvent_t copied[2] = {nullptr, nullptr}, used[2] = {nullptr, nullptr};
   ~Workspace() {
     for (DevBuf* b : {&msg, &L, &hard, &phase, &iter, &fail, &flags, &active, &out_hard, &out_ext, &out_app,
-                      &stage_in, &stage_syst, &stage_state, &stage_out})
+                      &stage_in[0], &stage_in[1], &stage_syst[0], &stage_syst[1], &stage_state[0], &stage_state[1],
+                      &stage_out})
       b->release();
     if (host_active) (void)hipHostFree(host_active);
     for (auto& e : ev)
@@ -67,6 +70,9 @@ struct Workspace {
     for (auto& a : pev)
       for (auto& e : a) (void)hipEventDestroy(e);
     if (own_stream) (void)hipStreamDestroy(own_stream);
+    if (copy_stream) (void)hipStreamDestroy(copy_stream);
+    for (hipEvent_t e : {copied[0], copied[1], used[0], used[1]})
+      if (e) (void)hipEventDestroy(e);
   }
 };
 

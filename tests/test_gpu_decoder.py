@@ -218,6 +218,27 @@ def test_sodecode_state_and_syst_inputs(words):
     assert_bit_exact(out, ref, 2, oc.K, oc.N, oc.E)
 
 
+def test_host_buffer_path_in_several_chunks():
+    """FeCl's hooks pass host FP64 buffers: 1300 frames go through device staging in chunks of
+    512 whose input copies overlap the previous chunk's decode; with a-priori state and
+    systematic inputs, every output equals the oracle's."""
+    oc, H = gallager()
+    frames = 1300
+    llr = np.concatenate([gallager_llr(oc.N, oc.K, frames // 2, 2.0, 31),
+                          gallager_llr(oc.N, oc.K, frames - frames // 2, 0.5, 32)]).astype(np.float32).astype(np.float64)
+    first = oracle_run(oc, llr, F.Approximate, 4, threads=16)
+    state = first["state"].astype(np.float32).astype(np.float64)
+    syst = np.random.default_rng(98).normal(0, 1, (frames, oc.K)).astype(np.float32).astype(np.float64)
+    ref = oracle_run(oc, llr, F.Approximate, 12, state_in=state, syst_in=syst, threads=16)
+    dec = gpu_decoder(H, F.Approximate, 12, words=2)
+    assert dec.layout()[1] < 512
+    out = dec.soDecode(llr, syst=syst, state=state)
+    assert_bit_exact(out, ref, frames, oc.K, oc.N, oc.E)
+    hard, it = dec.decode(llr, iterations=True)
+    ref2 = oracle_run(oc, llr, F.Approximate, 12, want=("hard", "iters"), threads=16)
+    assert np.array_equal(hard.reshape(frames, oc.K), ref2["hard"]) and np.array_equal(it, ref2["iters"])
+
+
 def test_autotune_picks_an_eligible_layout():
     oc, H = dvbs2()
     dec = gpu_decoder(H, F.Approximate, 50)
