@@ -328,3 +328,27 @@ def test_device_encoder_matches_host_for_all_dvbs2_codes():
             torch.cuda.synchronize()
             assert np.array_equal(dev.cpu().numpy(), host), (n, rate)
             assert dec.check(host), (n, rate)
+
+
+def test_concurrent_calls_on_one_handle():
+    """FeCl's Codec::decode calls decodeBlocks from several worker threads at once on disjoint
+    frames (src/Codec.h:224-237): concurrent host-path calls on one handle each get their own
+    workspace and the results equal the single-threaded ones."""
+    import threading
+    oc, H = dvbs2()
+    dec = gpu_decoder(H, F.Approximate, 50)
+    parts = [dvbs2_llr(24, 1.2 + 0.2 * i, seed=200 + i).astype(np.float32).astype(np.float64) for i in range(4)]
+    alone = [dec.decode(x, iterations=True) for x in parts]
+    got = [None] * len(parts)
+
+    def work(i):
+        for _ in range(3):
+            got[i] = dec.decode(parts[i], iterations=True)
+
+    threads = [threading.Thread(target=work, args=(i,)) for i in range(len(parts))]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    for (h1, i1), (h2, i2) in zip(alone, got):
+        assert np.array_equal(h1, h2) and np.array_equal(i1, i2)
