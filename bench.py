@@ -31,9 +31,9 @@ METRIC = "decoded info Gb/s at FER parity vs FP64 (DVB-S2 r1/2); p50/p99 latency
 # HBM bytes moved per algorithmic byte by the dominant kernels, from rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE runs of this workload (all frames live), each counter scaled by what
 # it reads for a plain 1 GiB copy (FETCH x2.000, WRITE x1.000).  Keyed by (msg format, lane words).
-PMC_TRAFFIC_RATIO = {("f32", 4): {"vn_kernel": 1.0031, "cn_kernel": 1.0115},
-                     ("f32", 2): {"vn_kernel": 1.0109, "cn_kernel": 1.0141},
-                     ("f16", 2): {"vn_kernel": 1.0132, "cn_kernel": 1.0227}}
+PMC_TRAFFIC_RATIO = {("f32", 4): {"vn_kernel": 1.0030, "cn_kernel": 1.0096},
+                     ("f32", 2): {"vn_kernel": 1.0109, "cn_kernel": 1.0093},
+                     ("f16", 2): {"vn_kernel": 1.0132, "cn_kernel": 1.0192}}
 
 # Extra Eb/N0 points (dB) of the precision study, across the waterfall of min-sum on DVB-S2 r1/2.
 PRECISION_WATERFALL = (1.45, 1.5, 1.55, 1.6, 1.65, 1.7)

@@ -22,6 +22,8 @@ SIGNATURES = {
     "fecl_ldpc_sizes": [P, P, P, P, P],
     "fecl_ldpc_encode": [P, P, I64, P],
     "fecl_ldpc_encode_device": [P, P, I64, P, P],
+    "fecl_ldpc_puncturing": [I64, I64, P, I64, P, I64, P, I64, P],
+    "fecl_permute_device": [P, I64, I64, P, P, I64, I32, P],
     "fecl_ldpc_check": [P, P, I64, P],
     "fecl_ldpc_decode": [P, P, I64, P, P],
     "fecl_ldpc_sodecode": [P, P, P, P, I64, P, P, P, P, P],
@@ -33,6 +35,7 @@ SIGNATURES = {
     "fecl_ldpc_get_layout": [P, P, P],
     "fecl_ldpc_set_fused_syndrome": [P, I32],
     "fecl_ldpc_set_pass_frames": [P, I64],
+    "fecl_ldpc_set_cache_policy": [P, I32],
     "fecl_ldpc_set_vars_per_wave": [P, I32],
     "fecl_ldpc_get_vars_per_wave": [P, P],
     "fecl_ldpc_autotune": [P, I64, I32, P, P],

@@ -224,7 +224,8 @@ int run_batch(fecl_ldpc* h, Workspace* ws, hipStream_t st, const BatchIO& io, in
     HIP_TRY(hipHostMalloc((void**)&ws->host_active, (size_t)h->iterations * 4, hipHostMallocDefault));
     ws->host_active_cap = h->iterations;
   }
-  const DevOpts o = make_opts(h);
+  DevOpts o = make_opts(h);
+  o.nt_rows = nt_rows_for(h, lay, s.tiles);
   const bool prof = h->profiling && !capture;
   if (prof) {
     while ((int)ws->pev.size() < h->iterations) {
@@ -246,7 +247,7 @@ int run_batch(fecl_ldpc* h, Workspace* ws, hipStream_t st, const BatchIO& io, in
     if (prof) HIP_TRY(hipEventRecord(ws->pev[step][0], st));
     HIP_TRY(kernels_check(lay, h->dc, s, o, c.max_row_degree, st));
     if (prof) HIP_TRY(hipEventRecord(ws->pev[step][1], st));
-    HIP_TRY(kernels_variable(lay, h->dc, s, out, c.max_var_degree, o.fused_syn, st));
+    HIP_TRY(kernels_variable(lay, h->dc, s, out, c.max_var_degree, o.fused_syn, (o.nt_rows & 2) != 0, st));
     if (prof) HIP_TRY(hipEventRecord(ws->pev[step][2], st));
     HIP_TRY(kernels_syndrome_control(lay, h->dc, s, h->iterations, o.fused_syn, step, c.max_row_degree, st));
     if (prof) HIP_TRY(hipEventRecord(ws->pev[step][3], st));
@@ -286,6 +287,12 @@ int run_passes(fecl_ldpc* h, Workspace* ws, hipStream_t st, const BatchIO& io, i
   for (int64_t f = 0; f < frames; f += pass)
     if (int rc = run_batch(h, ws, st, io, std::min(pass, frames - f), frame0 + f, capture)) return rc;
   return FECL_OK;
+}
+
+int32_t nt_rows_for(const fecl_ldpc* h, const Layout& lay, int64_t tiles) {
+  if (h->cache_policy >= 0) return h->cache_policy;
+  const int64_t bytes = tiles * ((int64_t)h->code.E * lay.rw() * 4 + (int64_t)h->code.N * lay.tw() * 4);
+  return bytes >= kNtBatchBytes ? 3 : 0;
 }
 
 int check_handle(const fecl_ldpc* h) {
@@ -565,6 +572,44 @@ int fecl_ldpc_encode_device(fecl_ldpc* h, const uint8_t* msg, int64_t frames, ui
   return FECL_OK;
 }
 
+int fecl_ldpc_puncturing(int64_t syst_size, int64_t parity_size, const uint8_t* mask, int64_t mask_len,
+                         const uint8_t* syst_mask, int64_t syst_mask_len, int64_t* sequence, int64_t capacity,
+                         int64_t* count) {
+  if (!count || syst_size < 0 || parity_size < syst_size || mask_len < 0 || syst_mask_len < 0 ||
+      (mask_len > 0 && !mask) || (syst_mask_len > 0 && !syst_mask))
+    return fail(FECL_INVALID_ARGUMENT, "invalid arguments");
+  // FeCl's rule, index for index: systematic positions advance the index only when kept (and
+  // the systematic mask is read at that index); parity positions always advance it.
+  std::vector<int64_t> seq;
+  int64_t idx = 0;
+  for (int64_t i = 0; i < syst_size; ++i) {
+    const bool keep = syst_mask_len == 0 ? (mask_len == 0 || mask[i % mask_len] != 0)
+                                         : syst_mask[idx % syst_mask_len] != 0;
+    if (keep) seq.push_back(idx++);
+  }
+  for (int64_t i = 0; i < parity_size - syst_size; ++i) {
+    const bool keep = syst_mask_len == 0 ? (mask_len == 0 || mask[i % mask_len] != 0)
+                                         : (mask_len == 0 || mask[idx % mask_len] != 0);
+    if (keep) seq.push_back(idx);
+    ++idx;
+  }
+  *count = (int64_t)seq.size();
+  if (sequence) {
+    if (capacity < (int64_t)seq.size()) return fail(FECL_INVALID_ARGUMENT, "sequence buffer too small");
+    std::copy(seq.begin(), seq.end(), sequence);
+  }
+  return FECL_OK;
+}
+
+int fecl_permute_device(const int64_t* sequence, int64_t output_size, int64_t input_size, const float* in,
+                        float* out, int64_t frames, int32_t inverse, void* stream) {
+  if (frames < 0 || output_size < 0 || input_size < 0 || (frames > 0 && output_size > 0 && (!sequence || !in || !out)))
+    return fail(FECL_INVALID_ARGUMENT, "invalid arguments");
+  if (frames == 0) return FECL_OK;
+  HIP_TRY(kernels_permute(sequence, output_size, input_size, in, out, frames, inverse != 0, (hipStream_t)stream));
+  return FECL_OK;
+}
+
 int fecl_ldpc_check(const fecl_ldpc* h, const uint8_t* parity, int64_t frames, int32_t* ok) {
   if (int rc = check_handle(h)) return rc;
   if (!ok || frames < 0 || (frames > 0 && !parity)) return fail(FECL_INVALID_ARGUMENT, "Invalid size for parity");
@@ -667,6 +712,14 @@ int fecl_ldpc_set_vars_per_wave(fecl_ldpc* h, int32_t vars) {
 int fecl_ldpc_get_vars_per_wave(const fecl_ldpc* h, int32_t* vars) {
   if (int rc = check_handle(h)) return rc;
   if (vars) *vars = h->vpw;
+  return FECL_OK;
+}
+
+int fecl_ldpc_set_cache_policy(fecl_ldpc* h, int32_t policy) {
+  if (int rc = check_handle(h)) return rc;
+  if (policy < -1 || policy > 3) return fail(FECL_INVALID_ARGUMENT, "cache policy must be -1 (automatic) or 0..3");
+  std::lock_guard<std::mutex> lk(h->mu);
+  h->cache_policy = policy;
   return FECL_OK;
 }
 

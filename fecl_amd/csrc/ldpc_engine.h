@@ -101,6 +101,7 @@ struct fecl_ldpc {
   // tile's fail word.
   bool fused_syndrome = false;
   int64_t pass_frames = 0;  // frames per pass through the iteration loop; 0 = kMaxPassFrames
+  int32_t cache_policy = -1; // message-row cache hints: -1 by working set, else DevOpts::nt_rows bits
   std::mutex mu;
   std::vector<std::unique_ptr<fecl::engine::Workspace>> pool;
   // kernel timing (fecl_ldpc_profile): check, variable, syndrome+control
@@ -127,6 +128,13 @@ struct BatchIO {
 };
 
 int check_handle(const fecl_ldpc* h);
+// DevOpts::nt_rows for a batch of `tiles` tiles of h's layout: the handle's policy, or with
+// the automatic policy non-temporal rows in both kernels once the batch's rows and channel
+// values exceed kNtBatchBytes (2.5x the 256 MB Infinity Cache, so little of them stays cached
+// from one kernel to the next anyway).  Measured on DVB-S2 r1/2 and r3/4: from ~0.7 GB up
+// the hints gain 2-5 %; at 0.3-0.6 GB they lose up to 7 % in the check kernel.
+constexpr int64_t kNtBatchBytes = int64_t(640) << 20;
+int32_t nt_rows_for(const fecl_ldpc* h, const Layout& lay, int64_t tiles);
 Layout layout_of(const fecl_ldpc* h);
 DevOpts make_opts(const fecl_ldpc* h);
 // Sizes ws for `frames` frames of h's layout and returns the device views.

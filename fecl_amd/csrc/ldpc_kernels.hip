@@ -70,31 +70,40 @@ struct ElemBF16 {
   __device__ static inline float quantize(float x) { return __uint_as_float(f32x2_to_bf16x2(x, 0.0f) << 16); }
 };
 
-// Message rows are read once and written once per kernel: non-temporal loads and stores keep
-// them from displacing the graph indices and frame state in the caches.
+// Message rows are read once and written once per kernel.  NT = true marks those accesses
+// non-temporal, which pays off when a batch's rows far exceed the caches; smaller batches keep
+// the default policy, under which the rows one kernel writes are partly still in the 256 MB
+// Infinity Cache when the next kernel reads them.
 typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
-template <int W>
+template <int W, bool NT = false>
 __device__ static inline void load_words(const uint32_t* p, uint32_t (&w)[W]) {
   if constexpr (W == 1) {
-    w[0] = __builtin_nontemporal_load(p);
+    w[0] = NT ? __builtin_nontemporal_load(p) : p[0];
   } else if constexpr (W == 2) {
-    const u32x2 v = __builtin_nontemporal_load(reinterpret_cast<const u32x2*>(p));
+    const u32x2* q = reinterpret_cast<const u32x2*>(p);
+    const u32x2 v = NT ? __builtin_nontemporal_load(q) : *q;
     w[0] = v.x; w[1] = v.y;
   } else {
     static_assert(W == 4, "W is 1, 2 or 4");
-    const u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(p));
+    const u32x4* q = reinterpret_cast<const u32x4*>(p);
+    const u32x4 v = NT ? __builtin_nontemporal_load(q) : *q;
     w[0] = v.x; w[1] = v.y; w[2] = v.z; w[3] = v.w;
   }
 }
-template <int W>
+template <int W, bool NT = false>
 __device__ static inline void store_words(uint32_t* p, const uint32_t (&w)[W]) {
   if constexpr (W == 1) {
-    __builtin_nontemporal_store(w[0], p);
+    if constexpr (NT) __builtin_nontemporal_store(w[0], p);
+    else p[0] = w[0];
   } else if constexpr (W == 2) {
-    __builtin_nontemporal_store(u32x2{w[0], w[1]}, reinterpret_cast<u32x2*>(p));
+    const u32x2 v = {w[0], w[1]};
+    if constexpr (NT) __builtin_nontemporal_store(v, reinterpret_cast<u32x2*>(p));
+    else *reinterpret_cast<u32x2*>(p) = v;
   } else {
-    __builtin_nontemporal_store(u32x4{w[0], w[1], w[2], w[3]}, reinterpret_cast<u32x4*>(p));
+    const u32x4 v = {w[0], w[1], w[2], w[3]};
+    if constexpr (NT) __builtin_nontemporal_store(v, reinterpret_cast<u32x4*>(p));
+    else *reinterpret_cast<u32x4*>(p) = v;
   }
 }
 template <int N>
@@ -117,17 +126,19 @@ template <class E, int W_>
 struct Fmt {
   using Elem = E;
   static constexpr int W = W_, PW = E::PW, FPL = W_ * E::PW, TW = kWave * FPL, RW = kWave * W_;
+  template <bool NT = false>
   __device__ static inline void load(const uint32_t* p, float (&x)[FPL]) {
     uint32_t w[W];
-    load_words<W>(p, w);
+    load_words<W, NT>(p, w);
 #pragma unroll
     for (int i = 0; i < W; ++i) E::unpack(w[i], &x[i * PW]);
   }
+  template <bool NT = false>
   __device__ static inline void store(uint32_t* p, const float (&x)[FPL]) {
     uint32_t w[W];
 #pragma unroll
     for (int i = 0; i < W; ++i) w[i] = E::pack(&x[i * PW]);
-    store_words<W>(p, w);
+    store_words<W, NT>(p, w);
   }
   __device__ static inline float quantize(float x) { return E::quantize(x); }
 };
@@ -388,7 +399,7 @@ struct RawMinSum<Fmt<ElemF16, W>> : RawMinSum16<ElemF16, W> {};
 template <int W>
 struct RawMinSum<Fmt<ElemBF16, W>> : RawMinSum16<ElemBF16, W> {};
 
-template <class F, int DMAX>
+template <class F, int DMAX, bool NT>
 __global__ __launch_bounds__(kBlock) void cn_minsum_raw_kernel(DevCode c, DevState s, DevOpts o) {
   static_assert(DMAX > 0, "register-staged rows");
   using R = RawMinSum<F>;
@@ -412,7 +423,7 @@ __global__ __launch_bounds__(kBlock) void cn_minsum_raw_kernel(DevCode c, DevSta
   uint32_t w[DMAX][W];
 #pragma unroll
   for (int j = 0; j < DMAX; ++j)
-    if (j < d) load_words<W>(mrow + (int64_t)j * RW, w[j]);
+    if (j < d) load_words<W, NT>(mrow + (int64_t)j * RW, w[j]);
   CnLanes<FPL> ln;
   if (!cn_decide<FPL>(o, ph, it, ln)) return;
   if (ln.wave_fresh) {   // frames whose v2c is still L[col(e)], rounded to the storage format
@@ -460,7 +471,7 @@ __global__ __launch_bounds__(kBlock) void cn_minsum_raw_kernel(DevCode c, DevSta
         const uint32_t out = R::pick(R::mag(w[j][i]), m1[i], m2[i]) | ((sx[i] ^ flip ^ w[j][i]) & R::kSign);
         y[i] = (out & am[i]) | (w[j][i] & ~am[i]);   // frames without an update keep their word
       }
-      store_words<W>(mrow + (int64_t)j * RW, y);
+      store_words<W, NT>(mrow + (int64_t)j * RW, y);
     }
   }
 }
@@ -619,7 +630,7 @@ __global__ __launch_bounds__(kBlock) void cn_box_kernel(DevCode c, DevState s, D
 //   soDecode : ext = 0 + c_1 + ... + c_dv, msg = L + ext
 // and leaves their messages (the final state) untouched.  init_pass builds v2c from a
 // supplied state (BpDecoderImpl.cpp:81-88) for frames flagged PH_VNINIT.
-template <class F, int VMAX, int VPW>
+template <class F, int VMAX, int VPW, bool NT>
 __global__ __launch_bounds__(kBlock) void vn_kernel(DevCode c, DevState s, DevOut out, int fused_syn, int init_pass) {
   constexpr int FPL = F::FPL, TW = F::TW, RW = F::RW, W = F::W;
   // Tiles in the opposite order to the check kernel: workgroups are dispatched in grid order,
@@ -663,7 +674,7 @@ __global__ __launch_bounds__(kBlock) void vn_kernel(DevCode c, DevState s, DevOu
   for (int i = 0; i < VPW; ++i)
 #pragma unroll
     for (int j = 0; j < VMAX; ++j)
-      if (j < dv[i]) load_words<W>(mt + (int64_t)eid[i][j] * RW, wv[i][j]);
+      if (j < dv[i]) load_words<W, NT>(mt + (int64_t)eid[i][j] * RW, wv[i][j]);
   bool run[FPL], fin[FPL];
   bool lane_run = false, lane_fin = false;
 #pragma unroll
@@ -749,7 +760,7 @@ __global__ __launch_bounds__(kBlock) void vn_kernel(DevCode c, DevState s, DevOu
           acc[k] = acc[k] + cv[j][k];
           if (!run[k]) y[k] = cv[j][k];   // other frames keep their word (exact round trip)
         }
-        if (lane_run) F::store(mt + (int64_t)eid[i][j] * RW, y);
+        if (lane_run) F::template store<NT>(mt + (int64_t)eid[i][j] * RW, y);
       }
     }
     if (init_pass) continue;
@@ -1089,6 +1100,38 @@ __global__ void fill_i32_kernel(int32_t* p, int64_t n, int32_t v) {
 }
 
 // ------------------------------------------------------------------------------------------
+// Permutation::permute / dePermute of FP32 blocks (src/Permutation.h): a grid-stride loop over
+// (frame, output index); dePermute zeroes its output first.
+__global__ __launch_bounds__(kBlock) void permute_kernel(const int64_t* seq, int64_t n_seq, int64_t n_in,
+                                                        const float* in, float* out, int64_t frames, int inverse) {
+  const int64_t total = frames * n_seq;
+  for (int64_t x = (int64_t)blockIdx.x * kBlock + threadIdx.x; x < total; x += (int64_t)gridDim.x * kBlock) {
+    const int64_t f = x / n_seq, i = x % n_seq;
+    if (inverse) out[f * n_in + seq[i]] = in[f * n_seq + i];
+    else out[f * n_seq + i] = in[f * n_in + seq[i]];
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void zero_kernel(float* p, int64_t n) {
+  for (int64_t x = (int64_t)blockIdx.x * kBlock + threadIdx.x; x < n; x += (int64_t)gridDim.x * kBlock) p[x] = 0.0f;
+}
+
+hipError_t kernels_permute(const int64_t* seq, int64_t out_size, int64_t in_size, const float* in, float* out,
+                           int64_t frames, bool inverse, hipStream_t st) {
+  // inverse: in is [frames][out_size] (the permuted blocks), out is [frames][in_size]
+  if (inverse) {
+    const int64_t n = frames * in_size;
+    hipLaunchKernelGGL(zero_kernel, dim3((unsigned)std::min<int64_t>((n + kBlock - 1) / kBlock, 65536)), dim3(kBlock), 0,
+                       st, out, n);
+  }
+  const int64_t total = frames * out_size;
+  if (total == 0) return hipGetLastError();
+  hipLaunchKernelGGL(permute_kernel, dim3((unsigned)std::min<int64_t>((total + kBlock - 1) / kBlock, 65536)),
+                     dim3(kBlock), 0, st, seq, out_size, in_size, in, out, frames, inverse ? 1 : 0);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------------------
 // Systematic encoder for staircase codes (H = [A | dual diagonal], DVB-S2): codeword =
 // [msg | p] with p_r = p_{r-1} xor (xor of the message bits in row r), the accumulator form
 // of FeCl's generator transform for these codes (src/detail/Ldpc.cpp:180-209).  One workgroup
@@ -1147,7 +1190,7 @@ hipError_t kernels_encode(const DevCode& c, const uint8_t* msg, uint8_t* cw, int
 // values fit the register budget (degree bound x frames per lane <= 64); the host picks a
 // layout that satisfies it (layout_fits()).
 
-template <class F, int VPW>
+template <class F, int VPW, bool NT>
 static hipError_t launch_vn_w(const DevCode& c, const DevState& s, const DevOut& o, int max_dv, int fused, int init,
                               hipStream_t st) {
   constexpr int FPL = F::FPL, W = F::W;
@@ -1157,19 +1200,19 @@ static hipError_t launch_vn_w(const DevCode& c, const DevState& s, const DevOut&
   // several variables per wave only where their packed rows fit (VPW * VMAX * W <= 32 words)
   if (max_dv <= 4) {
     if constexpr (VPW == 1 || VPW * 4 * W <= 32)
-      hipLaunchKernelGGL((vn_kernel<F, 4, VPW>), grid, dim3(kBlock), 0, st, c, s, o, fused, init);
+      hipLaunchKernelGGL((vn_kernel<F, 4, VPW, NT>), grid, dim3(kBlock), 0, st, c, s, o, fused, init);
     else return hipErrorInvalidValue;
   } else if (max_dv <= 8) {
     if constexpr (VPW == 1 || VPW * 8 * W <= 32)
-      hipLaunchKernelGGL((vn_kernel<F, 8, VPW>), grid, dim3(kBlock), 0, st, c, s, o, fused, init);
+      hipLaunchKernelGGL((vn_kernel<F, 8, VPW, NT>), grid, dim3(kBlock), 0, st, c, s, o, fused, init);
     else return hipErrorInvalidValue;
   } else if (max_dv <= 16) {
     if constexpr (16 * FPL <= 64 && (VPW == 1 || VPW * 16 * W <= 32))
-      hipLaunchKernelGGL((vn_kernel<F, 16, VPW>), grid, dim3(kBlock), 0, st, c, s, o, fused, init);
+      hipLaunchKernelGGL((vn_kernel<F, 16, VPW, NT>), grid, dim3(kBlock), 0, st, c, s, o, fused, init);
     else return hipErrorInvalidValue;
   } else if (max_dv <= 32) {
     if constexpr (32 * FPL <= 64 && VPW == 1)
-      hipLaunchKernelGGL((vn_kernel<F, 32, VPW>), grid, dim3(kBlock), 0, st, c, s, o, fused, init);
+      hipLaunchKernelGGL((vn_kernel<F, 32, VPW, NT>), grid, dim3(kBlock), 0, st, c, s, o, fused, init);
     else return hipErrorInvalidValue;
   } else {
     return hipErrorInvalidValue;
@@ -1179,9 +1222,11 @@ static hipError_t launch_vn_w(const DevCode& c, const DevState& s, const DevOut&
 
 template <class F>
 static hipError_t launch_vn(const DevCode& c, const DevState& s, const DevOut& o, int max_dv, int fused, int init,
-                            int vpw, hipStream_t st) {
-  if (vpw == 2) return launch_vn_w<F, 2>(c, s, o, max_dv, fused, init, st);
-  if (vpw == 1) return launch_vn_w<F, 1>(c, s, o, max_dv, fused, init, st);
+                            int vpw, bool nt, hipStream_t st) {
+  if (vpw == 2) return nt ? launch_vn_w<F, 2, true>(c, s, o, max_dv, fused, init, st)
+                          : launch_vn_w<F, 2, false>(c, s, o, max_dv, fused, init, st);
+  if (vpw == 1) return nt ? launch_vn_w<F, 1, true>(c, s, o, max_dv, fused, init, st)
+                          : launch_vn_w<F, 1, false>(c, s, o, max_dv, fused, init, st);
   return hipErrorInvalidValue;
 }
 
@@ -1204,6 +1249,28 @@ static hipError_t launch_cn_box(const DevCode& c, const DevState& s, const DevOp
   return hipGetLastError();
 }
 
+// Raw-encoding min-sum where its registers fit; false if no form fits this layout.
+template <class F, bool NT>
+static bool launch_cn_raw(const DevCode& c, const DevState& s, const DevOpts& o, int max_dc, hipStream_t st) {
+  constexpr int W = F::W;
+  dim3 grid((c.M + kWavesPerBlock - 1) / kWavesPerBlock, s.tiles);
+  if (max_dc <= 8) {
+    hipLaunchKernelGGL((cn_minsum_raw_kernel<F, 8, NT>), grid, dim3(kBlock), 0, st, c, s, o);
+    return true;
+  } else if (max_dc <= 16) {
+    if constexpr (16 * W <= 64) {
+      hipLaunchKernelGGL((cn_minsum_raw_kernel<F, 16, NT>), grid, dim3(kBlock), 0, st, c, s, o);
+      return true;
+    }
+  } else if (max_dc <= 32) {
+    if constexpr (32 * W <= 64) {
+      hipLaunchKernelGGL((cn_minsum_raw_kernel<F, 32, NT>), grid, dim3(kBlock), 0, st, c, s, o);
+      return true;
+    }
+  }
+  return false;
+}
+
 template <class F>
 static hipError_t launch_cn(const DevCode& c, const DevState& s, const DevOpts& o, int max_dc, hipStream_t st) {
   constexpr int FPL = F::FPL;
@@ -1211,21 +1278,7 @@ static hipError_t launch_cn(const DevCode& c, const DevState& s, const DevOpts& 
   if (o.algorithm == 2) {
     bool done = false;
     if (o.sf_mode == 0) {   // raw encodings (no scaling factor to apply)
-      constexpr int W = F::W;
-      if (max_dc <= 8) {
-        hipLaunchKernelGGL((cn_minsum_raw_kernel<F, 8>), grid, dim3(kBlock), 0, st, c, s, o);
-        done = true;
-      } else if (max_dc <= 16) {
-        if constexpr (16 * W <= 64) {
-          hipLaunchKernelGGL((cn_minsum_raw_kernel<F, 16>), grid, dim3(kBlock), 0, st, c, s, o);
-          done = true;
-        }
-      } else if (max_dc <= 32) {
-        if constexpr (32 * W <= 64) {
-          hipLaunchKernelGGL((cn_minsum_raw_kernel<F, 32>), grid, dim3(kBlock), 0, st, c, s, o);
-          done = true;
-        }
-      }
+      done = (o.nt_rows & 1) ? launch_cn_raw<F, true>(c, s, o, max_dc, st) : launch_cn_raw<F, false>(c, s, o, max_dc, st);
       if (done) return hipGetLastError();
     }
     if (max_dc <= 8) {
@@ -1372,15 +1425,15 @@ hipError_t kernels_state_in(const Layout& lay, const DevCode& c, const DevState&
   FECL_DISPATCH(lay, launch_state_in<F>(c, s, state, dbl, frames, frame0, st));
 }
 hipError_t kernels_vn_init(const Layout& lay, const DevCode& c, const DevState& s, int max_dv, hipStream_t st) {
-  FECL_DISPATCH(lay, launch_vn<F>(c, s, DevOut{}, max_dv, 0, 1, lay.vpw, st));
+  FECL_DISPATCH(lay, launch_vn<F>(c, s, DevOut{}, max_dv, 0, 1, lay.vpw, false, st));
 }
 hipError_t kernels_check(const Layout& lay, const DevCode& c, const DevState& s, const DevOpts& o, int max_dc,
                          hipStream_t st) {
   FECL_DISPATCH(lay, launch_cn<F>(c, s, o, max_dc, st));
 }
 hipError_t kernels_variable(const Layout& lay, const DevCode& c, const DevState& s, const DevOut& out, int max_dv,
-                            int fused, hipStream_t st) {
-  FECL_DISPATCH(lay, launch_vn<F>(c, s, out, max_dv, fused, 0, lay.vpw, st));
+                            int fused, bool nt, hipStream_t st) {
+  FECL_DISPATCH(lay, launch_vn<F>(c, s, out, max_dv, fused, 0, lay.vpw, nt, st));
 }
 hipError_t kernels_syndrome_control(const Layout& lay, const DevCode& c, const DevState& s, int iterations,
                                     int fused, int step, int max_dc, hipStream_t st) {

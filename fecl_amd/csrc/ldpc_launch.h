@@ -42,6 +42,10 @@ bool vpw_fits(const Layout& lay, int max_dv);
 inline int var_table_stride(int max_dv) { return max_dv <= 4 ? 4 : max_dv <= 8 ? 8 : max_dv <= 16 ? 16 : 32; }
 
 hipError_t kernels_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t st);
+// Permutation of FP32 blocks: out[f][i] = in[f][seq[i]] (permute), or out[f][seq[i]] = in[f][i]
+// with the other entries zeroed (inverse).
+hipError_t kernels_permute(const int64_t* seq, int64_t out_size, int64_t in_size, const float* in, float* out,
+                           int64_t frames, bool inverse, hipStream_t st);
 // Systematic staircase encoder: msg [frames][K] (0/1 bytes) -> codeword [frames][N].
 hipError_t kernels_encode(const DevCode& c, const uint8_t* msg, uint8_t* cw, int64_t frames, hipStream_t st);
 hipError_t kernels_ingest(const Layout& lay, const DevCode& c, const DevState& s, const void* llr, const void* syst,
@@ -51,8 +55,9 @@ hipError_t kernels_state_in(const Layout& lay, const DevCode& c, const DevState&
 hipError_t kernels_vn_init(const Layout& lay, const DevCode& c, const DevState& s, int max_dv, hipStream_t st);
 hipError_t kernels_check(const Layout& lay, const DevCode& c, const DevState& s, const DevOpts& o, int max_dc,
                          hipStream_t st);
+// nt: non-temporal loads and stores for the message rows (see DevOpts::nt_rows)
 hipError_t kernels_variable(const Layout& lay, const DevCode& c, const DevState& s, const DevOut& out, int max_dv,
-                            int fused, hipStream_t st);
+                            int fused, bool nt, hipStream_t st);
 hipError_t kernels_syndrome_control(const Layout& lay, const DevCode& c, const DevState& s, int iterations, int fused,
                                     int step, int max_dc, hipStream_t st);
 // Serving: write admitted frames' LLRs into their slots; read finished slots' results.

@@ -232,7 +232,8 @@ int enqueue_round(fecl_ldpc_server* sv, Events* ev, int64_t e, const std::vector
     Pool& P = *pp;
     if (!P.live) continue;
     HIP_TRY(kernels_check(P.lay, P.h->dc, P.s, P.opts, P.h->code.max_row_degree, sv->st));
-    HIP_TRY(kernels_variable(P.lay, P.h->dc, P.s, P.out, P.h->code.max_var_degree, P.opts.fused_syn, sv->st));
+    HIP_TRY(kernels_variable(P.lay, P.h->dc, P.s, P.out, P.h->code.max_var_degree, P.opts.fused_syn,
+                             (P.opts.nt_rows & 2) != 0, sv->st));
     HIP_TRY(kernels_syndrome_control(P.lay, P.h->dc, P.s, P.h->iterations, P.opts.fused_syn, -1,
                                      P.h->code.max_row_degree, sv->st));
   }
@@ -417,6 +418,7 @@ int fecl_ldpc_server_create(fecl_ldpc* const* handles, int32_t count, const int6
     int64_t want = (slots && slots[i] > 0) ? slots[i] : (memory_bytes / unset) / slot_bytes(h, P->lay);
     int64_t tiles = std::max<int64_t>(1, want / tw);
     P->nslots = tiles * tw;
+    P->opts.nt_rows = nt_rows_for(h, P->lay, tiles);
     if (int rc = prepare(h, &P->ws, P->lay, P->nslots, false, &P->s, &P->out)) return rc;
     HIP_TRY(kernels_fill_i32(P->s.iter, P->nslots, 0, nullptr));
     HIP_TRY(hipMemset(P->s.phase, PH_FREE, (size_t)P->nslots));

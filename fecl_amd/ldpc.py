@@ -237,6 +237,10 @@ class Ldpc:
         """Syndrome in the next check kernel (True) or in its own kernel (False, default)."""
         check(lib().fecl_ldpc_set_fused_syndrome(self._h, int(bool(enable))))
 
+    def setCachePolicy(self, policy):
+        """Non-temporal message rows: bit 0 check kernel, bit 1 variable kernel; -1 automatic."""
+        check(lib().fecl_ldpc_set_cache_policy(self._h, int(policy)))
+
     def setPassFrames(self, frames):
         """Decode batches in passes of at most `frames` frames (0 = 8192); results are the same."""
         check(lib().fecl_ldpc_set_pass_frames(self._h, int(frames)))
@@ -266,6 +270,10 @@ class Ldpc:
         return best.value, {k: ms[i] for i, k in enumerate(keys) if ms[i] >= 0}
 
     # ---- encode / check (host)
+    def puncturing(self, options):
+        """Ldpc::puncturing (src/Ldpc.h:94): FeCl's Permutation for these PunctureOptions."""
+        return puncturing(self._syst, self._parity, options)
+
     def encode(self, msg, stream=None):
         """Codec::encode: message bits [F*K] (0/1) -> codeword [F*N].  A CUDA uint8 tensor is
         encoded on the device (on the current stream) and the codeword returned as one."""
@@ -393,6 +401,94 @@ def _is_cuda_tensor(x):
     except ImportError:
         return False
     return isinstance(x, torch.Tensor) and x.is_cuda
+
+
+class PunctureOptions:
+    """Ldpc::PunctureOptions (src/detail/Ldpc.h:76-82): mask and systMask, lists of 0/1."""
+
+    def __init__(self):
+        self._mask, self._syst = [], []
+
+    def mask(self, m=None):
+        if m is None:
+            return list(self._mask)
+        self._mask = [1 if x else 0 for x in m]
+        return self
+
+    def systMask(self, m=None):
+        if m is None:
+            return list(self._syst)
+        self._syst = [1 if x else 0 for x in m]
+        return self
+
+
+class Permutation:
+    """fec::Permutation (src/Permutation.h): output element i is input element sequence[i].
+    permute: blocks of inputSize -> outputSize; dePermute: blocks of outputSize -> inputSize
+    with the entries no index names left 0.  numpy arrays are handled on the host; CUDA float32
+    tensors on the device (fecl_permute_device, on the current stream)."""
+
+    def __init__(self, sequence, inputSize=None):
+        self._seq = np.ascontiguousarray(sequence, dtype=np.int64).ravel()
+        self._in = int(inputSize) if inputSize is not None else (int(self._seq.max()) + 1 if self._seq.size else 0)
+        self._dev = {}
+
+    def inputSize(self):
+        return self._in
+
+    def outputSize(self):
+        return int(self._seq.size)
+
+    def __getitem__(self, i):
+        return int(self._seq[i])
+
+    def _device(self, x, inverse, stream):
+        import torch
+        v = x.contiguous().view(-1)
+        if v.dtype != torch.float32:
+            raise ValueError("device permutation takes float32 tensors")
+        n_from = self.outputSize() if inverse else self._in
+        n_to = self._in if inverse else self.outputSize()
+        if n_from == 0 or v.numel() % n_from:
+            raise ValueError("Invalid input size")
+        frames = v.numel() // n_from
+        key = v.device.index
+        if key not in self._dev:
+            self._dev[key] = torch.from_numpy(self._seq).to(v.device)
+        out = torch.empty(frames * n_to, dtype=torch.float32, device=v.device)
+        st = stream if stream is not None else torch.cuda.current_stream(v.device).cuda_stream
+        check(lib().fecl_permute_device(ctypes.c_void_p(self._dev[key].data_ptr()), self.outputSize(), self._in,
+                                        ctypes.c_void_p(v.data_ptr()), ctypes.c_void_p(out.data_ptr()), frames,
+                                        int(inverse), ctypes.c_void_p(st)))
+        return out
+
+    def permute(self, x, stream=None):
+        if _is_cuda_tensor(x):
+            return self._device(x, False, stream)
+        x = np.asarray(x).reshape(-1, self._in)
+        return x[:, self._seq].ravel()
+
+    def dePermute(self, x, stream=None):
+        if _is_cuda_tensor(x):
+            return self._device(x, True, stream)
+        x = np.asarray(x).reshape(-1, self.outputSize())
+        out = np.zeros((x.shape[0], self._in), dtype=x.dtype)
+        out[:, self._seq] = x
+        return out.ravel()
+
+
+def puncturing(systSize, paritySize, options):
+    """Ldpc::Structure::puncturing (src/detail/Ldpc.cpp:333-353) for a code of systSize
+    systematic bits in paritySize: FeCl's Permutation for these PunctureOptions."""
+    m = np.asarray(options.mask(), np.uint8)
+    sm = np.asarray(options.systMask(), np.uint8)
+    n = ctypes.c_int64()
+    check(lib().fecl_ldpc_puncturing(int(systSize), int(paritySize), _p(m) if m.size else None, m.size,
+                                     _p(sm) if sm.size else None, sm.size, None, 0, ctypes.byref(n)))
+    seq = np.zeros(max(n.value, 1), np.int64)
+    check(lib().fecl_ldpc_puncturing(int(systSize), int(paritySize), _p(m) if m.size else None, m.size,
+                                     _p(sm) if sm.size else None, sm.size, _p(seq), seq.size, ctypes.byref(n)))
+    return Permutation(seq[:n.value], paritySize)
 
 
 class BatchGraph:

@@ -19,6 +19,7 @@
 #include <string>
 #include <unordered_map>
 #include <utility>
+#include <algorithm>
 #include <vector>
 
 #include "../fecl_ldpc.h"
@@ -101,6 +102,41 @@ class SparseBitMatrix {
   size_t cols_ = 0;
 };
 
+// fec::Permutation (src/Permutation.h): output element i of a block is input element
+// sequence[i]; permute maps blocks of inputSize() to outputSize(), dePermute the other way with
+// the entries no index names left 0 (value-initialised, as FeCl's resized output vector).
+class Permutation {
+ public:
+  Permutation() = default;
+  explicit Permutation(const std::vector<size_t>& sequence) : sequence_(sequence) {
+    if (!sequence_.empty()) inputSize_ = *std::max_element(sequence_.begin(), sequence_.end()) + 1;
+  }
+  Permutation(std::vector<size_t> sequence, size_t inputSize) : sequence_(std::move(sequence)), inputSize_(inputSize) {}
+  size_t inputSize() const { return inputSize_; }
+  size_t outputSize() const { return sequence_.size(); }
+  size_t operator[](size_t i) const { return sequence_[i]; }
+  template <typename T1, typename T2 = T1>
+  void permute(const std::vector<T1>& input, std::vector<T2>& output) const {
+    output.resize(input.size() / inputSize() * outputSize());
+    for (size_t b = 0, o = 0; b < input.size(); b += inputSize(), o += outputSize())
+      for (size_t i = 0; i < sequence_.size(); ++i) output[o + i] = input[b + sequence_[i]];
+  }
+  template <typename T1, typename T2 = T1>
+  void dePermute(const std::vector<T1>& input, std::vector<T2>& output) const {
+    output.assign(input.size() / outputSize() * inputSize(), T2());
+    for (size_t b = 0, o = 0; b < input.size(); b += outputSize(), o += inputSize())
+      for (size_t i = 0; i < sequence_.size(); ++i) output[o + sequence_[i]] = input[b + i];
+  }
+  template <typename T>
+  std::vector<T> permute(const std::vector<T>& input) const { std::vector<T> o; permute(input, o); return o; }
+  template <typename T>
+  std::vector<T> dePermute(const std::vector<T>& input) const { std::vector<T> o; dePermute(input, o); return o; }
+
+ private:
+  std::vector<size_t> sequence_;
+  size_t inputSize_ = 0;
+};
+
 class Codec {
  public:
   template <template <typename> class A = std::allocator>
@@ -132,6 +168,14 @@ class Ldpc : public Codec {
     EncoderOptions(const SparseBitMatrix& checkMatrix) : checkMatrix_(checkMatrix) {}
     SparseBitMatrix checkMatrix() const { return checkMatrix_; }
     SparseBitMatrix checkMatrix_;
+  };
+
+  // Ldpc::PunctureOptions (src/detail/Ldpc.h:76-82)
+  struct PunctureOptions {
+    PunctureOptions() = default;
+    PunctureOptions& mask(std::vector<bool> m) { mask_ = std::move(m); return *this; }
+    PunctureOptions& systMask(std::vector<bool> m) { systMask_ = std::move(m); return *this; }
+    std::vector<bool> mask_, systMask_;
   };
 
   struct DecoderOptions {
@@ -193,6 +237,19 @@ class Ldpc : public Codec {
     decoder_ = decoder;
   }
   DecoderOptions getDecoderOptions() const { return decoder_; }
+
+  // Ldpc::puncturing (src/Ldpc.h:94; index rule of src/detail/Ldpc.cpp:333-353)
+  Permutation puncturing(const PunctureOptions& options) const {
+    std::vector<uint8_t> m(options.mask_.begin(), options.mask_.end());
+    std::vector<uint8_t> sm(options.systMask_.begin(), options.systMask_.end());
+    int64_t n = 0;
+    detail::throw_status(fecl_ldpc_puncturing(syst_, parity_, m.data(), (int64_t)m.size(), sm.data(),
+                                              (int64_t)sm.size(), nullptr, 0, &n));
+    std::vector<int64_t> seq((size_t)std::max<int64_t>(n, 1));
+    detail::throw_status(fecl_ldpc_puncturing(syst_, parity_, m.data(), (int64_t)m.size(), sm.data(),
+                                              (int64_t)sm.size(), seq.data(), (int64_t)seq.size(), &n));
+    return Permutation(std::vector<size_t>(seq.begin(), seq.begin() + n), (size_t)parity_);
+  }
 
   template <template <typename> class A, class B>
   bool check(const std::vector<B, A<B>>& parity) const {

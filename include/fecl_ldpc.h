@@ -154,6 +154,11 @@ int fecl_ldpc_get_layout(const fecl_ldpc* h, int32_t* words, int32_t* frames_per
  * messages and channel values can stay in the device's last-level cache between
  * iterations.  0 (default) = 8192 frames per pass. */
 int fecl_ldpc_set_pass_frames(fecl_ldpc* h, int64_t frames);
+/* Cache hints for the message rows (no counterpart in the reference; results are identical
+ * for any value): bit 0 = non-temporal loads/stores in the check kernel, bit 1 = in the
+ * variable kernel; -1 (default) = both once a batch's rows and channel values exceed 640 MiB,
+ * neither below (smaller batches gain from rows that stay in the Infinity Cache). */
+int fecl_ldpc_set_cache_policy(fecl_ldpc* h, int32_t policy);
 int fecl_ldpc_autotune(fecl_ldpc* h, int64_t frames, int32_t probe_iterations, int32_t* best_words,
                        double* ms /*[6]*/);
 
@@ -195,6 +200,23 @@ int fecl_ldpc_server_wait(fecl_ldpc_server* s, uint64_t ticket, double timeout_s
 int fecl_ldpc_server_stats(fecl_ldpc_server* s, int64_t* frames_done, int64_t* steps,
                            int64_t* queued_frames);
 int fecl_ldpc_server_destroy(fecl_ldpc_server* s);
+
+/* Ldpc::puncturing (src/Ldpc.h:94, src/detail/Ldpc.cpp:333-353): the index sequence of FeCl's
+ * Permutation for PunctureOptions().mask(mask).systMask(syst_mask) (0/1 bytes; a length of 0
+ * is an empty mask), for a code of syst_size systematic bits in parity_size.  The
+ * permutation's input size is parity_size; sequence (capacity entries, nullable) receives
+ * its `count` indices, exactly as FeCl forms them.  Host computation. */
+int fecl_ldpc_puncturing(int64_t syst_size, int64_t parity_size, const uint8_t* mask, int64_t mask_len,
+                         const uint8_t* syst_mask, int64_t syst_mask_len, int64_t* sequence, int64_t capacity,
+                         int64_t* count);
+/* Permutation::permute / dePermute (src/Permutation.h) of `frames` FP32 blocks on the device:
+ * permute (inverse = 0) out[f][i] = in[f][sequence[i]], blocks of input_size -> output_size;
+ * dePermute (inverse = 1) out[f][sequence[i]] = in[f][i], other entries 0, blocks of
+ * output_size -> input_size (so punctured positions enter the decoder as 0-LLR).
+ * sequence [output_size] is a device buffer of indices < input_size without repeats (as
+ * puncturing produces); enqueued on `stream` (a hipStream_t, NULL = the null stream). */
+int fecl_permute_device(const int64_t* sequence, int64_t output_size, int64_t input_size, const float* in,
+                        float* out, int64_t frames, int32_t inverse, void* stream);
 
 /* Releases the cached device workspaces of a handle (they are reused across calls). */
 int fecl_ldpc_trim(fecl_ldpc* h);

@@ -17,6 +17,9 @@
 //   iters  <code> <llr> <alg> <imax> <sf> <prec> <out>
 //          per-frame CN-update count, derived from where soDecode's state output stops
 //          changing as the iteration budget grows (FeCl does not expose the count).
+//   puncture <code> <mask> <systmask> <out>
+//          Structure::puncturing(PunctureOptions().mask(..).systMask(..)): masks are comma
+//          lists of 0/1 ("-" = empty); writes int64 inputSize, outputSize, sequence[]
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -193,6 +196,23 @@ int main(int argc, char** argv) {
         count[f] = (int32_t)I0;
       }
       write_out(std::string(argv[8]), F, 1, count.data());
+      return 0;
+    }
+    if (mode == "puncture" && argc == 6) {
+      auto H = make_code(argv[2]);
+      fd::Ldpc::Structure s{fec::Ldpc::EncoderOptions(H), make_options(2, 1, "1")};
+      auto bits = [](const std::string& a) {
+        std::vector<bool> v;
+        if (a != "-")
+          for (auto& t : split(a, ',')) v.push_back(t == "1");
+        return v;
+      };
+      auto perm = s.puncturing(fec::Ldpc::PunctureOptions().mask(bits(argv[3])).systMask(bits(argv[4])));
+      FILE* f = fopen(argv[5], "wb");
+      int64_t hdr[2] = {(int64_t)perm.inputSize(), (int64_t)perm.outputSize()};
+      fwrite(hdr, 8, 2, f);
+      for (size_t i = 0; i < perm.outputSize(); ++i) { int64_t x = (int64_t)perm[i]; fwrite(&x, 8, 1, f); }
+      fclose(f);
       return 0;
     }
     fprintf(stderr, "bad arguments\n");

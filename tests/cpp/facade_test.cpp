@@ -83,6 +83,28 @@ int main() {
     std::vector<fec::BitField<size_t>> badmsg(code.msgSize() + 1);
     expect_throw([&] { code.encode(badmsg); }, "bad message");
   }
+  {
+    // puncturing (test_decode_puncture, test/Cpp/operations.h:130-145): transmit the permuted
+    // codeword, dePermute the received LLRs (punctured positions enter as 0), decode
+    fec::Ldpc code(fec::Ldpc::EncoderOptions(H), fec::Ldpc::DecoderOptions().algorithm(fec::Approximate).iterations(50));
+    auto perm = code.puncturing(fec::Ldpc::PunctureOptions().mask({1, 0}).systMask({1}));
+    CHECK(perm.inputSize() == code.paritySize() && perm.outputSize() < code.paritySize());
+    std::vector<fec::BitField<size_t>> msg(code.msgSize() * 3, 1), parity;
+    code.encode(msg, parity);
+    auto tx = perm.permute(parity);
+    CHECK(tx.size() == perm.outputSize() * 3);
+    auto rx = distort(tx, 4.0);
+    auto llr = perm.dePermute(rx);
+    CHECK(llr.size() == code.paritySize() * 3);
+    size_t zeros = 0;
+    for (double x : llr) zeros += x == 0.0;
+    CHECK(zeros == (code.paritySize() - perm.outputSize()) * 3);
+    std::vector<fec::BitField<size_t>> out;
+    code.decode(llr, out);
+    size_t errs = 0;
+    for (size_t i = 0; i < out.size(); ++i) errs += (size_t)out[i] != (size_t)msg[i];
+    CHECK(errs == 0);
+  }
   expect_throw([] { fec::Ldpc::DvbS2::matrix(1000, 0.5); }, "bad length");
   expect_throw([] { fec::Ldpc::DvbS2::matrix(64800, 0.7); }, "bad rate");
   std::printf("%s (%d failures)\n", failures ? "FAILED" : "OK", failures);

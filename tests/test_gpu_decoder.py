@@ -352,3 +352,34 @@ def test_concurrent_calls_on_one_handle():
         t.join()
     for (h1, i1), (h2, i2) in zip(alone, got):
         assert np.array_equal(h1, h2) and np.array_equal(i1, i2)
+
+
+def test_punctured_decode_device_permutation():
+    """Puncturing (src/detail/Ldpc.cpp:333-353) as FeCl's tests use it (test/Cpp/operations.h:
+    130-145): transmit perm.permute(codeword), receive, dePermute (punctured positions enter
+    as 0-LLR), decode.  The device permutation equals the host one bit for bit, and so do the
+    decodes that follow; at this Eb/N0 every frame decodes."""
+    import torch
+    oc, H = dvbs2()
+    dec = gpu_decoder(H, F.Approximate, 50)
+    perm = dec.puncturing(F.PunctureOptions().mask([1, 0]).systMask([1]))
+    assert perm.inputSize() == oc.N and oc.K < perm.outputSize() < oc.N
+    rng = np.random.default_rng(41)
+    frames = 6
+    info = rng.integers(0, 2, (frames, oc.K), dtype=np.uint8)
+    cw = dec.encode(info).reshape(frames, oc.N)
+    tx = perm.permute(cw.astype(np.float64))
+    rate = oc.K / perm.outputSize()
+    sigma2 = 1.0 / (2 * rate * 10 ** (4.0 / 10))
+    rx = (-2.0 * ((1.0 - 2.0 * tx) + rng.normal(0, np.sqrt(sigma2), tx.shape)) / sigma2).astype(np.float32)
+    host_full = perm.dePermute(rx.astype(np.float64)).astype(np.float32)
+    dev_full = perm.dePermute(torch.from_numpy(rx).cuda())
+    dev_tx = perm.permute(dev_full)
+    torch.cuda.synchronize()
+    assert np.array_equal(dev_full.cpu().numpy(), host_full)
+    assert np.array_equal(dev_tx.cpu().numpy(), rx)
+    bits_dev, _ = dec.decode(dev_full, iterations=True)
+    torch.cuda.synchronize()
+    bits_host = dec.decode(host_full.astype(np.float64))
+    assert np.array_equal(bits_dev.cpu().numpy(), bits_host)
+    assert np.array_equal(bits_host.reshape(frames, oc.K), info)
