@@ -14,12 +14,16 @@ P, I32, I64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64
 SIGNATURES = {
     "fecl_last_error": [],
     "fecl_dvbs2_matrix": [I32, I32, I32, P, P, P],
+    "fecl_gallager_matrix": [I64, I64, I64, ctypes.c_uint64, P, P, P],
+    "fecl_ldpc_structure": [P, P, I64, I64, P, P, P],
+    "fecl_encode": [P, P, I64, I64, P, I64, P],
     "fecl_ldpc_scaling_table": [P, I64, I64, P, P, P, P, I64, P, I64, P, P],
     "fecl_ldpc_create": [P, P, I64, I64, P, P],
     "fecl_ldpc_destroy": [P],
     "fecl_ldpc_set_options": [P, P],
     "fecl_ldpc_get_options": [P, P, P, I64],
     "fecl_ldpc_sizes": [P, P, P, P, P],
+    "fecl_ldpc_checks": [P, P, P, P],
     "fecl_ldpc_encode": [P, P, I64, P],
     "fecl_ldpc_encode_device": [P, P, I64, P, P],
     "fecl_ldpc_puncturing": [I64, I64, P, I64, P, I64, P, I64, P],

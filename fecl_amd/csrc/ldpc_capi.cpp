@@ -448,6 +448,49 @@ int fecl_dvbs2_matrix(int32_t n, int32_t rate_num, int32_t rate_den, int64_t* di
   return FECL_OK;
 }
 
+int fecl_gallager_matrix(int64_t n, int64_t wc, int64_t wr, uint64_t seed, int64_t* dims, int32_t* row_ptr,
+                         int32_t* col) {
+  if (!dims) return fail(FECL_INVALID_ARGUMENT, "null dims");
+  Csr m;
+  std::string err = gallager_matrix(n, wc, wr, seed, &m);
+  if (!err.empty()) return fail(FECL_INVALID_ARGUMENT, err);
+  dims[0] = m.rows();
+  dims[1] = n;
+  dims[2] = (int64_t)m.idx.size();
+  if (row_ptr && col) {
+    std::copy(m.ptr.begin(), m.ptr.end(), row_ptr);
+    std::copy(m.idx.begin(), m.idx.end(), col);
+  }
+  return FECL_OK;
+}
+
+int fecl_ldpc_structure(const int32_t* row_ptr, const int32_t* col, int64_t M, int64_t N, int64_t* dims,
+                        int32_t* out_row_ptr, int32_t* out_col) {
+  if (!dims) return fail(FECL_INVALID_ARGUMENT, "null dims");
+  GeneratorForm g;
+  std::string err = generator_form(row_ptr, col, M, N, &g);
+  if (!err.empty()) return fail(FECL_INVALID_ARGUMENT, err);
+  dims[0] = g.M;
+  dims[1] = g.N;
+  dims[2] = g.K;
+  dims[3] = (int64_t)g.H.idx.size();
+  if (out_row_ptr && out_col) {
+    std::copy(g.H.ptr.begin(), g.H.ptr.end(), out_row_ptr);
+    std::copy(g.H.idx.begin(), g.H.idx.end(), out_col);
+  }
+  return FECL_OK;
+}
+
+int fecl_encode(const int32_t* row_ptr, const int32_t* col, int64_t M, int64_t N, const uint8_t* msg,
+                int64_t frames, uint8_t* parity) {
+  GeneratorForm g;
+  std::string err = generator_form(row_ptr, col, M, N, &g);
+  if (!err.empty()) return fail(FECL_INVALID_ARGUMENT, err);
+  if (frames < 0 || (frames > 0 && (!msg || !parity))) return fail(FECL_INVALID_ARGUMENT, "Invalid size for message");
+  for (int64_t f = 0; f < frames; ++f) encode_generator(g, msg + f * g.K, parity + f * g.N);
+  return FECL_OK;
+}
+
 int fecl_ldpc_scaling_table(const int32_t* row_ptr, int64_t M, int64_t iterations, const int64_t* keys,
                             const int64_t* lengths, const int64_t* offsets, const double* values, int64_t entries,
                             double* table, int64_t table_capacity, int32_t* rows, int32_t* cols) {
@@ -476,8 +519,13 @@ int fecl_ldpc_create(const int32_t* row_ptr, const int32_t* col, int64_t M, int6
   if (!out) return fail(FECL_INVALID_ARGUMENT, "null output handle");
   *out = nullptr;
   std::unique_ptr<fecl_ldpc> h(new fecl_ldpc());
-  std::string err = build_code(row_ptr, col, M, N, &h->code);
+  // FeCl decodes and encodes with H in its generator form (src/detail/Ldpc.cpp:46-55); for
+  // staircase codes such as DVB-S2 that form is H itself.
+  std::string err = generator_form(row_ptr, col, M, N, &h->gen);
   if (!err.empty()) return fail(FECL_INVALID_ARGUMENT, err);
+  err = build_code(h->gen.H.ptr.data(), h->gen.H.idx.data(), M, N, &h->code, h->gen.K);
+  if (!err.empty()) return fail(FECL_INVALID_ARGUMENT, err);
+  h->gen.H = Csr();  // the code holds it
   const Code& c = h->code;
   if (c.max_var_degree > 32) return fail(FECL_NOT_SUPPORTED, "variable degree above 32 is not supported");
   HIP_TRY(hipGetDevice(&h->device));
@@ -553,11 +601,25 @@ int fecl_ldpc_sizes(const fecl_ldpc* h, int64_t* msg, int64_t* syst, int64_t* pa
   return FECL_OK;
 }
 
+int fecl_ldpc_checks(const fecl_ldpc* h, int64_t* dims, int32_t* row_ptr, int32_t* col) {
+  if (int rc = check_handle(h)) return rc;
+  if (!dims) return fail(FECL_INVALID_ARGUMENT, "null dims");
+  const Code& c = h->code;
+  dims[0] = c.M;
+  dims[1] = c.N;
+  dims[2] = c.K;
+  dims[3] = c.E;
+  if (row_ptr && col) {
+    std::copy(c.row_ptr.begin(), c.row_ptr.end(), row_ptr);
+    std::copy(c.col.begin(), c.col.end(), col);
+  }
+  return FECL_OK;
+}
+
 int fecl_ldpc_encode(const fecl_ldpc* h, const uint8_t* msg, int64_t frames, uint8_t* parity) {
   if (int rc = check_handle(h)) return rc;
   if (frames < 0 || (frames > 0 && (!msg || !parity))) return fail(FECL_INVALID_ARGUMENT, "Invalid size for message");
-  if (!h->code.staircase) return fail(FECL_NOT_SUPPORTED, "encoder supports staircase (DVB-S2 type) codes only");
-  for (int64_t f = 0; f < frames; ++f) encode_staircase(h->code, msg + f * h->code.K, parity + f * h->code.N);
+  for (int64_t f = 0; f < frames; ++f) encode_generator(h->gen, msg + f * h->code.K, parity + f * h->code.N);
   return FECL_OK;
 }
 
@@ -781,12 +843,15 @@ int fecl_ldpc_autotune(fecl_ldpc* h, int64_t frames, int32_t probe_iterations, i
     choose_layout(h);
     float t = 0.0f;
     for (int rep = 0; rep < 2 && rc == FECL_OK; ++rep) {  // first run sizes the workspace
-      hipEventRecord(e0, st);
+      if (hipEventRecord(e0, st) != hipSuccess) {
+        rc = fail(FECL_DEVICE_ERROR, "autotune: hipEventRecord failed");
+        break;
+      }
       rc = device_decode(h, llr.as<float>(), nullptr, nullptr, frames, bits.as<uint8_t>(), nullptr, nullptr,
                          nullptr, nullptr, nullptr, st);
-      hipEventRecord(e1, st);
-      hipEventSynchronize(e1);
-      hipEventElapsedTime(&t, e0, e1);
+      if (rc == FECL_OK && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                            hipEventElapsedTime(&t, e0, e1) != hipSuccess))
+        rc = fail(FECL_DEVICE_ERROR, "autotune: event timing failed");
     }
     if (ms) ms[i] = t;
     if (rc == FECL_OK && t < best) { best = t; best_w = lay.words; best_v = lay.vpw; }

@@ -80,7 +80,8 @@ struct Workspace {
 }  // namespace fecl
 
 struct fecl_ldpc {
-  fecl::Code code;
+  fecl::Code code;                 // H in generator form, as the decoder uses it
+  fecl::GeneratorForm gen;         // encoder blocks of that form (its H is kept in `code`)
   int device = 0;
   fecl::engine::DevBuf d_row_ptr, d_col, d_var_ptr, d_var_edges, d_var_tab, d_sf, d_sf32;
   fecl::DevCode dc{};

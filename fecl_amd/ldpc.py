@@ -93,6 +93,49 @@ class DvbS2:
         return CheckMatrix(rp, col, N)
 
 
+class Gallager:
+    @staticmethod
+    def matrix(n, wc, wr, seed=0):
+        """Ldpc::Gallager::matrix (src/Ldpc.cpp:77-100): the regular (wc, wr) matrix FeCl builds
+        for the same arguments, n columns and n//wr*wc rows."""
+        dims = np.zeros(3, np.int64)
+        check(lib().fecl_gallager_matrix(int(n), int(wc), int(wr), int(seed) & (2**64 - 1), _p(dims), None, None))
+        M, N, E = (int(x) for x in dims)
+        rp = np.zeros(M + 1, np.int32)
+        col = np.zeros(E, np.int32)
+        check(lib().fecl_gallager_matrix(int(n), int(wc), int(wr), int(seed) & (2**64 - 1), _p(dims), _p(rp), _p(col)))
+        return CheckMatrix(rp, col, N)
+
+
+def structure(checkMatrix):
+    """(H_, K): the matrix FeCl decodes and encodes with for checkMatrix, and its message size
+    (Structure(EncoderOptions(H)).checks() / msgSize(), src/detail/Ldpc.cpp:46-55, 216-331).
+    Equal to checkMatrix for staircase codes such as DVB-S2."""
+    H = checkMatrix
+    dims = np.zeros(4, np.int64)
+    check(lib().fecl_ldpc_structure(_p(H.row_ptr), _p(H.col), H.rows(), H.cols(), _p(dims), None, None))
+    M, N, K, E = (int(x) for x in dims)
+    rp = np.zeros(M + 1, np.int32)
+    col = np.zeros(E, np.int32)
+    check(lib().fecl_ldpc_structure(_p(H.row_ptr), _p(H.col), H.rows(), H.cols(), _p(dims), _p(rp), _p(col)))
+    return CheckMatrix(rp, col, N), K
+
+
+def encode(checkMatrix, msg):
+    """Codec::encode for the code of checkMatrix on the host, without a decoder or a device:
+    msg holds frames of K = structure(checkMatrix)[1] bits; returns frames of N codeword bits in
+    the order of structure(checkMatrix)[0] (src/detail/Ldpc.cpp:180-209)."""
+    H = checkMatrix
+    _, K = structure(H)
+    m = np.ascontiguousarray(np.asarray(msg).reshape(-1), dtype=np.uint8)
+    if K == 0 or m.size % K:
+        raise ValueError("Invalid size for message")
+    frames = m.size // K
+    out = np.zeros(frames * H.cols(), np.uint8)
+    check(lib().fecl_encode(_p(H.row_ptr), _p(H.col), H.rows(), H.cols(), _p(m), frames, _p(out)))
+    return out
+
+
 class DecoderOptions:
     """Ldpc::DecoderOptions builder.  Defaults as FeCl's C++ API: Approximate, scaling 1.0;
     iterations has no default and must be set before use."""
@@ -211,7 +254,19 @@ class Ldpc:
         self._wgs = int(n)
 
     def checkMatrix(self):
+        """The matrix the decoder was built from, as given."""
         return self._H
+
+    def checks(self):
+        """Structure::checks(): the matrix in FeCl's generator form, which the decoder and the
+        encoder use (message bit k is codeword bit k; the state follows its edge order)."""
+        dims = np.zeros(4, np.int64)
+        check(lib().fecl_ldpc_checks(self._h, _p(dims), None, None))
+        M, N, K, E = (int(x) for x in dims)
+        rp = np.zeros(M + 1, np.int32)
+        col = np.zeros(E, np.int32)
+        check(lib().fecl_ldpc_checks(self._h, _p(dims), _p(rp), _p(col)))
+        return CheckMatrix(rp, col, N)
 
     def setDecoderOptions(self, decoderOptions):
         opts, keep = self._c_options(decoderOptions)

@@ -5,7 +5,7 @@
 //   fec::DecoderAlgorithm                 src/DecoderAlgorithm.h:30-34
 //   fec::BitField<T> (value use)          src/BitField.h:62-124
 //   fec::Codec::Input / Output            src/Codec.h:51-54, src/detail/Codec.h:150-189
-//   fec::Ldpc, EncoderOptions, DecoderOptions, DvbS2::matrix
+//   fec::Ldpc, EncoderOptions, DecoderOptions, DvbS2::matrix, Gallager::matrix
 //                                         src/Ldpc.h:59-108, src/detail/Ldpc.h:44-74
 //   decode / soDecode / encode / check    src/Codec.h:68-82, 134-307
 // Size errors throw std::invalid_argument with FeCl's messages; device errors throw
@@ -147,6 +147,16 @@ class Codec {
 
 class Ldpc : public Codec {
  public:
+  // Ldpc::Gallager (src/Ldpc.h:63-65): the same matrix as FeCl for the same arguments
+  struct Gallager {
+    static SparseBitMatrix matrix(size_t n, size_t wc, size_t wr, uint64_t seed = 0) {
+      int64_t dims[3];
+      detail::throw_status(fecl_gallager_matrix((int64_t)n, (int64_t)wc, (int64_t)wr, seed, dims, nullptr, nullptr));
+      std::vector<int32_t> rp(dims[0] + 1), col(dims[2]);
+      detail::throw_status(fecl_gallager_matrix((int64_t)n, (int64_t)wc, (int64_t)wr, seed, dims, rp.data(), col.data()));
+      return SparseBitMatrix(std::move(rp), std::move(col), (size_t)dims[1]);
+    }
+  };
   struct DvbS2 {
     static SparseBitMatrix matrix(size_t n, double rate) {
       static const int rates[][2] = {{1, 4}, {1, 3}, {2, 5}, {1, 2}, {3, 5}, {2, 3},

@@ -71,7 +71,30 @@ int fecl_ldpc_scaling_table(const int32_t* row_ptr, int64_t M, int64_t iteration
                             const double* values, int64_t entries,
                             double* table, int64_t table_capacity, int32_t* rows, int32_t* cols);
 
-/* fec::Ldpc(EncoderOptions(H), DecoderOptions) (src/Ldpc.h:82-84): copies H to the device. */
+/* Ldpc::Gallager::matrix(n, wc, wr, seed) (src/Ldpc.h:63-65, src/Ldpc.cpp:77-100): the same
+ * matrix as FeCl for the same arguments.  dims = {M, N, E}; pass NULL row_ptr/col to query the
+ * sizes (row_ptr has M+1 entries, col has E). */
+int fecl_gallager_matrix(int64_t n, int64_t wc, int64_t wr, uint64_t seed, int64_t* dims,
+                         int32_t* row_ptr, int32_t* col);
+
+/* The matrix FeCl decodes and encodes with for a given H: Structure(EncoderOptions(H)).checks()
+ * and msgSize() (src/detail/Ldpc.cpp:46-55, 216-331).  Rows and columns of H are permuted into
+ * [A | B | T] over [C | D | E] with T lower triangular; a codeword is then [message | parity],
+ * and message bit k is codeword bit k of this matrix.  K exceeds N - M by the number of
+ * dependent checks.  H is unchanged for staircase codes such as DVB-S2.  dims = {M, N, K, E};
+ * pass NULL out arrays to query the sizes.  Columns must be ascending within each row. */
+int fecl_ldpc_structure(const int32_t* row_ptr, const int32_t* col, int64_t M, int64_t N,
+                        int64_t* dims, int32_t* out_row_ptr, int32_t* out_col);
+
+/* Codec::encode (src/Codec.h:68-72, src/detail/Ldpc.cpp:180-209) for the code of H without a
+ * decoder handle or a device: msg [frames][K] in, codewords [frames][N] out (0/1 bytes), K and
+ * the bit order as fecl_ldpc_structure gives them. */
+int fecl_encode(const int32_t* row_ptr, const int32_t* col, int64_t M, int64_t N, const uint8_t* msg,
+                int64_t frames, uint8_t* parity);
+
+/* fec::Ldpc(EncoderOptions(H), DecoderOptions) (src/Ldpc.h:82-84): puts H in FeCl's form
+ * (fecl_ldpc_structure) and copies that to the device.  Columns must be ascending within each
+ * row. */
 int fecl_ldpc_create(const int32_t* row_ptr, const int32_t* col, int64_t M, int64_t N,
                      const fecl_decoder_options* options, fecl_ldpc** out);
 int fecl_ldpc_destroy(fecl_ldpc* h);
@@ -85,9 +108,12 @@ int fecl_ldpc_get_options(const fecl_ldpc* h, fecl_decoder_options* options, dou
 /* Codec::msgSize/systSize/paritySize/stateSize (src/Codec.h:60-63). */
 int fecl_ldpc_sizes(const fecl_ldpc* h, int64_t* msg, int64_t* syst, int64_t* parity,
                     int64_t* state);
+/* Structure::checks() (src/detail/Ldpc.h:115): the handle's matrix in FeCl's form, whose edge
+ * order is the state order.  dims = {M, N, K, E}; NULL arrays query the sizes. */
+int fecl_ldpc_checks(const fecl_ldpc* h, int64_t* dims, int32_t* row_ptr, int32_t* col);
 
-/* Codec::encode / Codec::check (src/Codec.h:68-72; src/detail/Ldpc.cpp:160-209) for
- * staircase codes such as DVB-S2 (host computation). */
+/* Codec::encode / Codec::check (src/Codec.h:68-72; src/detail/Ldpc.cpp:160-209), host
+ * computation, any code: msg [frames][K] and parity [frames][N] are 0/1 bytes. */
 int fecl_ldpc_encode(const fecl_ldpc* h, const uint8_t* msg, int64_t frames, uint8_t* parity);
 int fecl_ldpc_check(const fecl_ldpc* h, const uint8_t* parity, int64_t frames, int32_t* ok);
 /* Codec::encode on the device (same results as fecl_ldpc_encode): msg [frames][K] and parity

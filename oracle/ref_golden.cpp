@@ -17,6 +17,11 @@
 //   iters  <code> <llr> <alg> <imax> <sf> <prec> <out>
 //          per-frame CN-update count, derived from where soDecode's state output stops
 //          changing as the iteration budget grows (FeCl does not expose the count).
+//   gallager-raw <n> <wc> <wr> <seed> <out>       Gallager::matrix(...) as built (CSR, K = N-M)
+//   structure <csr> <out>                          Structure(EncoderOptions(H)).checks() and
+//          msgSize() for the H read from <csr> (CSR layout above; rows listed in order)
+//   encode <csr> <msg> <out>                       Structure::encode of every frame: msg file
+//          is int64 F, K, uint8[F*K]; writes int64 F, N, uint8[F*N]
 //   puncture <code> <mask> <systmask> <out>
 //          Structure::puncturing(PunctureOptions().mask(..).systMask(..)): masks are comma
 //          lists of 0/1 ("-" = empty); writes int64 inputSize, outputSize, sequence[]
@@ -69,6 +74,38 @@ static void write_csr(const SparseBitMatrix& H, size_t K, const char* path) {
   for (auto row = H.begin(); row < H.end(); ++row) { acc += row->size(); fwrite(&acc, 8, 1, f); }
   for (size_t e = 0; e < H.size(); ++e) { int64_t c = (int64_t)H.at(e); fwrite(&c, 8, 1, f); }
   fclose(f);
+}
+
+// CSR writer that walks the rows in order (a matrix whose rows were swapped keeps its edges
+// where they were stored, so the storage order is not used here).
+static void write_csr_rows(const SparseBitMatrix& H, size_t K, const char* path) {
+  FILE* f = fopen(path, "wb");
+  int64_t hdr[4] = {(int64_t)H.rows(), (int64_t)H.cols(), (int64_t)K, (int64_t)H.size()};
+  fwrite(hdr, 8, 4, f);
+  int64_t acc = 0;
+  fwrite(&acc, 8, 1, f);
+  for (auto row = H.begin(); row < H.end(); ++row) { acc += row->size(); fwrite(&acc, 8, 1, f); }
+  for (auto row = H.begin(); row < H.end(); ++row)
+    for (auto e = row->begin(); e < row->end(); ++e) { int64_t c = (int64_t)*e; fwrite(&c, 8, 1, f); }
+  fclose(f);
+}
+
+static SparseBitMatrix read_csr(const char* path) {
+  FILE* f = fopen(path, "rb");
+  if (!f) throw std::runtime_error("cannot open csr file");
+  int64_t hdr[4];
+  if (fread(hdr, 8, 4, f) != 4) throw std::runtime_error("short csr file");
+  std::vector<int64_t> rp(hdr[0] + 1), col(hdr[3]);
+  if (fread(rp.data(), 8, rp.size(), f) != rp.size() || fread(col.data(), 8, col.size(), f) != col.size())
+    throw std::runtime_error("short csr file");
+  fclose(f);
+  std::vector<size_t> sizes(hdr[0]);
+  for (int64_t r = 0; r < hdr[0]; ++r) sizes[r] = (size_t)(rp[r + 1] - rp[r]);
+  SparseBitMatrix H(sizes, (size_t)hdr[1]);
+  auto row = H.begin();
+  for (int64_t r = 0; r < hdr[0]; ++r, ++row)
+    for (int64_t e = rp[r]; e < rp[r + 1]; ++e) row->set((size_t)col[e]);
+  return H;
 }
 
 static std::vector<double> read_llr(const char* path, int64_t& F, int64_t& N) {
@@ -196,6 +233,33 @@ int main(int argc, char** argv) {
         count[f] = (int32_t)I0;
       }
       write_out(std::string(argv[8]), F, 1, count.data());
+      return 0;
+    }
+    if (mode == "gallager-raw" && argc == 7) {
+      auto H = fec::Ldpc::Gallager::matrix(std::stoul(argv[2]), std::stoul(argv[3]), std::stoul(argv[4]), std::stoull(argv[5]));
+      write_csr_rows(H, H.cols() - H.rows(), argv[6]);
+      return 0;
+    }
+    if (mode == "structure" && argc == 4) {
+      fd::Ldpc::Structure s{fec::Ldpc::EncoderOptions(read_csr(argv[2])), make_options(2, 1, "1")};
+      write_csr_rows(s.checks(), s.msgSize(), argv[3]);
+      return 0;
+    }
+    if (mode == "encode" && argc == 5) {
+      fd::Ldpc::Structure s{fec::Ldpc::EncoderOptions(read_csr(argv[2])), make_options(2, 1, "1")};
+      FILE* f = fopen(argv[3], "rb");
+      if (!f) throw std::runtime_error("cannot open msg file");
+      int64_t hdr[2];
+      if (fread(hdr, 8, 2, f) != 2 || hdr[1] != (int64_t)s.msgSize()) throw std::runtime_error("msg file does not fit the code");
+      std::vector<uint8_t> m(hdr[0] * hdr[1]);
+      if (fread(m.data(), 1, m.size(), f) != m.size()) throw std::runtime_error("short msg file");
+      fclose(f);
+      std::vector<fec::BitField<size_t>> msg(m.begin(), m.end()), cw(hdr[0] * s.paritySize());
+      for (int64_t i = 0; i < hdr[0]; ++i)
+        s.encode(msg.begin() + i * s.msgSize(), cw.begin() + i * s.paritySize());
+      std::vector<uint8_t> out(cw.size());
+      for (size_t i = 0; i < cw.size(); ++i) out[i] = (uint8_t)(size_t)cw[i];
+      write_out(argv[4], hdr[0], (int64_t)s.paritySize(), out.data());
       return 0;
     }
     if (mode == "puncture" && argc == 6) {

@@ -1,6 +1,6 @@
-// C++ facade test: FeCl's own LDPC test cases (test/Cpp/Ldpc.cpp:51-89, test/Cpp/operations.h)
-// written against include/fecl/Ldpc.hpp, on a DVB-S2 code (the encoder here is the staircase
-// one, so the Gallager test matrix is exercised from Python instead).  Needs a GPU.
+// C++ facade test: FeCl's own LDPC test cases (test/Cpp/Ldpc.cpp:51-113, test/Cpp/operations.h)
+// written against include/fecl/Ldpc.hpp, on FeCl's test code Gallager(2048, 3, 5, seed 0) and on
+// a DVB-S2 code.  Needs a GPU.
 #include <cmath>
 #include <cstdio>
 #include <random>
@@ -33,8 +33,7 @@ static void expect_throw(F f, const char* what) {
   ++failures;
 }
 
-int main() {
-  auto H = fec::Ldpc::DvbS2::matrix(16200, 0.5);
+static void reference_cases(const fec::SparseBitMatrix& H) {
   for (auto alg : {fec::Exact, fec::Linear, fec::Approximate}) {
     auto dec = fec::Ldpc::DecoderOptions().algorithm(alg).iterations(30);
     fec::Ldpc code(fec::Ldpc::EncoderOptions(H), dec);
@@ -83,6 +82,14 @@ int main() {
     std::vector<fec::BitField<size_t>> badmsg(code.msgSize() + 1);
     expect_throw([&] { code.encode(badmsg); }, "bad message");
   }
+}
+
+int main() {
+  auto G = fec::Ldpc::Gallager::matrix(2048, 3, 5, 0);
+  CHECK(G.rows() == 2048 / 5 * 3 && G.cols() == 2048 && G.size() == G.rows() * 5);
+  reference_cases(G);
+  auto H = fec::Ldpc::DvbS2::matrix(16200, 0.5);
+  reference_cases(H);
   {
     // puncturing (test_decode_puncture, test/Cpp/operations.h:130-145): transmit the permuted
     // codeword, dePermute the received LLRs (punctured positions enter as 0), decode
@@ -105,6 +112,20 @@ int main() {
     for (size_t i = 0; i < out.size(); ++i) errs += (size_t)out[i] != (size_t)msg[i];
     CHECK(errs == 0);
   }
+  {
+    // a code with dependent checks carries more message bits than N - M (src/detail/Ldpc.cpp:307)
+    fec::Ldpc code(fec::Ldpc::EncoderOptions(fec::Ldpc::Gallager::matrix(1000, 3, 5, 1)),
+                   fec::Ldpc::DecoderOptions().iterations(30));
+    CHECK(code.msgSize() == 402);
+    std::mt19937 g(3);
+    std::vector<fec::BitField<size_t>> msg(code.msgSize() * 4), parity, out;
+    for (auto& b : msg) b = g() & 1;
+    code.encode(msg, parity);
+    CHECK(code.check(parity));
+    code.decode(distort(parity, 4.0), out);
+    CHECK(out == msg);
+  }
+  expect_throw([] { fec::Ldpc::Gallager::matrix(10, 3, 11); }, "bad Gallager shape");
   expect_throw([] { fec::Ldpc::DvbS2::matrix(1000, 0.5); }, "bad length");
   expect_throw([] { fec::Ldpc::DvbS2::matrix(64800, 0.7); }, "bad rate");
   std::printf("%s (%d failures)\n", failures ? "FAILED" : "OK", failures);

@@ -154,3 +154,130 @@ def test_permutation_host_semantics():
 def test_puncturing_argument_errors():
     with pytest.raises(ValueError):
         F.puncturing(10, 5, F.PunctureOptions())
+
+
+# ---- Gallager matrices, FeCl's parity-check form and its encoder --------------------------
+STRUCT_GOLD = os.path.join(HERE, "golden", "fecl_structure.json")
+STRUCT_NPZ = os.path.join(HERE, "golden", "fecl_structure.npz")
+
+
+def _structure_cases():
+    import json
+    return json.load(open(STRUCT_GOLD))["cases"]
+
+
+def _csr_sha(rp, col):
+    import hashlib
+    h = hashlib.sha256()
+    h.update(np.asarray(rp, "<i4").tobytes())
+    h.update(np.asarray(col, "<i4").tobytes())
+    return h.hexdigest()
+
+
+def _case_matrix(c):
+    if "gallager" in c:
+        return F.Gallager.matrix(*c["gallager"])
+    z = np.load(STRUCT_NPZ)
+    return F.CheckMatrix(z[c["name"] + "_row_ptr"], z[c["name"] + "_col"], c["N"])
+
+
+def _gf2_rank(H):
+    """Rank of H over GF(2), rows as Python integers (Gaussian elimination)."""
+    pivots = {}
+    rank = 0
+    for r in range(H.rows()):
+        x = 0
+        for c in H.col[H.row_ptr[r]:H.row_ptr[r + 1]]:
+            x ^= 1 << int(c)
+        while x:
+            top = x.bit_length() - 1
+            if top not in pivots:
+                pivots[top] = x
+                rank += 1
+                break
+            x ^= pivots[top]
+    return rank
+
+
+def _col_degrees(H):
+    return np.bincount(H.col, minlength=H.cols())
+
+
+@pytest.mark.parametrize("case", [c for c in _structure_cases() if "gallager" in c], ids=lambda c: c["name"])
+def test_gallager_matrix_matches_stock_fecl(case):
+    """Gallager::matrix (src/Ldpc.cpp:77-100) gives FeCl's matrix for the same arguments,
+    including n not a multiple of wr and a seed above 2^32 (FeCl seeds with (int)seed).  What
+    the construction fixes independently: n//wr*wc rows of wr ones, block 0 in order, and each
+    block a partition of (a permutation of) the covered columns."""
+    n, wc, wr, seed = case["gallager"]
+    H = F.Gallager.matrix(n, wc, wr, seed)
+    assert _csr_sha(H.row_ptr, H.col) == case["input_sha256"]
+    per = n // wr
+    assert H.rows() == per * wc and H.cols() == n and np.all(np.diff(H.row_ptr) == wr)
+    assert np.array_equal(H.col[:per * wr], np.arange(per * wr))
+    for b in range(1, wc):
+        block = H.col[b * per * wr:(b + 1) * per * wr]
+        assert len(np.unique(block)) == per * wr
+    assert _col_degrees(H).max() <= wc
+
+
+@pytest.mark.parametrize("case", _structure_cases(), ids=lambda c: c["name"])
+def test_structure_matches_stock_fecl(case):
+    """Structure(EncoderOptions(H)).checks() and msgSize() (src/detail/Ldpc.cpp:216-331) equal
+    stock FeCl's.  What the mathematics fixes without FeCl: the form is H with its rows and
+    columns permuted (same row and column degree multisets and edge count), and the message
+    size is N - rank(H) over GF(2), so dependent checks add message bits."""
+    H = _case_matrix(case)
+    assert _csr_sha(H.row_ptr, H.col) == case["input_sha256"]
+    S, K = F.structure(H)
+    assert K == case["K"]
+    assert _csr_sha(S.row_ptr, S.col) == case["checks_sha256"]
+    assert (S.rows(), S.cols(), S.size()) == (H.rows(), H.cols(), H.size())
+    assert sorted(np.diff(S.row_ptr)) == sorted(np.diff(H.row_ptr))
+    assert sorted(_col_degrees(S)) == sorted(_col_degrees(H))
+    assert K == H.cols() - _gf2_rank(H)
+    assert _gf2_rank(S) == _gf2_rank(H)
+    assert all(np.all(np.diff(S.col[a:b]) > 0) for a, b in zip(S.row_ptr[:-1], S.row_ptr[1:]))
+
+
+@pytest.mark.parametrize("case", _structure_cases(), ids=lambda c: c["name"])
+def test_encoder_matches_stock_fecl(case):
+    """Structure::encode (src/detail/Ldpc.cpp:180-209): FeCl's codewords for the same messages,
+    systematic in the form's column order, and every one a codeword of that form."""
+    H = _case_matrix(case)
+    z = np.load(STRUCT_NPZ)
+    K, N, frames = case["K"], case["N"], case["frames"]
+    msg = np.unpackbits(z[case["name"] + "_msg"], axis=1)[:, :K]
+    cw = np.unpackbits(z[case["name"] + "_cw"], axis=1)[:, :N]
+    out = F.encode(H, msg).reshape(frames, N)
+    assert np.array_equal(out, cw)
+    assert np.array_equal(out[:, :K], msg)
+    S, _ = F.structure(H)
+    oc = O.Code(S.row_ptr, S.col, N, K)
+    assert all(O.syndrome_weight(oc, out[f]) == 0 for f in range(frames))
+
+
+@pytest.mark.parametrize("n,rate", [(64800, (1, 2)), (64800, (1, 4)), (64800, (9, 10)), (16200, (1, 3)),
+                                    (16200, (8, 9))])
+def test_dvbs2_form_is_the_matrix_itself_and_the_encoder_the_accumulator(n, rate):
+    """For staircase codes FeCl's form leaves H unchanged (so decoding a DVB-S2 matrix uses its
+    own column order), and its encoder reduces to the parity accumulator of the oracle."""
+    H = F.DvbS2.matrix(n, rate)
+    S, K = F.structure(H)
+    assert S == H and K == H.cols() - H.rows()
+    oc = O.dvbs2(n, *rate)
+    info = np.random.default_rng(5).integers(0, 2, size=(3, K), dtype=np.uint8)
+    assert np.array_equal(F.encode(H, info).reshape(3, -1), O.encode(oc, info))
+
+
+def test_structure_argument_errors():
+    unsorted = F.CheckMatrix(np.array([0, 3, 6], np.int32), np.array([0, 2, 1, 1, 3, 4], np.int32), 6)
+    with pytest.raises(ValueError, match="ascending"):
+        F.structure(unsorted)
+    short = F.CheckMatrix(np.array([0, 3, 4], np.int32), np.array([0, 1, 2, 3], np.int32), 6)
+    with pytest.raises(ValueError, match="two edges"):
+        F.structure(short)
+    with pytest.raises(ValueError):
+        F.Gallager.matrix(10, 3, 11)
+    with pytest.raises(ValueError, match="Invalid size"):
+        F.encode(F.Gallager.matrix(30, 3, 5), np.zeros(5, np.uint8))

@@ -23,9 +23,11 @@ GOLD = np.load(__import__("os").path.join(__import__("os").path.dirname(__file__
 
 
 def gallager():
+    """FeCl's test code: the decoder is built from Gallager::matrix(2048, 3, 5, 0) as generated;
+    the oracle runs on the form stock FeCl decodes with (its Structure's checks(), a fixture)."""
     N, K = (int(x) for x in GOLD["gallager_nk"])
     oc = O.Code(GOLD["gallager_row_ptr"], GOLD["gallager_col"], N, K)
-    return oc, F.CheckMatrix(oc.row_ptr, oc.col, N)
+    return oc, F.Gallager.matrix(2048, 3, 5, 0)
 
 
 def dvbs2(n=64800, rate=(1, 2)):
@@ -108,13 +110,34 @@ def test_two_variables_per_wave(fmt, ofmt, words):
     g, _ = gallager()
     keep = g.col < g.N - 1
     row_ptr = np.concatenate([[0], np.cumsum([int(keep[a:b].sum()) for a, b in zip(g.row_ptr[:-1], g.row_ptr[1:])])])
-    oc2 = O.Code(row_ptr, g.col[keep], g.N - 1, g.N - 1 - g.M)
-    H2 = F.CheckMatrix(oc2.row_ptr, oc2.col, oc2.N)
+    H2 = F.CheckMatrix(row_ptr, g.col[keep], g.N - 1)
+    dec2 = gpu_decoder(H2, F.Linear, 20, fmt=fmt, words=words)
+    S2 = dec2.checks()
+    oc2 = O.Code(S2.row_ptr, S2.col, S2.cols(), dec2.msgSize())
     llr2 = gallager_llr(oc2.N, oc2.K, 9, 2.0, seed=5)
     ref2 = oracle_run(oc2, llr2, F.Linear, 20, fmt=ofmt)
-    dec2 = gpu_decoder(H2, F.Linear, 20, fmt=fmt, words=words)
     dec2.setVarsPerWave(2)
     assert_bit_exact(dec2.soDecode(llr2), ref2, 9, oc2.K, oc2.N, oc2.E)
+
+
+@pytest.mark.parametrize("shape", [(1000, 3, 5, 1), (1200, 4, 8, 9)])
+def test_codes_with_dependent_checks(shape):
+    """A Gallager code whose checks are dependent: FeCl's form carries K > N - M message bits
+    (src/detail/Ldpc.cpp:307) and the decoder follows that form: soDecode is bit-exact to the
+    oracle run on it, and messages from the host encoder come back through the channel."""
+    from tests.golden.vectors import awgn_llr
+    H = F.Gallager.matrix(*shape)
+    dec = gpu_decoder(H, F.Approximate, 30)
+    S, K = dec.checks(), dec.msgSize()
+    assert K > H.cols() - H.rows() and (S, K) == F.structure(H)
+    oc = O.Code(S.row_ptr, S.col, S.cols(), K)
+    llr = gallager_llr(oc.N, oc.K, 6, 2.0, seed=8)
+    assert_bit_exact(dec.soDecode(llr), oracle_run(oc, llr, F.Approximate, 30), 6, oc.K, oc.N, oc.E)
+    rng = np.random.default_rng(4)
+    msg = rng.integers(0, 2, size=(5, K), dtype=np.uint8)
+    cw = dec.encode(msg).reshape(5, oc.N)
+    assert dec.check(cw)
+    assert np.array_equal(dec.decode(awgn_llr(cw, K / oc.N, 4.0, rng)).reshape(5, K), msg)
 
 
 @pytest.mark.parametrize("pass_frames", [0, 256, 300])
