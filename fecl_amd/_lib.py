@@ -24,6 +24,10 @@ SIGNATURES = {
     "fecl_ldpc_get_options": [P, P, P, I64],
     "fecl_ldpc_sizes": [P, P, P, P, P],
     "fecl_ldpc_checks": [P, P, P, P],
+    "fecl_ldpc_save": [P, I32, P, I64, P],
+    "fecl_ldpc_load": [P, I64, P, P],
+    "fecl_archive_save": [P, P, I64, I64, P, I32, P, I64, P],
+    "fecl_archive_load": [P, I64, P, P, P, P, P, I64, P],
     "fecl_ldpc_encode": [P, P, I64, P],
     "fecl_ldpc_encode_device": [P, P, I64, P, P],
     "fecl_ldpc_puncturing": [I64, I64, P, I64, P, I64, P, I64, P],

@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "../../include/fecl_ldpc.h"
+#include "ldpc_archive.h"
 #include "ldpc_code.h"
 #include "ldpc_engine.h"
 #include "ldpc_launch.h"
@@ -514,16 +515,18 @@ int fecl_ldpc_scaling_table(const int32_t* row_ptr, int64_t M, int64_t iteration
   return FECL_OK;
 }
 
-int fecl_ldpc_create(const int32_t* row_ptr, const int32_t* col, int64_t M, int64_t N,
-                     const fecl_decoder_options* options, fecl_ldpc** out) {
+}  // extern "C"
+
+namespace {
+
+// fec::Ldpc(const Structure&) (src/Ldpc.h:83): a decoder for a structure already in generator
+// form (from fecl_ldpc_create's transform or from a saved codec).
+int create_from_form(GeneratorForm&& form, const fecl_decoder_options* options, fecl_ldpc** out) {
   if (!out) return fail(FECL_INVALID_ARGUMENT, "null output handle");
   *out = nullptr;
   std::unique_ptr<fecl_ldpc> h(new fecl_ldpc());
-  // FeCl decodes and encodes with H in its generator form (src/detail/Ldpc.cpp:46-55); for
-  // staircase codes such as DVB-S2 that form is H itself.
-  std::string err = generator_form(row_ptr, col, M, N, &h->gen);
-  if (!err.empty()) return fail(FECL_INVALID_ARGUMENT, err);
-  err = build_code(h->gen.H.ptr.data(), h->gen.H.idx.data(), M, N, &h->code, h->gen.K);
+  h->gen = std::move(form);
+  std::string err = build_code(h->gen.H.ptr.data(), h->gen.H.idx.data(), h->gen.M, h->gen.N, &h->code, h->gen.K);
   if (!err.empty()) return fail(FECL_INVALID_ARGUMENT, err);
   h->gen.H = Csr();  // the code holds it
   const Code& c = h->code;
@@ -559,6 +562,132 @@ int fecl_ldpc_create(const int32_t* row_ptr, const int32_t* col, int64_t M, int6
   fecl_decoder_options def{FECL_APPROXIMATE, 1, nullptr, 1, c.max_row_degree - 1, FECL_MSG_F32};
   if (int rc = set_options(h.get(), options ? options : &def)) return rc;
   *out = h.release();
+  return FECL_OK;
+}
+
+// The archive fields of a codec with options o on a code of maximum row degree maxdeg.
+int archive_options(const fecl_decoder_options* o, int32_t maxdeg, int32_t wgs, ArchiveOptions* a) {
+  if (!o) return fail(FECL_INVALID_ARGUMENT, "null options");
+  if (o->algorithm < 0 || o->algorithm > 2) return fail(FECL_INVALID_ARGUMENT, "Invalid decoder algorithm");
+  if (o->iterations < 1) return fail(FECL_INVALID_ARGUMENT, "iterations must be >= 1");
+  a->algorithm = o->algorithm;
+  a->iterations = o->iterations;
+  a->cols = maxdeg - 1;
+  if (o->scaling) {
+    if (o->scaling_cols != a->cols || (o->scaling_rows != 1 && o->scaling_rows != o->iterations))
+      return fail(FECL_INVALID_ARGUMENT, "Wrong size for scaling factor");
+    a->rows = o->scaling_rows;
+    a->scaling.assign(o->scaling, o->scaling + (size_t)(a->rows * a->cols));
+  } else {
+    a->rows = 1;
+    a->scaling.assign((size_t)a->cols, 1.0);
+  }
+  a->work_group_size = wgs;
+  return FECL_OK;
+}
+
+int put_archive(const std::vector<uint8_t>& bytes, uint8_t* buffer, int64_t capacity, int64_t* size) {
+  if (!size) return fail(FECL_INVALID_ARGUMENT, "null size");
+  *size = (int64_t)bytes.size();
+  if (!buffer) return FECL_OK;
+  if (capacity < (int64_t)bytes.size()) return fail(FECL_INVALID_ARGUMENT, "archive buffer too small");
+  std::memcpy(buffer, bytes.data(), bytes.size());
+  return FECL_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fecl_ldpc_create(const int32_t* row_ptr, const int32_t* col, int64_t M, int64_t N,
+                     const fecl_decoder_options* options, fecl_ldpc** out) {
+  if (!out) return fail(FECL_INVALID_ARGUMENT, "null output handle");
+  *out = nullptr;
+  // FeCl decodes and encodes with H in its generator form (src/detail/Ldpc.cpp:46-55); for
+  // staircase codes such as DVB-S2 that form is H itself.
+  GeneratorForm form;
+  std::string err = generator_form(row_ptr, col, M, N, &form);
+  if (!err.empty()) return fail(FECL_INVALID_ARGUMENT, err);
+  return create_from_form(std::move(form), options, out);
+}
+
+int fecl_ldpc_save(const fecl_ldpc* h, int32_t work_group_size, uint8_t* buffer, int64_t capacity, int64_t* size) {
+  if (int rc = check_handle(h)) return rc;
+  const Code& c = h->code;
+  GeneratorForm g = h->gen;
+  g.H.ptr = c.row_ptr;
+  g.H.idx = c.col;
+  ArchiveOptions a;
+  a.algorithm = h->algorithm;
+  a.iterations = h->iterations;
+  a.rows = h->sf_rows;
+  a.cols = h->sf_cols;
+  a.scaling = h->sf;
+  a.work_group_size = work_group_size;
+  std::vector<uint8_t> bytes;
+  archive_write(g, c.E, a, &bytes);
+  return put_archive(bytes, buffer, capacity, size);
+}
+
+int fecl_ldpc_load(const uint8_t* buffer, int64_t size, fecl_ldpc** out, int32_t* work_group_size) {
+  if (!out) return fail(FECL_INVALID_ARGUMENT, "null output handle");
+  *out = nullptr;
+  GeneratorForm g;
+  ArchiveOptions a;
+  std::string err = archive_read(buffer, size, &g, &a);
+  if (!err.empty()) return fail(FECL_INVALID_ARGUMENT, err);
+  if (a.iterations < 1 || a.iterations > INT32_MAX) return fail(FECL_INVALID_ARGUMENT, "iterations must be >= 1");
+  fecl_decoder_options o{a.algorithm, (int32_t)a.iterations, a.scaling.data(), (int32_t)a.rows, (int32_t)a.cols,
+                         FECL_MSG_F32};
+  if (int rc = create_from_form(std::move(g), &o, out)) return rc;
+  if (work_group_size) *work_group_size = a.work_group_size;
+  return FECL_OK;
+}
+
+int fecl_archive_save(const int32_t* row_ptr, const int32_t* col, int64_t M, int64_t N,
+                      const fecl_decoder_options* options, int32_t work_group_size, uint8_t* buffer,
+                      int64_t capacity, int64_t* size) {
+  GeneratorForm g;
+  std::string err = generator_form(row_ptr, col, M, N, &g);
+  if (!err.empty()) return fail(FECL_INVALID_ARGUMENT, err);
+  int32_t maxdeg = 0;
+  for (int64_t r = 0; r < M; ++r) maxdeg = std::max(maxdeg, row_ptr[r + 1] - row_ptr[r]);
+  ArchiveOptions a;
+  if (int rc = archive_options(options, maxdeg, work_group_size, &a)) return rc;
+  std::vector<uint8_t> bytes;
+  archive_write(g, row_ptr[M], a, &bytes);
+  return put_archive(bytes, buffer, capacity, size);
+}
+
+int fecl_archive_load(const uint8_t* buffer, int64_t size, int64_t* dims, int32_t* row_ptr, int32_t* col,
+                      fecl_decoder_options* options, double* scaling, int64_t capacity, int32_t* work_group_size) {
+  if (!dims) return fail(FECL_INVALID_ARGUMENT, "null dims");
+  GeneratorForm g;
+  ArchiveOptions a;
+  std::string err = archive_read(buffer, size, &g, &a);
+  if (!err.empty()) return fail(FECL_INVALID_ARGUMENT, err);
+  dims[0] = g.M;
+  dims[1] = g.N;
+  dims[2] = g.K;
+  dims[3] = (int64_t)g.H.idx.size();
+  if (row_ptr && col) {
+    std::copy(g.H.ptr.begin(), g.H.ptr.end(), row_ptr);
+    std::copy(g.H.idx.begin(), g.H.idx.end(), col);
+  }
+  if (options) {
+    options->algorithm = a.algorithm;
+    options->iterations = (int32_t)std::min<int64_t>(a.iterations, INT32_MAX);
+    options->scaling_rows = (int32_t)a.rows;
+    options->scaling_cols = (int32_t)a.cols;
+    options->msg_format = FECL_MSG_F32;
+    options->scaling = nullptr;
+    if (scaling) {
+      if ((int64_t)a.scaling.size() > capacity) return fail(FECL_INVALID_ARGUMENT, "scaling buffer too small");
+      std::copy(a.scaling.begin(), a.scaling.end(), scaling);
+      options->scaling = scaling;
+    }
+  }
+  if (work_group_size) *work_group_size = a.work_group_size;
   return FECL_OK;
 }
 

@@ -200,6 +200,50 @@ class _COptions(ctypes.Structure):
                 ("scaling_cols", ctypes.c_int32), ("msg_format", ctypes.c_int32)]
 
 
+def _options_from_table(algorithm, iterations, table, msg_format=MsgFormat.F32):
+    scaling = {d + 2: [float(x) for x in table[:, d]] for d in range(table.shape[1])}
+    return DecoderOptions().algorithm(algorithm).iterations(iterations).scalingFactor(scaling).msgFormat(msg_format)
+
+
+def archive_save(checkMatrix, decoderOptions, workGroupSize=8):
+    """FeCl's saved form of fec::Ldpc(EncoderOptions(checkMatrix), decoderOptions,
+    workGroupSize): the bytes fec::detail::save writes (src/Serialization.h:92-101) with the
+    Matlab wrapper's type registration.  Host only; no device is needed."""
+    H = checkMatrix
+    o = decoderOptions
+    if o._iterations is None or o._iterations < 1:
+        raise ValueError("iterations must be >= 1")
+    table = scaling_table(H, o)
+    opts = _COptions(int(o._algorithm), int(o._iterations), table.ctypes.data, table.shape[0], table.shape[1],
+                     int(o._msg_format))
+    n = ctypes.c_int64()
+    check(lib().fecl_archive_save(_p(H.row_ptr), _p(H.col), H.rows(), H.cols(), ctypes.byref(opts),
+                                  int(workGroupSize), None, 0, ctypes.byref(n)))
+    buf = np.zeros(n.value, np.uint8)
+    check(lib().fecl_archive_save(_p(H.row_ptr), _p(H.col), H.rows(), H.cols(), ctypes.byref(opts),
+                                  int(workGroupSize), _p(buf), len(buf), ctypes.byref(n)))
+    return buf.tobytes()
+
+
+def archive_load(data):
+    """Reads a saved fec::Ldpc (fec::detail::load, src/Serialization.h:103-115) without a
+    device: returns (checks, K, DecoderOptions, workGroupSize), checks being the stored matrix
+    in FeCl's generator form."""
+    buf = np.frombuffer(bytes(data), np.uint8)
+    dims = np.zeros(4, np.int64)
+    o = _COptions()
+    check(lib().fecl_archive_load(_p(buf), len(buf), _p(dims), None, None, ctypes.byref(o), None, 0, None))
+    M, N, K, E = (int(x) for x in dims)
+    rp = np.zeros(M + 1, np.int32)
+    col = np.zeros(E, np.int32)
+    table = np.zeros(max(1, o.scaling_rows * o.scaling_cols), np.float64)
+    wgs = ctypes.c_int32()
+    check(lib().fecl_archive_load(_p(buf), len(buf), _p(dims), _p(rp), _p(col), ctypes.byref(o), _p(table),
+                                  len(table), ctypes.byref(wgs)))
+    table = table[:o.scaling_rows * o.scaling_cols].reshape(o.scaling_rows, o.scaling_cols)
+    return CheckMatrix(rp, col, N), K, _options_from_table(o.algorithm, o.iterations, table), wgs.value
+
+
 class Ldpc:
     """fec::Ldpc over the HIP decoder.  workGroupSize is accepted for signature parity with
     FeCl (src/Ldpc.h:82-84); frames run as one device batch regardless of it."""
@@ -275,13 +319,38 @@ class Ldpc:
 
     def getDecoderOptions(self):
         o = _COptions()
-        cap = self._H.rows() and int(np.max(np.diff(self._H.row_ptr))) * 4096
-        buf = np.zeros(max(cap, 1), np.float64)
+        check(lib().fecl_ldpc_get_options(self._h, ctypes.byref(o), None, 0))
+        buf = np.zeros(max(1, o.scaling_rows * o.scaling_cols), np.float64)
         check(lib().fecl_ldpc_get_options(self._h, ctypes.byref(o), _p(buf), len(buf)))
         table = buf[:o.scaling_rows * o.scaling_cols].reshape(o.scaling_rows, o.scaling_cols)
-        scaling = {d + 2: [float(x) for x in table[:, d]] for d in range(o.scaling_cols)}
-        return (DecoderOptions().algorithm(o.algorithm).iterations(o.iterations)
-                .scalingFactor(scaling).msgFormat(o.msg_format))
+        return _options_from_table(o.algorithm, o.iterations, table, o.msg_format)
+
+    # ---- saved codecs (src/Serialization.h:81-115)
+    def save(self):
+        """FeCl's archive of this codec (fec::detail::save with the Matlab wrapper's type
+        registration), as bytes; the work group size is stored too."""
+        n = ctypes.c_int64()
+        check(lib().fecl_ldpc_save(self._h, self._wgs, None, 0, ctypes.byref(n)))
+        buf = np.zeros(n.value, np.uint8)
+        check(lib().fecl_ldpc_save(self._h, self._wgs, _p(buf), len(buf), ctypes.byref(n)))
+        return buf.tobytes()
+
+    @classmethod
+    def load(cls, data):
+        """A decoder for a codec saved by FeCl or by save() (fec::detail::load): the stored
+        structure as is (no transform), options and work group size; FP32 messages."""
+        buf = np.frombuffer(bytes(data), np.uint8)
+        self = cls.__new__(cls)
+        self._h = ctypes.c_void_p()
+        wgs = ctypes.c_int32()
+        check(lib().fecl_ldpc_load(_p(buf), len(buf), ctypes.byref(self._h), ctypes.byref(wgs)))
+        self._wgs = wgs.value
+        self._H = self.checks()
+        self._options = self.getDecoderOptions()
+        sizes = [ctypes.c_int64() for _ in range(4)]
+        check(lib().fecl_ldpc_sizes(self._h, *[ctypes.byref(s) for s in sizes]))
+        self._msg, self._syst, self._parity, self._state = (s.value for s in sizes)
+        return self
 
     # ---- device layout (no counterpart in FeCl)
     def setLayout(self, words):

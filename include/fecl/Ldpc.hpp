@@ -8,6 +8,7 @@
 //   fec::Ldpc, EncoderOptions, DecoderOptions, DvbS2::matrix, Gallager::matrix
 //                                         src/Ldpc.h:59-108, src/detail/Ldpc.h:44-74
 //   decode / soDecode / encode / check    src/Codec.h:68-82, 134-307
+//   detail::archiveSize / save / load     src/Serialization.h:81-115
 // Size errors throw std::invalid_argument with FeCl's messages; device errors throw
 // std::runtime_error.  workGroupSize is kept for signature parity: frames are decoded as one
 // device batch whatever its value.
@@ -15,7 +16,9 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <memory>
 #include <stdexcept>
+#include <type_traits>
 #include <string>
 #include <unordered_map>
 #include <utility>
@@ -248,6 +251,23 @@ class Ldpc : public Codec {
   }
   DecoderOptions getDecoderOptions() const { return decoder_; }
 
+  // fec::detail::save / load (src/Serialization.h:81-115) of this codec: FeCl's archive bytes
+  // (Boost binary archive, the Matlab wrapper's type registration).  load() keeps the stored
+  // structure as it is and uses FP32 messages.
+  std::vector<char> save() const {
+    int64_t n = 0;
+    detail::throw_status(fecl_ldpc_save(h_, workGroupSize_, nullptr, 0, &n));
+    std::vector<char> out((size_t)n);
+    detail::throw_status(fecl_ldpc_save(h_, workGroupSize_, reinterpret_cast<uint8_t*>(out.data()), n, &n));
+    return out;
+  }
+  static Ldpc load(const char* buffer, size_t size) {
+    fecl_ldpc* h = nullptr;
+    int32_t wgs = 8;
+    detail::throw_status(fecl_ldpc_load(reinterpret_cast<const uint8_t*>(buffer), (int64_t)size, &h, &wgs));
+    return Ldpc(h, wgs);
+  }
+
   // Ldpc::puncturing (src/Ldpc.h:94; index rule of src/detail/Ldpc.cpp:333-353)
   Permutation puncturing(const PunctureOptions& options) const {
     std::vector<uint8_t> m(options.mask_.begin(), options.mask_.end());
@@ -344,6 +364,24 @@ class Ldpc : public Codec {
   fecl_ldpc* handle() const { return h_; }
 
  private:
+  // a codec around a handle made by fecl_ldpc_load: its matrix and options come from the handle
+  Ldpc(fecl_ldpc* h, int workGroupSize) : workGroupSize_(workGroupSize), h_(h) {
+    int64_t dims[4];
+    detail::throw_status(fecl_ldpc_checks(h_, dims, nullptr, nullptr));
+    std::vector<int32_t> rp(dims[0] + 1), col(dims[3]);
+    detail::throw_status(fecl_ldpc_checks(h_, dims, rp.data(), col.data()));
+    H_ = SparseBitMatrix(std::move(rp), std::move(col), (size_t)dims[1]);
+    fecl_decoder_options o{};
+    detail::throw_status(fecl_ldpc_get_options(h_, &o, nullptr, 0));
+    std::vector<double> table((size_t)o.scaling_rows * o.scaling_cols);
+    detail::throw_status(fecl_ldpc_get_options(h_, &o, table.data(), (int64_t)table.size()));
+    std::unordered_map<size_t, std::vector<double>> map;
+    for (int32_t d = 0; d < o.scaling_cols; ++d)
+      for (int32_t r = 0; r < o.scaling_rows; ++r) map[(size_t)d + 2].push_back(table[(size_t)r * o.scaling_cols + d]);
+    decoder_ = DecoderOptions().algorithm((DecoderAlgorithm)o.algorithm).iterations((size_t)o.iterations).scalingFactor(map);
+    detail::throw_status(fecl_ldpc_sizes(h_, &msg_, &syst_, &parity_, &state_));
+  }
+
   fecl_decoder_options options(const DecoderOptions& d, std::vector<double>* table) const {
     if (d.iterations_ < 1) throw std::invalid_argument("iterations must be >= 1");
     std::vector<int64_t> keys, lengths, offsets;
@@ -378,5 +416,21 @@ class Ldpc : public Codec {
   fecl_ldpc* h_ = nullptr;
   int64_t msg_ = 0, syst_ = 0, parity_ = 0, state_ = 0;
 };
+
+namespace detail {
+// fec::detail::archiveSize / save / load (src/Serialization.h:81-115) for Ldpc codecs.  The
+// size is exact (FeCl's adds 4096 bytes of slack); load() also accepts buffers with slack.
+inline size_t archiveSize(const Ldpc* codec) { return codec->save().size(); }
+inline void save(const Ldpc* codec, char* buffer, size_t size) {
+  std::vector<char> bytes = codec->save();
+  if (bytes.size() > size) throw std::invalid_argument("archive buffer too small");
+  std::copy(bytes.begin(), bytes.end(), buffer);
+}
+template <class T = Ldpc>
+std::unique_ptr<T> load(const char* buffer, size_t size) {
+  static_assert(std::is_same<T, Ldpc>::value, "saved codecs load as fec::Ldpc");
+  return std::unique_ptr<T>(new Ldpc(Ldpc::load(buffer, size)));
+}
+}  // namespace detail
 
 }  // namespace fec

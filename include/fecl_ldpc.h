@@ -105,6 +105,30 @@ int fecl_ldpc_set_options(fecl_ldpc* h, const fecl_decoder_options* options);
 int fecl_ldpc_get_options(const fecl_ldpc* h, fecl_decoder_options* options, double* scaling,
                           int64_t capacity);
 
+/* Saved codecs: fec::detail::save / load of a fec::Ldpc (src/Serialization.h:81-115), in FeCl's
+ * format: a Boost 1.58 binary archive of a `const fec::Codec*` with the types Convolutional,
+ * Turbo and Ldpc registered in that order, as FeCl's Matlab wrapper writes them
+ * (wrap/wrap.h:35, wrap/Matlab/util/SaveLoad.h:40-64).  The archive holds the structure (the
+ * matrix in generator form and its encoder blocks), the decoder options and the work group
+ * size; the message format is not part of it (loaded handles use FP32 messages).
+ *
+ * fecl_ldpc_save writes the handle's archive (*size = its length; buffer NULL queries the
+ * length).  fecl_ldpc_load makes a handle from an archive, without repeating the transform;
+ * bytes after the archive are ignored.  *work_group_size (nullable) receives the stored value. */
+int fecl_ldpc_save(const fecl_ldpc* h, int32_t work_group_size, uint8_t* buffer, int64_t capacity,
+                   int64_t* size);
+int fecl_ldpc_load(const uint8_t* buffer, int64_t size, fecl_ldpc** out, int32_t* work_group_size);
+/* The same archives without a decoder handle or a device: fecl_archive_save writes the archive
+ * of fec::Ldpc(EncoderOptions(H), options, work_group_size); fecl_archive_load reads one back
+ * into the stored matrix (dims = {M, N, K, E}; NULL arrays query the sizes), options (the
+ * scaling table is copied into `scaling`, capacity doubles, when non-NULL) and work group size. */
+int fecl_archive_save(const int32_t* row_ptr, const int32_t* col, int64_t M, int64_t N,
+                      const fecl_decoder_options* options, int32_t work_group_size, uint8_t* buffer,
+                      int64_t capacity, int64_t* size);
+int fecl_archive_load(const uint8_t* buffer, int64_t size, int64_t* dims, int32_t* row_ptr,
+                      int32_t* col, fecl_decoder_options* options, double* scaling,
+                      int64_t capacity, int32_t* work_group_size);
+
 /* Codec::msgSize/systSize/paritySize/stateSize (src/Codec.h:60-63). */
 int fecl_ldpc_sizes(const fecl_ldpc* h, int64_t* msg, int64_t* syst, int64_t* parity,
                     int64_t* state);

@@ -22,6 +22,12 @@
 //          msgSize() for the H read from <csr> (CSR layout above; rows listed in order)
 //   encode <csr> <msg> <out>                       Structure::encode of every frame: msg file
 //          is int64 F, K, uint8[F*K]; writes int64 F, N, uint8[F*N]
+//   save <csr> <alg> <iters> <sf> <wgs> <out>     fec::detail::save of fec::Ldpc(Structure(H, options),
+//          wgs) as a fec::Codec*, types registered as <Convolutional, Turbo, Ldpc> (the Matlab
+//          wrapper's registration, wrap/wrap.h:35); writes the archive bytes only
+//   load <archive> <out>                           fec::detail::load<fec::Codec> of those bytes;
+//          writes int64 msg, syst, parity, state sizes, workGroupSize, algorithm, iterations,
+//          then checks() as CSR (layout above, K = msgSize) appended
 //   puncture <code> <mask> <systmask> <out>
 //          Structure::puncturing(PunctureOptions().mask(..).systMask(..)): masks are comma
 //          lists of 0/1 ("-" = empty); writes int64 inputSize, outputSize, sequence[]
@@ -33,8 +39,12 @@
 #include <vector>
 #include <memory>
 #include <stdexcept>
+#include <sstream>
 
+#include "Serialization.h"   // first: archives use typeid type info for the standard containers
 #include "Ldpc.h"
+#include "Turbo.h"
+#include "Convolutional.h"
 #include "detail/LlrMetrics.h"
 #include "BpDecoderImpl.cpp"   // FeCl's template definitions, instantiated below for float
 
@@ -260,6 +270,56 @@ int main(int argc, char** argv) {
       std::vector<uint8_t> out(cw.size());
       for (size_t i = 0; i < cw.size(); ++i) out[i] = (uint8_t)(size_t)cw[i];
       write_out(argv[4], hdr[0], (int64_t)s.paritySize(), out.data());
+      return 0;
+    }
+    if (mode == "save" && argc == 8) {
+      fd::Ldpc::Structure s{fec::Ldpc::EncoderOptions(read_csr(argv[2])),
+                            make_options(atoi(argv[3]), std::stoul(argv[4]), argv[5])};
+      fec::Ldpc codec(s, atoi(argv[6]));
+      const fec::Codec* base_pointer = &codec;
+      fd::DerivedTypeHolder<fec::Convolutional, fec::Turbo, fec::Ldpc> derived;
+      // the archive fd::save writes (src/Serialization.h:92-100), into a stream of exact length
+      // (fd::archiveSize only bounds it)
+      std::ostringstream os;
+      {
+        boost::archive::binary_oarchive oa(os);
+        derived.register_type(oa);
+        oa & BOOST_SERIALIZATION_NVP(base_pointer);
+      }
+      std::string bytes = os.str();
+      std::vector<char> check(fd::archiveSize(base_pointer, derived), 0);
+      fd::save(base_pointer, check.data(), check.size(), derived);
+      if (bytes.size() > check.size() || std::memcmp(bytes.data(), check.data(), bytes.size()) != 0)
+        throw std::runtime_error("archive differs from fec::detail::save");
+      FILE* f = fopen(argv[7], "wb");
+      fwrite(bytes.data(), 1, bytes.size(), f);
+      fclose(f);
+      return 0;
+    }
+    if (mode == "load" && argc == 4) {
+      FILE* f = fopen(argv[2], "rb");
+      if (!f) throw std::runtime_error("cannot open archive");
+      std::vector<char> buf;
+      char tmp[65536];
+      size_t n;
+      while ((n = fread(tmp, 1, sizeof tmp, f)) > 0) buf.insert(buf.end(), tmp, tmp + n);
+      fclose(f);
+      fd::DerivedTypeHolder<fec::Convolutional, fec::Turbo, fec::Ldpc> derived;
+      auto codec = fd::load<fec::Codec>(buf.data(), buf.size(), derived);
+      auto* ldpc = dynamic_cast<fec::Ldpc*>(codec.get());
+      if (!ldpc) throw std::runtime_error("archive does not hold an Ldpc codec");
+      struct Peek : fec::Ldpc {
+        static const fd::Ldpc::Structure& of(const fec::Ldpc& c) { return static_cast<const Peek&>(c).structure(); }
+      };
+      const auto& st = Peek::of(*ldpc);
+      auto opt = ldpc->getDecoderOptions();
+      FILE* o = fopen(argv[3], "wb");
+      int64_t hdr[7] = {(int64_t)codec->msgSize(), (int64_t)codec->systSize(), (int64_t)codec->paritySize(),
+                        (int64_t)codec->stateSize(), (int64_t)codec->getWorkGroupSize(), (int64_t)opt.algorithm(),
+                        (int64_t)st.iterations()};
+      fwrite(hdr, 8, 7, o);
+      fclose(o);
+      write_csr_rows(st.checks(), st.msgSize(), (std::string(argv[3]) + ".csr").c_str());
       return 0;
     }
     if (mode == "puncture" && argc == 6) {

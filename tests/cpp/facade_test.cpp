@@ -125,6 +125,29 @@ int main() {
     code.decode(distort(parity, 4.0), out);
     CHECK(out == msg);
   }
+  {
+    // save / load (test_saveLoad, test/Cpp/operations.h:106-111): the loaded codec decodes like
+    // the saved one and saves the same bytes
+    fec::Ldpc code(fec::Ldpc::EncoderOptions(G), fec::Ldpc::DecoderOptions().algorithm(fec::Linear).iterations(20), 4);
+    std::vector<char> archive(fec::detail::archiveSize(&code) + 4096);
+    fec::detail::save(&code, archive.data(), archive.size());
+    auto recover = fec::detail::load<fec::Ldpc>(archive.data(), archive.size());
+    CHECK(recover->getWorkGroupSize() == 4 && recover->msgSize() == code.msgSize());
+    CHECK(recover->getDecoderOptions().algorithm() == fec::Linear && recover->getDecoderOptions().iterations() == 20);
+    CHECK(recover->save() == code.save());
+    std::vector<fec::BitField<size_t>> msg(code.msgSize() * 2, 1), parity;
+    code.encode(msg, parity);
+    std::vector<fec::BitField<size_t>> parity2;
+    recover->encode(msg, parity2);
+    CHECK(parity == parity2);
+    auto llr = distort(parity, 1.0);
+    std::vector<double> a, b;
+    code.soDecode(fec::Codec::Input<>().parity(llr), fec::Codec::Output<>().msg(a));
+    recover->soDecode(fec::Codec::Input<>().parity(llr), fec::Codec::Output<>().msg(b));
+    CHECK(a == b);
+    std::vector<char> junk(100, 7);
+    expect_throw([&] { fec::detail::load<fec::Ldpc>(junk.data(), junk.size()); }, "bad archive");
+  }
   expect_throw([] { fec::Ldpc::Gallager::matrix(10, 3, 11); }, "bad Gallager shape");
   expect_throw([] { fec::Ldpc::DvbS2::matrix(1000, 0.5); }, "bad length");
   expect_throw([] { fec::Ldpc::DvbS2::matrix(64800, 0.7); }, "bad rate");

@@ -281,3 +281,83 @@ def test_structure_argument_errors():
         F.Gallager.matrix(10, 3, 11)
     with pytest.raises(ValueError, match="Invalid size"):
         F.encode(F.Gallager.matrix(30, 3, 5), np.zeros(5, np.uint8))
+
+
+# ---- saved codecs: FeCl's archive format ------------------------------------------------
+ARCH_GOLD = os.path.join(HERE, "golden", "fecl_archive.json")
+ARCH_NPZ = os.path.join(HERE, "golden", "fecl_archive.npz")
+
+
+def _archive_cases():
+    import json
+    return json.load(open(ARCH_GOLD))["cases"]
+
+
+def _archive_matrix(code):
+    if "gallager" in code:
+        return F.Gallager.matrix(*code["gallager"])
+    if "dvbs2" in code:
+        n, a, b = code["dvbs2"]
+        return F.DvbS2.matrix(n, (a, b))
+    z = np.load(STRUCT_NPZ)
+    return F.CheckMatrix(z[code["csr"] + "_row_ptr"], z[code["csr"] + "_col"], code["N"])
+
+
+def _archive_options(c):
+    sf = c["scaling"]
+    f = {0: [float(x) for x in sf[2:].split(",")]} if sf.startswith("v:") else float(sf)
+    return F.DecoderOptions().algorithm(c["algorithm"]).iterations(c["iterations"]).scalingFactor(f)
+
+
+@pytest.mark.parametrize("case", _archive_cases(), ids=lambda c: c["name"])
+def test_archive_save_is_byte_identical_to_stock_fecl(case):
+    """fec::detail::save of fec::Ldpc (src/Serialization.h:92-101, Matlab wrapper registration):
+    the same bytes as stock FeCl for Gallager codes (with dependent checks too), hand-made
+    matrices, a DVB-S2 code, scalar and per-iteration scaling, each algorithm."""
+    import hashlib
+    H = _archive_matrix(case["code"])
+    data = F.archive_save(H, _archive_options(case), case["work_group_size"])
+    assert len(data) == case["archive_bytes"]
+    assert hashlib.sha256(data).hexdigest() == case["archive_sha256"]
+    if case["full"]:
+        assert data == np.load(ARCH_NPZ)[case["name"]].tobytes()
+
+
+@pytest.mark.parametrize("case", [c for c in _archive_cases() if c["full"]], ids=lambda c: c["name"])
+def test_archive_load_reads_stock_fecl_archives(case):
+    """fec::detail::load (src/Serialization.h:103-115) of archives written by stock FeCl gives
+    what FeCl's own loader gives: sizes, work group size, algorithm, iterations, checks(); the
+    options and the structure are those the codec was saved with.  Bytes after the archive (the
+    Matlab wrapper saves into a buffer with 4096 bytes of slack) are ignored."""
+    data = np.load(ARCH_NPZ)[case["name"]].tobytes()
+    ld = case["loaded"]
+    for blob in (data, data + bytes(4096)):
+        S, K, opts, wgs = F.archive_load(blob)
+        assert (K, S.cols(), S.size(), wgs) == (ld["msg"], ld["parity"], ld["state"], ld["work_group_size"])
+        assert (int(opts._algorithm), opts._iterations) == (ld["algorithm"], ld["iterations"])
+        assert _csr_sha(S.row_ptr, S.col) == ld["checks_sha256"]
+        H = _archive_matrix(case["code"])
+        assert (S, K) == F.structure(H)
+        want = F.scaling_table(H, _archive_options(case))
+        for d in np.unique(np.diff(S.row_ptr)):
+            assert np.array_equal(opts._scaling[int(d)], want[:, d - 2])
+
+
+def test_archive_load_rejects_damaged_bytes():
+    """Truncated or altered archives are reported as invalid arguments, never read past."""
+    data = np.load(ARCH_NPZ)["gallager30_minsum"].tobytes()
+    for cut in list(range(0, 80)) + list(range(80, len(data), 97)):
+        with pytest.raises(ValueError):
+            F.archive_load(data[:cut])
+    wrong_class = bytearray(data)
+    wrong_class[40] = 1            # the pointer's class id: Turbo instead of Ldpc
+    with pytest.raises(ValueError, match="Ldpc codec"):
+        F.archive_load(bytes(wrong_class))
+    wrong_version = bytearray(data)
+    wrong_version[30] = 11         # Boost library version
+    with pytest.raises(ValueError, match="version"):
+        F.archive_load(bytes(wrong_version))
+    bad_size = bytearray(data)
+    bad_size[97] = 0x1f            # paritySize_ no longer the matrix's column count
+    with pytest.raises(ValueError, match="sizes"):
+        F.archive_load(bytes(bad_size))
