@@ -364,7 +364,8 @@ def copy_reference(device, gib=1, reps=5):
 
 def time_form(F, H, alg, fmt, iterations, batches, frames, K, steps, device):
     """Gb/s, FER and kernel GB/s of one algorithm / message format on the bench batches
-    (autotuned layout, one warm-up decode, then `steps` timed decodes)."""
+    (autotuned layout, one warm-up decode, `steps` timed decodes; then the same decodes again
+    with per-kernel timing events, which slow the decode, for the kernel GB/s)."""
     import torch
     from fecl_amd._lib import lib, check
     import ctypes
@@ -373,7 +374,6 @@ def time_form(F, H, alg, fmt, iterations, batches, frames, K, steps, device):
     stream = torch.cuda.current_stream(device).cuda_stream
     dec.decode(batches[0]["llr"], stream=stream)
     torch.cuda.synchronize()
-    check(lib().fecl_ldpc_profile(dec._h, 1))
     errs, its = 0, 0
     t0 = time.perf_counter()
     outs = []
@@ -382,6 +382,10 @@ def time_form(F, H, alg, fmt, iterations, batches, frames, K, steps, device):
         outs.append((b, dec.decode(b["llr"], iterations=True, stream=stream)))
     torch.cuda.synchronize()
     total = time.perf_counter() - t0
+    check(lib().fecl_ldpc_profile(dec._h, 1))
+    for i in range(steps):
+        dec.decode(batches[i % len(batches)]["llr"], stream=stream)
+    torch.cuda.synchronize()
     ms = (ctypes.c_double * 3)()
     launches = (ctypes.c_int64 * 3)()
     check(lib().fecl_ldpc_profile_read(dec._h, ms, launches))
@@ -611,15 +615,14 @@ def main():
     }
 
     if not args.no_forms:
-        # the same batches through the other forms of the path (fewer timed decodes each)
+        # the same batches through the other forms of the path (one timed decode of each batch)
         forms = {}
         for name, a2, f2 in (("linear_f32", F.Linear, F.MsgFormat.F32), ("exact_f32", F.Exact, F.MsgFormat.F32),
                              ("minsum_f16", F.Approximate, F.MsgFormat.F16),
                              ("minsum_bf16", F.Approximate, F.MsgFormat.BF16)):
             if a2 == alg and f2 == fmt:
                 continue
-            forms[name] = time_form(F, H, a2, f2, args.iterations, batches, args.frames, K, max(2, args.steps // 3),
-                                    device)
+            forms[name] = time_form(F, H, a2, f2, args.iterations, batches, args.frames, K, len(batches), device)
             print(f"[bench] {name}: {forms[name]}", file=sys.stderr, flush=True)
         result["other_forms"] = forms
         # FeCl's hooks hand over host FP64 buffers: the same decode from host memory, transfers

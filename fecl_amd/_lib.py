@@ -19,6 +19,7 @@ SIGNATURES = {
     "fecl_encode": [P, P, I64, I64, P, I64, P],
     "fecl_ldpc_scaling_table": [P, I64, I64, P, P, P, P, I64, P, I64, P, P],
     "fecl_ldpc_create": [P, P, I64, I64, P, P],
+    "fecl_ldpc_create_checks": [P, P, I64, I64, I64, P, P],
     "fecl_ldpc_destroy": [P],
     "fecl_ldpc_set_options": [P, P],
     "fecl_ldpc_get_options": [P, P, P, I64],

@@ -17,6 +17,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -302,9 +303,28 @@ int check_handle(const fecl_ldpc* h) {
 }
 
 // Host-buffer decode/soDecode (FeCl's hooks hand over host FP64 buffers): the frames go through
-// device staging in chunks, with the inputs of chunk k + 1 copied (on a second stream, into the
-// other staging buffer) while chunk k decodes.
+// device staging in chunks.  While chunk k decodes, host threads convert the inputs of chunk
+// k + 1 to FP32 (the device would round them the same way) into pinned memory, and a second
+// stream copies them to the other device staging buffer: half the bytes of the FP64 values
+// cross PCIe, at the DMA rate of pinned memory rather than through the runtime's pageable path.
 constexpr int64_t kHostChunkFrames = 512;
+constexpr unsigned kConvertThreads = 8;
+
+// dst[i] = (float)src[i] on up to kConvertThreads threads (round to nearest even, as the
+// device's double-to-float conversion).
+void to_float(const double* src, float* dst, size_t n) {
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const size_t threads = std::max<size_t>(1, std::min<size_t>({kConvertThreads, hw, n / 65536 + 1}));
+  const size_t per = (n + threads - 1) / threads;
+  auto part = [=](size_t t) {
+    const size_t a = std::min(n, per * t), b = std::min(n, per * (t + 1));
+    for (size_t i = a; i < b; ++i) dst[i] = (float)src[i];
+  };
+  std::vector<std::thread> pool;
+  for (size_t t = 1; t < threads; ++t) pool.emplace_back(part, t);
+  part(0);
+  for (auto& th : pool) th.join();
+}
 
 int host_decode(fecl_ldpc* h, const double* llr, const double* syst_in, const double* state_in, int64_t frames,
                 uint8_t* msg_bits, double* msg, double* syst, double* parity, double* state, int32_t* iterations) {
@@ -329,9 +349,16 @@ int host_decode(fecl_ldpc* h, const double* llr, const double* syst_in, const do
   const int64_t tw = layout_of(h).tw();
   const int64_t chunk = std::min<int64_t>(frames, std::max<int64_t>(tw, kHostChunkFrames / tw * tw));
   for (int b = 0; b < 2; ++b) {
-    HIP_TRY(ws->stage_in[b].ensure((size_t)chunk * c.N * 8));
-    if (syst_in) HIP_TRY(ws->stage_syst[b].ensure((size_t)chunk * c.K * 8));
-    if (state_in) HIP_TRY(ws->stage_state[b].ensure((size_t)chunk * c.E * 8));
+    HIP_TRY(ws->stage_in[b].ensure((size_t)chunk * c.N * 4));
+    HIP_TRY(ws->pin_in[b].ensure((size_t)chunk * c.N * 4));
+    if (syst_in) {
+      HIP_TRY(ws->stage_syst[b].ensure((size_t)chunk * c.K * 4));
+      HIP_TRY(ws->pin_syst[b].ensure((size_t)chunk * c.K * 4));
+    }
+    if (state_in) {
+      HIP_TRY(ws->stage_state[b].ensure((size_t)chunk * c.E * 4));
+      HIP_TRY(ws->pin_state[b].ensure((size_t)chunk * c.E * 4));
+    }
   }
   size_t out_bytes = 0;
   auto take = [&](bool want, size_t n) { size_t off = out_bytes; if (want) out_bytes += (n + 255) / 256 * 256; return off; };
@@ -344,21 +371,26 @@ int host_decode(fecl_ldpc* h, const double* llr, const double* syst_in, const do
   HIP_TRY(ws->stage_out.ensure(std::max<size_t>(out_bytes, 256)));
   char* ob = ws->stage_out.as<char>();
   const int64_t nchunks = (frames + chunk - 1) / chunk;
-  // inputs of chunk k into staging buffer k & 1, once the decode of chunk k - 2 has read it
+  // inputs of chunk k: FP32 into pinned buffer k & 1 once the copy of chunk k - 2 out of it is
+  // done, then to device staging buffer k & 1 once the decode of chunk k - 2 has read that
   auto copy_in = [&](int64_t k) -> int {
     const int b = (int)(k & 1);
     const int64_t f0 = k * chunk, n = std::min(chunk, frames - f0);
+    if (k >= 2) HIP_TRY(hipEventSynchronize(ws->copied[b]));
+    to_float(llr + f0 * c.N, ws->pin_in[b].as<float>(), (size_t)n * c.N);
+    if (syst_in) to_float(syst_in + f0 * c.K, ws->pin_syst[b].as<float>(), (size_t)n * c.K);
+    if (state_in) to_float(state_in + f0 * c.E, ws->pin_state[b].as<float>(), (size_t)n * c.E);
     if (k >= 2) HIP_TRY(hipStreamWaitEvent(cs, ws->used[b], 0));
-    HIP_TRY(hipMemcpyAsync(ws->stage_in[b].p, llr + f0 * c.N, (size_t)n * c.N * 8, hipMemcpyHostToDevice, cs));
+    HIP_TRY(hipMemcpyAsync(ws->stage_in[b].p, ws->pin_in[b].p, (size_t)n * c.N * 4, hipMemcpyHostToDevice, cs));
     if (syst_in)
-      HIP_TRY(hipMemcpyAsync(ws->stage_syst[b].p, syst_in + f0 * c.K, (size_t)n * c.K * 8, hipMemcpyHostToDevice, cs));
+      HIP_TRY(hipMemcpyAsync(ws->stage_syst[b].p, ws->pin_syst[b].p, (size_t)n * c.K * 4, hipMemcpyHostToDevice, cs));
     if (state_in)
-      HIP_TRY(hipMemcpyAsync(ws->stage_state[b].p, state_in + f0 * c.E, (size_t)n * c.E * 8, hipMemcpyHostToDevice, cs));
+      HIP_TRY(hipMemcpyAsync(ws->stage_state[b].p, ws->pin_state[b].p, (size_t)n * c.E * 4, hipMemcpyHostToDevice, cs));
     HIP_TRY(hipEventRecord(ws->copied[b], cs));
     return FECL_OK;
   };
-  // A copy from pageable memory holds the calling thread for its duration, and the decode of a
-  // chunk holds it too (it polls the device), so the next chunk's copy runs on a helper thread.
+  // The conversion holds host threads, and the decode of a chunk holds the calling thread (it
+  // polls the device), so the next chunk's inputs are prepared on a helper thread.
   // (errors are recorded per thread, so the helper hands its message back with the status)
   using Copied = std::pair<int, std::string>;
   auto copy_async = [&](int64_t k) {
@@ -380,7 +412,7 @@ int host_decode(fecl_ldpc* h, const double* llr, const double* syst_in, const do
     io.llr = ws->stage_in[b].p;
     io.syst = syst_in ? ws->stage_syst[b].p : nullptr;
     io.state = state_in ? ws->stage_state[b].p : nullptr;
-    io.dbl_in = true;
+    io.dbl_in = false;
     io.dbl_out = true;
     io.res.msg_bits = msg_bits ? (uint8_t*)(ob + o_bits) : nullptr;
     io.res.iterations = iterations ? (int32_t*)(ob + o_iter) : nullptr;
@@ -611,8 +643,34 @@ int fecl_ldpc_create(const int32_t* row_ptr, const int32_t* col, int64_t M, int6
   return create_from_form(std::move(form), options, out);
 }
 
+int fecl_ldpc_create_checks(const int32_t* row_ptr, const int32_t* col, int64_t M, int64_t N, int64_t K,
+                            const fecl_decoder_options* options, fecl_ldpc** out) {
+  if (!out) return fail(FECL_INVALID_ARGUMENT, "null output handle");
+  *out = nullptr;
+  GeneratorForm form;
+  // the encoder blocks are those of the matrix's own form when that form is the matrix itself
+  // (every staircase code); otherwise the handle decodes only
+  std::string err = generator_form(row_ptr, col, M, N, &form);
+  const bool own = err.empty() && form.K == K &&
+                   std::equal(form.H.ptr.begin(), form.H.ptr.end(), row_ptr) &&
+                   std::equal(form.H.idx.begin(), form.H.idx.end(), col);
+  if (!own) {
+    form = GeneratorForm();
+    form.M = M;
+    form.N = N;
+    form.K = K;
+    if (!row_ptr || !col || M < 1 || N <= M) return fail(FECL_INVALID_ARGUMENT, "parity-check matrix must have 1 <= rows < cols");
+    form.H.ptr.assign(row_ptr, row_ptr + M + 1);
+    form.H.idx.assign(col, col + row_ptr[M]);
+  }
+  const int rc = create_from_form(std::move(form), options, out);
+  if (rc == FECL_OK) (*out)->has_encoder = own;
+  return rc;
+}
+
 int fecl_ldpc_save(const fecl_ldpc* h, int32_t work_group_size, uint8_t* buffer, int64_t capacity, int64_t* size) {
   if (int rc = check_handle(h)) return rc;
+  if (!h->has_encoder) return fail(FECL_NOT_SUPPORTED, "this handle was made without FeCl's encoder blocks");
   const Code& c = h->code;
   GeneratorForm g = h->gen;
   g.H.ptr = c.row_ptr;
@@ -747,6 +805,7 @@ int fecl_ldpc_checks(const fecl_ldpc* h, int64_t* dims, int32_t* row_ptr, int32_
 
 int fecl_ldpc_encode(const fecl_ldpc* h, const uint8_t* msg, int64_t frames, uint8_t* parity) {
   if (int rc = check_handle(h)) return rc;
+  if (!h->has_encoder) return fail(FECL_NOT_SUPPORTED, "this handle was made without FeCl's encoder blocks");
   if (frames < 0 || (frames > 0 && (!msg || !parity))) return fail(FECL_INVALID_ARGUMENT, "Invalid size for message");
   for (int64_t f = 0; f < frames; ++f) encode_generator(h->gen, msg + f * h->code.K, parity + f * h->code.N);
   return FECL_OK;

@@ -47,9 +47,31 @@ struct DevBuf {
   template <typename T> T* as() const { return static_cast<T*>(p); }
 };
 
+// Pinned host memory (hipHostMalloc), grown on demand.
+struct HostBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  hipError_t ensure(size_t n) {
+    if (n <= bytes) return hipSuccess;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+    hipError_t e = hipHostMalloc(&p, n, hipHostMallocDefault);
+    if (e == hipSuccess) bytes = n;
+    return e;
+  }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <typename T> T* as() const { return static_cast<T*>(p); }
+};
+
 struct Workspace {
   DevBuf msg, L, hard, phase, iter, fail, flags, active, out_hard, out_ext, out_app;
   DevBuf stage_in[2], stage_syst[2], stage_state[2], stage_out;   // host path: inputs double-buffered
+  HostBuf pin_in[2], pin_syst[2], pin_state[2];                    // host path: FP32 inputs, pinned
   int32_t* host_active = nullptr;
   int host_active_cap = 0;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -62,6 +84,8 @@ struct Workspace {
     for (DevBuf* b : {&msg, &L, &hard, &phase, &iter, &fail, &flags, &active, &out_hard, &out_ext, &out_app,
                       &stage_in[0], &stage_in[1], &stage_syst[0], &stage_syst[1], &stage_state[0], &stage_state[1],
                       &stage_out})
+      b->release();
+    for (HostBuf* b : {&pin_in[0], &pin_in[1], &pin_syst[0], &pin_syst[1], &pin_state[0], &pin_state[1]})
       b->release();
     if (host_active) (void)hipHostFree(host_active);
     for (auto& e : ev)
@@ -82,6 +106,7 @@ struct Workspace {
 struct fecl_ldpc {
   fecl::Code code;                 // H in generator form, as the decoder uses it
   fecl::GeneratorForm gen;         // encoder blocks of that form (its H is kept in `code`)
+  bool has_encoder = true;         // false for a matrix given already in form whose blocks are unknown
   int device = 0;
   fecl::engine::DevBuf d_row_ptr, d_col, d_var_ptr, d_var_edges, d_var_tab, d_sf, d_sf32;
   fecl::DevCode dc{};

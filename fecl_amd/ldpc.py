@@ -325,6 +325,25 @@ class Ldpc:
         table = buf[:o.scaling_rows * o.scaling_cols].reshape(o.scaling_rows, o.scaling_cols)
         return _options_from_table(o.algorithm, o.iterations, table, o.msg_format)
 
+    @classmethod
+    def fromStructure(cls, checks, msgSize, decoderOptions=None, workGroupSize=8):
+        """fec::Ldpc(const Structure&) (src/Ldpc.h:83): a decoder for a matrix already in FeCl's
+        generator form (Structure::checks(), msgSize()), used as given.  It encodes and saves
+        only when the matrix is its own form (staircase codes)."""
+        self = cls.__new__(cls)
+        self._H = checks
+        self._wgs = int(workGroupSize)
+        self._h = ctypes.c_void_p()
+        self._options = None
+        opts, keep = self._c_options(decoderOptions or DecoderOptions().iterations(1))
+        check(lib().fecl_ldpc_create_checks(_p(checks.row_ptr), _p(checks.col), checks.rows(), checks.cols(),
+                                            int(msgSize), ctypes.byref(opts), ctypes.byref(self._h)))
+        self._options = decoderOptions
+        sizes = [ctypes.c_int64() for _ in range(4)]
+        check(lib().fecl_ldpc_sizes(self._h, *[ctypes.byref(s) for s in sizes]))
+        self._msg, self._syst, self._parity, self._state = (s.value for s in sizes)
+        return self
+
     # ---- saved codecs (src/Serialization.h:81-115)
     def save(self):
         """FeCl's archive of this codec (fec::detail::save with the Matlab wrapper's type

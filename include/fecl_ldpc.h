@@ -105,6 +105,13 @@ int fecl_ldpc_set_options(fecl_ldpc* h, const fecl_decoder_options* options);
 int fecl_ldpc_get_options(const fecl_ldpc* h, fecl_decoder_options* options, double* scaling,
                           int64_t capacity);
 
+/* fec::Ldpc(const Structure&) (src/Ldpc.h:83) for a binding inside FeCl: a decoder for a matrix
+ * already in FeCl's form (structure().checks(), msgSize() = K), used as given, without the
+ * transform.  Such a handle encodes and saves only when the matrix is its own form (staircase
+ * codes such as DVB-S2); otherwise FeCl keeps encoding with its Structure. */
+int fecl_ldpc_create_checks(const int32_t* row_ptr, const int32_t* col, int64_t M, int64_t N, int64_t K,
+                            const fecl_decoder_options* options, fecl_ldpc** out);
+
 /* Saved codecs: fec::detail::save / load of a fec::Ldpc (src/Serialization.h:81-115), in FeCl's
  * format: a Boost 1.58 binary archive of a `const fec::Codec*` with the types Convolutional,
  * Turbo and Ldpc registered in that order, as FeCl's Matlab wrapper writes them

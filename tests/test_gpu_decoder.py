@@ -433,3 +433,26 @@ def test_saved_codecs_round_trip_and_decode_like_the_original():
     a, b = built.soDecode(llr), loaded.soDecode(llr)
     for k in ("msg", "syst", "parity", "state", "iterations"):
         assert np.array_equal(np.asarray(a[k]), np.asarray(b[k])), k
+
+
+def test_decoder_from_a_structure_given_in_form():
+    """fec::Ldpc(const Structure&) (src/Ldpc.h:83), the entry a binding inside FeCl uses: the
+    matrix FeCl's Structure holds (a fixture) with its msgSize, taken as given, decodes bit for
+    bit like the decoder built from Gallager::matrix; it has no encoder unless the matrix is its
+    own form (DVB-S2), where encoding and saving match the ordinary handle."""
+    oc, H = gallager()
+    S = F.CheckMatrix(oc.row_ptr, oc.col, oc.N)
+    opts = F.DecoderOptions().algorithm(F.Approximate).iterations(30)
+    a = F.Ldpc(H, opts)
+    b = F.Ldpc.fromStructure(S, oc.K, opts)
+    assert b.checks() == a.checks() and b.msgSize() == a.msgSize()
+    llr = gallager_llr(oc.N, oc.K, 5, 2.0, seed=12)
+    ra, rb = a.soDecode(llr), b.soDecode(llr)
+    for k in ("msg", "syst", "parity", "state", "iterations"):
+        assert np.array_equal(np.asarray(ra[k]), np.asarray(rb[k])), k
+    with pytest.raises(Exception, match="encoder"):
+        b.encode(np.zeros(oc.K, np.uint8))
+    D = F.DvbS2.matrix(16200, 0.5)
+    d1, d2 = F.Ldpc(D, opts), F.Ldpc.fromStructure(D, 16200 - D.rows(), opts)
+    msg = np.random.default_rng(2).integers(0, 2, size=2 * d1.msgSize(), dtype=np.uint8)
+    assert np.array_equal(d1.encode(msg), d2.encode(msg)) and d1.save() == d2.save()
