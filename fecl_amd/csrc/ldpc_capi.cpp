@@ -308,6 +308,7 @@ int check_handle(const fecl_ldpc* h) {
 // stream copies them to the other device staging buffer: half the bytes of the FP64 values
 // cross PCIe, at the DMA rate of pinned memory rather than through the runtime's pageable path.
 constexpr int64_t kHostChunkFrames = 512;
+constexpr int64_t kHostFirstChunkFrames = 256;   // smaller: its conversion overlaps nothing
 constexpr unsigned kConvertThreads = 8;
 
 // dst[i] = (float)src[i] on up to kConvertThreads threads (round to nearest even, as the
@@ -348,6 +349,11 @@ int host_decode(fecl_ldpc* h, const double* llr, const double* syst_in, const do
   hipStream_t st = ws->own_stream, cs = ws->copy_stream;
   const int64_t tw = layout_of(h).tw();
   const int64_t chunk = std::min<int64_t>(frames, std::max<int64_t>(tw, kHostChunkFrames / tw * tw));
+  const int64_t first = std::min<int64_t>(chunk, std::max<int64_t>(tw, kHostFirstChunkFrames / tw * tw));
+  // frames [start(k), start(k) + size(k)) form chunk k
+  auto start = [&](int64_t k) { return k == 0 ? 0 : first + (k - 1) * chunk; };
+  auto size = [&](int64_t k) { return std::min(k == 0 ? first : chunk, frames - start(k)); };
+  const int64_t nchunks = 1 + (frames - first + chunk - 1) / chunk;
   for (int b = 0; b < 2; ++b) {
     HIP_TRY(ws->stage_in[b].ensure((size_t)chunk * c.N * 4));
     HIP_TRY(ws->pin_in[b].ensure((size_t)chunk * c.N * 4));
@@ -370,12 +376,11 @@ int host_decode(fecl_ldpc* h, const double* llr, const double* syst_in, const do
   const size_t o_state = take(state != nullptr, (size_t)chunk * c.E * 8);
   HIP_TRY(ws->stage_out.ensure(std::max<size_t>(out_bytes, 256)));
   char* ob = ws->stage_out.as<char>();
-  const int64_t nchunks = (frames + chunk - 1) / chunk;
   // inputs of chunk k: FP32 into pinned buffer k & 1 once the copy of chunk k - 2 out of it is
   // done, then to device staging buffer k & 1 once the decode of chunk k - 2 has read that
   auto copy_in = [&](int64_t k) -> int {
     const int b = (int)(k & 1);
-    const int64_t f0 = k * chunk, n = std::min(chunk, frames - f0);
+    const int64_t f0 = start(k), n = size(k);
     if (k >= 2) HIP_TRY(hipEventSynchronize(ws->copied[b]));
     to_float(llr + f0 * c.N, ws->pin_in[b].as<float>(), (size_t)n * c.N);
     if (syst_in) to_float(syst_in + f0 * c.K, ws->pin_syst[b].as<float>(), (size_t)n * c.K);
@@ -403,7 +408,7 @@ int host_decode(fecl_ldpc* h, const double* llr, const double* syst_in, const do
   std::future<Copied> next = copy_async(0);
   for (int64_t k = 0; k < nchunks; ++k) {
     const int b = (int)(k & 1);
-    const int64_t f0 = k * chunk, n = std::min(chunk, frames - f0);
+    const int64_t f0 = start(k), n = size(k);
     const Copied got = next.get();
     if (got.first) return fail(got.first, got.second);
     if (k + 1 < nchunks) next = copy_async(k + 1);   // overlaps this chunk's decode
