@@ -636,6 +636,19 @@ def main():
         result["host_buffer_path"] = {"Gb_per_s": round(args.frames * K / th / 1e9, 4), "ms_per_batch": round(th * 1e3, 2),
                                       "note": "FP64 LLRs in pageable host memory in, bits out, PCIe included"}
         print(f"[bench] host buffer path: {result['host_buffer_path']}", file=sys.stderr, flush=True)
+        # soDecode from host memory as an iterative receiver calls it: FP64 LLRs in, FP64
+        # a-posteriori LLRs and the state (one value per edge, 1.8 MB a frame) out
+        nso = min(512, args.frames)
+        llr_s = np.ascontiguousarray(llr_h[:nso])
+        dec.soDecode(llr_s, outputs=("msg", "state"))
+        t0 = time.perf_counter()
+        for _ in range(2):
+            dec.soDecode(llr_s, outputs=("msg", "state"))
+        ts = (time.perf_counter() - t0) / 2
+        result["host_buffer_path_sodecode"] = {
+            "Gb_per_s": round(nso * K / ts / 1e9, 4), "ms_per_batch": round(ts * 1e3, 2), "frames": nso,
+            "note": "FP64 LLRs in, FP64 a-posteriori LLRs and state out, pageable host memory, PCIe included"}
+        print(f"[bench] host soDecode path: {result['host_buffer_path_sodecode']}", file=sys.stderr, flush=True)
 
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
@@ -663,6 +676,13 @@ def main():
                                             f"{args.iterations} max iterations, {cb['seconds']:.2f} s wall",
                                   "cpu": _cpu_model()}
     if not args.no_serving:
+        # the serving pools start from a device heap without the batch bench's buffers (decoder
+        # workspaces, input batches, torch's cached blocks)
+        del dec, batches, out, kept
+        import gc
+        gc.collect()
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
         print(f"[bench] serving loop: {args.serve_seconds} s of arrivals", file=sys.stderr, flush=True)
         result["serving"] = serving_report(F, args, device, args.serve_seconds)
     print(json.dumps(result), flush=True)

@@ -302,29 +302,47 @@ int check_handle(const fecl_ldpc* h) {
   return FECL_OK;
 }
 
-// Host-buffer decode/soDecode (FeCl's hooks hand over host FP64 buffers): the frames go through
-// device staging in chunks.  While chunk k decodes, host threads convert the inputs of chunk
-// k + 1 to FP32 (the device would round them the same way) into pinned memory, and a second
-// stream copies them to the other device staging buffer: half the bytes of the FP64 values
-// cross PCIe, at the DMA rate of pinned memory rather than through the runtime's pageable path.
+// Host-buffer decode/soDecode (FeCl's hooks hand over host FP64 buffers).  The frames go
+// through the device in chunks, three stages deep:
+//   in : host threads round chunk k + 1's FP64 inputs to FP32 (the device would round them the
+//        same way) into pinned memory, and a copy stream moves them to device staging;
+//   run: chunk k decodes, its outputs (FP32 values, bits, counts) into device staging;
+//   out: a third stream moves chunk k - 1's outputs to pinned memory, and host threads widen
+//        them to FP64 (exact) and copy them into the caller's buffers.
+// Inputs and outputs cross PCIe as FP32 at the DMA rate of pinned memory; every stage has two
+// buffers, used by chunks of alternating parity.
 constexpr int64_t kHostChunkFrames = 512;
 constexpr int64_t kHostFirstChunkFrames = 256;   // smaller: its conversion overlaps nothing
 constexpr unsigned kConvertThreads = 8;
 
-// dst[i] = (float)src[i] on up to kConvertThreads threads (round to nearest even, as the
-// device's double-to-float conversion).
-void to_float(const double* src, float* dst, size_t n) {
+// Runs body(a, b) over [0, n) split across up to kConvertThreads threads.
+template <class Body>
+void parallel_ranges(size_t n, Body body) {
   const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
   const size_t threads = std::max<size_t>(1, std::min<size_t>({kConvertThreads, hw, n / 65536 + 1}));
   const size_t per = (n + threads - 1) / threads;
-  auto part = [=](size_t t) {
-    const size_t a = std::min(n, per * t), b = std::min(n, per * (t + 1));
-    for (size_t i = a; i < b; ++i) dst[i] = (float)src[i];
-  };
+  auto part = [&](size_t t) { body(std::min(n, per * t), std::min(n, per * (t + 1))); };
   std::vector<std::thread> pool;
   for (size_t t = 1; t < threads; ++t) pool.emplace_back(part, t);
   part(0);
   for (auto& th : pool) th.join();
+}
+
+// dst[i] = (float)src[i]: round to nearest even, as the device's double-to-float conversion.
+void to_float(const double* src, float* dst, size_t n) {
+  parallel_ranges(n, [=](size_t a, size_t b) {
+    for (size_t i = a; i < b; ++i) dst[i] = (float)src[i];
+  });
+}
+
+void to_double(const float* src, double* dst, size_t n) {
+  parallel_ranges(n, [=](size_t a, size_t b) {
+    for (size_t i = a; i < b; ++i) dst[i] = (double)src[i];
+  });
+}
+
+void copy_bytes(const void* src, void* dst, size_t n) {
+  parallel_ranges(n, [=](size_t a, size_t b) { std::memcpy((char*)dst + a, (const char*)src + a, b - a); });
 }
 
 int host_decode(fecl_ldpc* h, const double* llr, const double* syst_in, const double* state_in, int64_t frames,
@@ -346,7 +364,14 @@ int host_decode(fecl_ldpc* h, const double* llr, const double* syst_in, const do
       HIP_TRY(hipEventCreateWithFlags(&ws->used[b], hipEventDisableTiming));
     }
   }
-  hipStream_t st = ws->own_stream, cs = ws->copy_stream;
+  if (!ws->out_stream) {
+    HIP_TRY(hipStreamCreateWithFlags(&ws->out_stream, hipStreamNonBlocking));
+    for (int b = 0; b < 2; ++b) {
+      HIP_TRY(hipEventCreateWithFlags(&ws->decoded[b], hipEventDisableTiming));
+      HIP_TRY(hipEventCreateWithFlags(&ws->drained[b], hipEventDisableTiming));
+    }
+  }
+  hipStream_t st = ws->own_stream, cs = ws->copy_stream, os = ws->out_stream;
   const int64_t tw = layout_of(h).tw();
   const int64_t chunk = std::min<int64_t>(frames, std::max<int64_t>(tw, kHostChunkFrames / tw * tw));
   const int64_t first = std::min<int64_t>(chunk, std::max<int64_t>(tw, kHostFirstChunkFrames / tw * tw));
@@ -354,6 +379,15 @@ int host_decode(fecl_ldpc* h, const double* llr, const double* syst_in, const do
   auto start = [&](int64_t k) { return k == 0 ? 0 : first + (k - 1) * chunk; };
   auto size = [&](int64_t k) { return std::min(k == 0 ? first : chunk, frames - start(k)); };
   const int64_t nchunks = 1 + (frames - first + chunk - 1) / chunk;
+  // output sections of one chunk (same offsets in device staging and in pinned memory)
+  size_t out_bytes = 0;
+  auto take = [&](bool want, size_t n) { size_t off = out_bytes; if (want) out_bytes += (n + 255) / 256 * 256; return off; };
+  const size_t o_bits = take(msg_bits != nullptr, (size_t)chunk * c.K);
+  const size_t o_iter = take(iterations != nullptr, (size_t)chunk * 4);
+  const size_t o_msg = take(msg != nullptr, (size_t)chunk * c.K * 4);
+  const size_t o_syst = take(syst != nullptr, (size_t)chunk * c.K * 4);
+  const size_t o_par = take(parity != nullptr, (size_t)chunk * c.N * 4);
+  const size_t o_state = take(state != nullptr, (size_t)chunk * c.E * 4);
   for (int b = 0; b < 2; ++b) {
     HIP_TRY(ws->stage_in[b].ensure((size_t)chunk * c.N * 4));
     HIP_TRY(ws->pin_in[b].ensure((size_t)chunk * c.N * 4));
@@ -365,80 +399,136 @@ int host_decode(fecl_ldpc* h, const double* llr, const double* syst_in, const do
       HIP_TRY(ws->stage_state[b].ensure((size_t)chunk * c.E * 4));
       HIP_TRY(ws->pin_state[b].ensure((size_t)chunk * c.E * 4));
     }
+    HIP_TRY(ws->stage_out[b].ensure(std::max<size_t>(out_bytes, 256)));
+    HIP_TRY(ws->pin_out[b].ensure(std::max<size_t>(out_bytes, 256)));
   }
-  size_t out_bytes = 0;
-  auto take = [&](bool want, size_t n) { size_t off = out_bytes; if (want) out_bytes += (n + 255) / 256 * 256; return off; };
-  const size_t o_bits = take(msg_bits != nullptr, (size_t)chunk * c.K);
-  const size_t o_iter = take(iterations != nullptr, (size_t)chunk * 4);
-  const size_t o_msg = take(msg != nullptr, (size_t)chunk * c.K * 8);
-  const size_t o_syst = take(syst != nullptr, (size_t)chunk * c.K * 8);
-  const size_t o_par = take(parity != nullptr, (size_t)chunk * c.N * 8);
-  const size_t o_state = take(state != nullptr, (size_t)chunk * c.E * 8);
-  HIP_TRY(ws->stage_out.ensure(std::max<size_t>(out_bytes, 256)));
-  char* ob = ws->stage_out.as<char>();
-  // inputs of chunk k: FP32 into pinned buffer k & 1 once the copy of chunk k - 2 out of it is
-  // done, then to device staging buffer k & 1 once the decode of chunk k - 2 has read that
-  auto copy_in = [&](int64_t k) -> int {
-    const int b = (int)(k & 1);
-    const int64_t f0 = start(k), n = size(k);
-    if (k >= 2) HIP_TRY(hipEventSynchronize(ws->copied[b]));
-    to_float(llr + f0 * c.N, ws->pin_in[b].as<float>(), (size_t)n * c.N);
-    if (syst_in) to_float(syst_in + f0 * c.K, ws->pin_syst[b].as<float>(), (size_t)n * c.K);
-    if (state_in) to_float(state_in + f0 * c.E, ws->pin_state[b].as<float>(), (size_t)n * c.E);
-    if (k >= 2) HIP_TRY(hipStreamWaitEvent(cs, ws->used[b], 0));
-    HIP_TRY(hipMemcpyAsync(ws->stage_in[b].p, ws->pin_in[b].p, (size_t)n * c.N * 4, hipMemcpyHostToDevice, cs));
-    if (syst_in)
-      HIP_TRY(hipMemcpyAsync(ws->stage_syst[b].p, ws->pin_syst[b].p, (size_t)n * c.K * 4, hipMemcpyHostToDevice, cs));
-    if (state_in)
-      HIP_TRY(hipMemcpyAsync(ws->stage_state[b].p, ws->pin_state[b].p, (size_t)n * c.E * 4, hipMemcpyHostToDevice, cs));
-    HIP_TRY(hipEventRecord(ws->copied[b], cs));
-    return FECL_OK;
-  };
-  // The conversion holds host threads, and the decode of a chunk holds the calling thread (it
-  // polls the device), so the next chunk's inputs are prepared on a helper thread.
-  // (errors are recorded per thread, so the helper hands its message back with the status)
-  using Copied = std::pair<int, std::string>;
-  auto copy_async = [&](int64_t k) {
-    return std::async(std::launch::async, [&, k]() -> Copied {
+  // (errors are recorded per thread, so helpers hand their message back with the status)
+  using Status = std::pair<int, std::string>;
+  auto helper = [&](auto fn) {
+    return std::async(std::launch::async, [&, fn]() -> Status {
       if (hipSetDevice(h->device) != hipSuccess) return {FECL_DEVICE_ERROR, "hipSetDevice failed"};
-      const int rc = copy_in(k);
+      const int rc = fn();
       return {rc, rc ? g_err : std::string()};
     });
   };
-  std::future<Copied> next = copy_async(0);
+  // in: chunk k's inputs, FP32 into pinned buffer k & 1 once chunk k - 2's copy out of it is
+  // done, then to device staging k & 1 once chunk k - 2's decode has read that
+  auto copy_in = [&](int64_t k) {
+    return helper([&, k]() -> int {
+      const int b = (int)(k & 1);
+      const int64_t f0 = start(k), n = size(k);
+      if (k >= 2) HIP_TRY(hipEventSynchronize(ws->copied[b]));
+      to_float(llr + f0 * c.N, ws->pin_in[b].as<float>(), (size_t)n * c.N);
+      if (syst_in) to_float(syst_in + f0 * c.K, ws->pin_syst[b].as<float>(), (size_t)n * c.K);
+      if (state_in) to_float(state_in + f0 * c.E, ws->pin_state[b].as<float>(), (size_t)n * c.E);
+      if (k >= 2) HIP_TRY(hipStreamWaitEvent(cs, ws->used[b], 0));
+      HIP_TRY(hipMemcpyAsync(ws->stage_in[b].p, ws->pin_in[b].p, (size_t)n * c.N * 4, hipMemcpyHostToDevice, cs));
+      if (syst_in)
+        HIP_TRY(hipMemcpyAsync(ws->stage_syst[b].p, ws->pin_syst[b].p, (size_t)n * c.K * 4, hipMemcpyHostToDevice, cs));
+      if (state_in)
+        HIP_TRY(hipMemcpyAsync(ws->stage_state[b].p, ws->pin_state[b].p, (size_t)n * c.E * 4, hipMemcpyHostToDevice, cs));
+      HIP_TRY(hipEventRecord(ws->copied[b], cs));
+      return FECL_OK;
+    });
+  };
+  // out: chunk k's outputs from pinned buffer k & 1 (filled by the out stream) into the
+  // caller's buffers
+  auto copy_out = [&](int64_t k) {
+    return helper([&, k]() -> int {
+      const int b = (int)(k & 1);
+      const int64_t f0 = start(k), n = size(k);
+      HIP_TRY(hipEventSynchronize(ws->drained[b]));
+      const char* pb = ws->pin_out[b].as<char>();
+      if (msg_bits) copy_bytes(pb + o_bits, msg_bits + f0 * c.K, (size_t)n * c.K);
+      if (iterations) std::memcpy(iterations + f0, pb + o_iter, (size_t)n * 4);
+      if (msg) to_double((const float*)(pb + o_msg), msg + f0 * c.K, (size_t)n * c.K);
+      if (syst) to_double((const float*)(pb + o_syst), syst + f0 * c.K, (size_t)n * c.K);
+      if (parity) to_double((const float*)(pb + o_par), parity + f0 * c.N, (size_t)n * c.N);
+      if (state) to_double((const float*)(pb + o_state), state + f0 * c.E, (size_t)n * c.E);
+      return FECL_OK;
+    });
+  };
+  std::future<Status> next_in = copy_in(0), out_done[2];
+  // waits for every helper still running (they use the workspace and the caller's buffers)
+  auto settle = [&]() {
+    Status worst{FECL_OK, std::string()};
+    for (std::future<Status>* f : {&next_in, &out_done[0], &out_done[1]})
+      if (f->valid()) {
+        Status r = f->get();
+        if (r.first && !worst.first) worst = r;
+      }
+    return worst;
+  };
+  auto bail = [&](int rc) {
+    const std::string msg_here = g_err;
+    settle();
+    (void)hipStreamSynchronize(cs);
+    (void)hipStreamSynchronize(os);
+    (void)hipStreamSynchronize(st);
+    return fail(rc, msg_here);
+  };
   for (int64_t k = 0; k < nchunks; ++k) {
     const int b = (int)(k & 1);
-    const int64_t f0 = start(k), n = size(k);
-    const Copied got = next.get();
-    if (got.first) return fail(got.first, got.second);
-    if (k + 1 < nchunks) next = copy_async(k + 1);   // overlaps this chunk's decode
-    HIP_TRY(hipStreamWaitEvent(st, ws->copied[b], 0));
+    const int64_t n = size(k);
+    const Status got = next_in.get();
+    if (got.first) {
+      g_err = got.second;
+      return bail(got.first);
+    }
+    if (k + 1 < nchunks) next_in = copy_in(k + 1);   // overlaps this chunk's decode
+    if (hipStreamWaitEvent(st, ws->copied[b], 0) != hipSuccess ||
+        (k >= 2 && hipStreamWaitEvent(st, ws->drained[b], 0) != hipSuccess)) {
+      g_err = "hipStreamWaitEvent failed";
+      return bail(FECL_DEVICE_ERROR);
+    }
+    char* ob = ws->stage_out[b].as<char>();
     BatchIO io{};
     io.llr = ws->stage_in[b].p;
     io.syst = syst_in ? ws->stage_syst[b].p : nullptr;
     io.state = state_in ? ws->stage_state[b].p : nullptr;
     io.dbl_in = false;
-    io.dbl_out = true;
+    io.dbl_out = false;
     io.res.msg_bits = msg_bits ? (uint8_t*)(ob + o_bits) : nullptr;
     io.res.iterations = iterations ? (int32_t*)(ob + o_iter) : nullptr;
     io.res.msg = msg ? (void*)(ob + o_msg) : nullptr;
     io.res.syst = syst ? (void*)(ob + o_syst) : nullptr;
     io.res.parity = parity ? (void*)(ob + o_par) : nullptr;
     io.res.state = state ? (void*)(ob + o_state) : nullptr;
-    if (int rc = run_passes(h, ws.get(), st, io, n, 0, false)) {
-      if (next.valid()) next.wait();
-      return rc;
+    if (int rc = run_passes(h, ws.get(), st, io, n, 0, false)) return bail(rc);
+    if (hipEventRecord(ws->used[b], st) != hipSuccess || hipEventRecord(ws->decoded[b], st) != hipSuccess) {
+      g_err = "hipEventRecord failed";
+      return bail(FECL_DEVICE_ERROR);
     }
-    HIP_TRY(hipEventRecord(ws->used[b], st));
-    if (msg_bits) HIP_TRY(hipMemcpyAsync(msg_bits + f0 * c.K, ob + o_bits, (size_t)n * c.K, hipMemcpyDeviceToHost, st));
-    if (iterations) HIP_TRY(hipMemcpyAsync(iterations + f0, ob + o_iter, (size_t)n * 4, hipMemcpyDeviceToHost, st));
-    if (msg) HIP_TRY(hipMemcpyAsync(msg + f0 * c.K, ob + o_msg, (size_t)n * c.K * 8, hipMemcpyDeviceToHost, st));
-    if (syst) HIP_TRY(hipMemcpyAsync(syst + f0 * c.K, ob + o_syst, (size_t)n * c.K * 8, hipMemcpyDeviceToHost, st));
-    if (parity) HIP_TRY(hipMemcpyAsync(parity + f0 * c.N, ob + o_par, (size_t)n * c.N * 8, hipMemcpyDeviceToHost, st));
-    if (state) HIP_TRY(hipMemcpyAsync(state + f0 * c.E, ob + o_state, (size_t)n * c.E * 8, hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
+    // pinned buffer b is free once chunk k - 2's outputs have been copied out of it
+    if (out_done[b].valid()) {
+      const Status r = out_done[b].get();
+      if (r.first) {
+        g_err = r.second;
+        return bail(r.first);
+      }
+    }
+    char* pb = ws->pin_out[b].as<char>();
+    bool ok = hipStreamWaitEvent(os, ws->decoded[b], 0) == hipSuccess;
+    auto d2h = [&](bool want, size_t off, size_t bytes) {
+      if (want && ok) ok = hipMemcpyAsync(pb + off, ob + off, bytes, hipMemcpyDeviceToHost, os) == hipSuccess;
+    };
+    d2h(msg_bits != nullptr, o_bits, (size_t)n * c.K);
+    d2h(iterations != nullptr, o_iter, (size_t)n * 4);
+    d2h(msg != nullptr, o_msg, (size_t)n * c.K * 4);
+    d2h(syst != nullptr, o_syst, (size_t)n * c.K * 4);
+    d2h(parity != nullptr, o_par, (size_t)n * c.N * 4);
+    d2h(state != nullptr, o_state, (size_t)n * c.E * 4);
+    if (!ok || hipEventRecord(ws->drained[b], os) != hipSuccess) {
+      g_err = "device-to-host copy failed";
+      return bail(FECL_DEVICE_ERROR);
+    }
+    out_done[b] = copy_out(k);
   }
+  const Status r = settle();
+  if (r.first) return fail(r.first, r.second);
   HIP_TRY(hipStreamSynchronize(cs));
+  HIP_TRY(hipStreamSynchronize(os));
+  HIP_TRY(hipStreamSynchronize(st));
   return FECL_OK;
 }
 

@@ -70,8 +70,8 @@ struct HostBuf {
 
 struct Workspace {
   DevBuf msg, L, hard, phase, iter, fail, flags, active, out_hard, out_ext, out_app;
-  DevBuf stage_in[2], stage_syst[2], stage_state[2], stage_out;   // host path: inputs double-buffered
-  HostBuf pin_in[2], pin_syst[2], pin_state[2];                    // host path: FP32 inputs, pinned
+  DevBuf stage_in[2], stage_syst[2], stage_state[2], stage_out[2];   // host path, double-buffered
+  HostBuf pin_in[2], pin_syst[2], pin_state[2], pin_out[2];          // host path: FP32, pinned
   int32_t* host_active = nullptr;
   int host_active_cap = 0;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -79,13 +79,16 @@ struct Workspace {
   std::vector<std::array<hipEvent_t, 4>> pev;   // per-step timing events (profiling only)
   hipStream_t own_stream = nullptr;
   hipStream_t copy_stream = nullptr;                        // host path: input copies
+  hipStream_t out_stream = nullptr;                         // host path: output copies
   hipEvent_t copied[2] = {nullptr, nullptr}, used[2] = {nullptr, nullptr};
+  hipEvent_t decoded[2] = {nullptr, nullptr}, drained[2] = {nullptr, nullptr};
   ~Workspace() {
     for (DevBuf* b : {&msg, &L, &hard, &phase, &iter, &fail, &flags, &active, &out_hard, &out_ext, &out_app,
                       &stage_in[0], &stage_in[1], &stage_syst[0], &stage_syst[1], &stage_state[0], &stage_state[1],
-                      &stage_out})
+                      &stage_out[0], &stage_out[1]})
       b->release();
-    for (HostBuf* b : {&pin_in[0], &pin_in[1], &pin_syst[0], &pin_syst[1], &pin_state[0], &pin_state[1]})
+    for (HostBuf* b : {&pin_in[0], &pin_in[1], &pin_syst[0], &pin_syst[1], &pin_state[0], &pin_state[1],
+                       &pin_out[0], &pin_out[1]})
       b->release();
     if (host_active) (void)hipHostFree(host_active);
     for (auto& e : ev)
@@ -95,7 +98,8 @@ struct Workspace {
       for (auto& e : a) (void)hipEventDestroy(e);
     if (own_stream) (void)hipStreamDestroy(own_stream);
     if (copy_stream) (void)hipStreamDestroy(copy_stream);
-    for (hipEvent_t e : {copied[0], copied[1], used[0], used[1]})
+    if (out_stream) (void)hipStreamDestroy(out_stream);
+    for (hipEvent_t e : {copied[0], copied[1], used[0], used[1], decoded[0], decoded[1], drained[0], drained[1]})
       if (e) (void)hipEventDestroy(e);
   }
 };
