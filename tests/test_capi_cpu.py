@@ -361,3 +361,14 @@ def test_archive_load_rejects_damaged_bytes():
     bad_size[97] = 0x1f            # paritySize_ no longer the matrix's column count
     with pytest.raises(ValueError, match="sizes"):
         F.archive_load(bytes(bad_size))
+
+
+def test_matlab_gateway_host_calls():
+    """The MATLAB gateway (fecl_amd/matlab/fecl_wrap.cpp) through the MEX test double, host-only
+    calls: FeCl's function ids, the DvbS2 matrix as a sparse logical array with the C++ API's
+    pattern, errors raised through mexErrMsgIdAndTxt, no arrays leaked."""
+    import subprocess
+    exe = os.path.join(HERE, "cpp", "mex_gateway_test")
+    assert os.path.exists(exe), "build it with `make -C tests/cpp` (done by __graft_entry__.build())"
+    r = subprocess.run([exe, "cpu"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
