@@ -36,6 +36,7 @@ PMC_TRAFFIC_RATIO = {("f32", 4): {"vn_kernel": 1.0030, "cn_kernel": 1.0096},
                      ("f16", 2): {"vn_kernel": 1.0132, "cn_kernel": 1.0192}}
 
 # Extra Eb/N0 points (dB) of the precision study, across the waterfall of min-sum on DVB-S2 r1/2.
+AGREEMENT_EBN0 = (0.8, 0.9, 1.0, 1.1, 1.2)   # Linear / Exact waterfall of r1/2
 PRECISION_WATERFALL = (1.45, 1.5, 1.55, 1.6, 1.65, 1.7)
 
 
@@ -316,19 +317,10 @@ def run_precision(args, device):
         print(f"[precision] {row}", file=sys.stderr, flush=True)
         del d_llr, llr, cw
 
-    def crossing(col, target=1e-2):
-        pts = sorted((r["ebn0"], r[col]) for r in rows)
-        for (x0, y0), (x1, y1) in zip(pts, pts[1:]):
-            if y0 >= target > y1:
-                if y1 <= 0:
-                    return x1
-                t = (np.log(target) - np.log(y0)) / (np.log(y1) - np.log(y0))
-                return x0 + t * (x1 - x0)
-        return None
-    x64 = crossing("fer_fp64")
+    x64 = fer_crossing(rows, "fer_fp64")
     gaps = {}
     for k in fmts:
-        xk = crossing(f"fer_{k}")
+        xk = fer_crossing(rows, f"fer_{k}")
         gaps[k] = None if (x64 is None or xk is None) else round(xk - x64, 4)
     # a-posteriori LLR error after a few iterations
     err = {}
@@ -348,6 +340,79 @@ def run_precision(args, device):
                         "Eb/N0, FP64 channel LLRs (GPU forms take them rounded to FP32)",
             "points": rows, "fer_1e-2_gap_db_vs_fp64": gaps, "fp64_fer_1e-2_ebn0": x64,
             "app_llr_error_vs_fp64": err}
+
+
+def fer_crossing(rows, col, target=1e-2):
+    """Eb/N0 where the FER column crosses `target`, by log-linear interpolation (None if it
+    does not cross between the points)."""
+    import numpy as np
+    pts = sorted((r["ebn0"], r[col]) for r in rows)
+    for (x0, y0), (x1, y1) in zip(pts, pts[1:]):
+        if y0 >= target > y1:
+            if y1 <= 0:
+                return x1
+            t = (np.log(target) - np.log(y0)) / (np.log(y1) - np.log(y0))
+            return x0 + t * (x1 - x0)
+    return None
+
+
+def run_agreement(args, device):
+    """Linear and Exact FP32 on the GPU against FeCl's FP64 decoder of the same algorithm (the
+    oracle's bit-identical restatement, host cores) on the same FP64 channel LLRs, DVB-S2
+    n=64800 r1/2, 50 max iterations.  Per Eb/N0: both FERs, a two-proportion z statistic for
+    their difference and the frames whose hard decisions differ; the gap of the FER curves at
+    1e-2; the a-posteriori LLR error after 1 and 5 iterations (max relative to max(1, |FP64|))."""
+    import numpy as np
+    import torch
+    import fecl_amd as F
+    from oracle import oracle as O
+    threads = min(16, os.cpu_count() or 1)
+    oc = O.dvbs2(64800, 1, 2)
+    H = F.DvbS2.matrix(64800, 0.5)
+    frames = args.agreement_frames
+    rng = np.random.default_rng(args.seed + 11)
+    out = {"workload": f"DVB-S2 n=64800 r1/2, {args.iterations} max iterations, {frames} frames per Eb/N0, "
+                       "FP64 channel LLRs (the GPU takes them rounded to FP32)"}
+    for name, alg, oalg in (("linear", F.Linear, O.LINEAR), ("exact", F.Exact, O.EXACT)):
+        dec = F.Ldpc(H, F.DecoderOptions().algorithm(alg).iterations(args.iterations))
+        K, N = dec.msgSize(), dec.paritySize()
+        rows = []
+        for e in AGREEMENT_EBN0:
+            info = rng.integers(0, 2, (frames, K), dtype=np.uint8)
+            cw = dec.encode(info).reshape(frames, N)
+            sigma2 = 1.0 / (2.0 * 0.5 * 10.0 ** (e / 10.0))
+            llr = -2.0 * ((1.0 - 2.0 * cw) + rng.normal(0.0, np.sqrt(sigma2), cw.shape)) / sigma2
+            t0 = time.perf_counter()
+            ref = O.decode(oc, llr, oalg, args.iterations, prec=64, want=("hard",), threads=threads)["hard"]
+            cpu_s = time.perf_counter() - t0
+            hard = dec.decode(torch.from_numpy(llr.astype(np.float32)).to(device)).view(frames, K).cpu().numpy()
+            f64 = float(np.mean(np.any(ref != info, axis=1)))
+            f32 = float(np.mean(np.any(hard != info, axis=1)))
+            p = (f64 + f32) / 2
+            z = 0.0 if p in (0.0, 1.0) else (f32 - f64) / np.sqrt(2 * p * (1 - p) / frames)
+            row = {"ebn0": e, "fer_fp64": f64, "fer_gpu_fp32": f32, "z": round(float(z), 3),
+                   "frames_differing": int(np.sum(np.any(hard != ref, axis=1))), "fp64_seconds": round(cpu_s, 2)}
+            rows.append(row)
+            print(f"[agreement] {name} {row}", file=sys.stderr, flush=True)
+        x64, x32 = fer_crossing(rows, "fer_fp64"), fer_crossing(rows, "fer_gpu_fp32")
+        err = {}
+        info = rng.integers(0, 2, (32, K), dtype=np.uint8)
+        cw = dec.encode(info).reshape(32, N)
+        sigma2 = 1.0 / 10.0 ** (1.0 / 10.0)
+        llr = -2.0 * ((1.0 - 2.0 * cw) + rng.normal(0.0, np.sqrt(sigma2), cw.shape)) / sigma2
+        llr = llr.astype(np.float32).astype(np.float64)
+        for iters in (1, 5):
+            ref = O.decode(oc, llr, oalg, iters, prec=64, want=("msg",), threads=threads)["msg"]
+            d = F.Ldpc(H, F.DecoderOptions().algorithm(alg).iterations(iters))
+            app = np.asarray(d.soDecode(llr, outputs=("msg",))["msg"]).reshape(ref.shape)
+            fin = np.isfinite(ref)
+            a = np.abs(app - ref)[fin]
+            err[f"iter{iters}"] = {"max_rel": float((a / np.maximum(1.0, np.abs(ref[fin]))).max()),
+                                   "mean_abs": float(a.mean())}
+        out[name] = {"points": rows, "fer_1e-2_ebn0_fp64": x64, "fer_1e-2_ebn0_gpu": x32,
+                     "gap_db": None if x64 is None or x32 is None else round(x32 - x64, 4),
+                     "app_llr_error_vs_fp64": err}
+    return out
 
 
 def copy_reference(device, gib=1, reps=5):
@@ -462,9 +527,10 @@ def main():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--words", type=int, default=-1,
                     help="lane layout: 1, 2, 4 words per lane per edge; 0 automatic; -1 autotune")
-    ap.add_argument("--config", choices=["batch", "uneven", "serving", "precision", "calibrate", "copy"],
+    ap.add_argument("--config", choices=["batch", "uneven", "serving", "precision", "agreement", "calibrate", "copy"],
                     default="batch")
     ap.add_argument("--precision-frames", type=int, default=4096)
+    ap.add_argument("--agreement-frames", type=int, default=1024)
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--serve-slots", type=int, default=2048)
     ap.add_argument("--offered-fps", type=float, default=200000.0)
@@ -498,6 +564,9 @@ def main():
         return
     if args.config == "precision":
         print(json.dumps(run_precision(args, device)), flush=True)
+        return
+    if args.config == "agreement":
+        print(json.dumps(run_agreement(args, device)), flush=True)
         return
     if args.config == "copy":
         print(json.dumps(copy_reference(device)), flush=True)

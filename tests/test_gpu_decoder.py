@@ -482,3 +482,26 @@ def test_every_dvbs2_code_bit_exact(alg, fmt, ofmt):
             ref = oracle_run(oc, llr, alg, 30, fmt=ofmt, threads=16)
             out = gpu_decoder(H, alg, 30, fmt=fmt).soDecode(llr)
             assert_bit_exact(out, ref, 3, oc.K, oc.N, oc.E)
+
+
+def test_large_batch_across_the_pass_boundary():
+    """8193 frames in one device call run as two passes (8192 + 1 frames): every frame's bits
+    and iteration count equal those of decoding the frames in separate calls (frames share no
+    state), and a sample equals the oracle."""
+    import torch
+    oc, H = dvbs2()
+    base = dvbs2_llr(64, 1.5, seed=77).astype(np.float32)
+    llr = np.concatenate([base] * 128 + [base[:1]])       # 8193 frames
+    assert llr.shape[0] == 8193
+    dec = gpu_decoder(H, F.Approximate, 50, fmt=F.MsgFormat.F16)
+    x = torch.from_numpy(llr).cuda()
+    bits, it = dec.decode(x, iterations=True)
+    b1, i1 = dec.decode(x[:4096], iterations=True)
+    b2, i2 = dec.decode(x[4096:], iterations=True)
+    torch.cuda.synchronize()
+    assert torch.equal(bits, torch.cat([b1, b2])) and torch.equal(it, torch.cat([i1, i2]))
+    K = oc.K
+    hb = bits.view(8193, K).cpu().numpy()
+    assert all(np.array_equal(hb[f], hb[f % 64]) for f in (64, 4095, 4096, 8191, 8192))
+    ref = oracle_run(oc, llr[:8].astype(np.float64), F.Approximate, 50, fmt=O.MSG_F16, want=("hard", "iters"))
+    assert np.array_equal(hb[:8], ref["hard"]) and np.array_equal(it[:8].cpu().numpy(), ref["iters"])
