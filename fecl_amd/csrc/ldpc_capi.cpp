@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <array>
 #include <cmath>
 #include <cstring>
@@ -316,10 +317,15 @@ constexpr int64_t kHostFirstChunkFrames = 256;   // smaller: its conversion over
 constexpr unsigned kConvertThreads = 8;
 
 // Runs body(a, b) over [0, n) split across up to kConvertThreads threads.
+// Host-buffer calls in flight in this process: FeCl's fan-out calls the hooks from
+// workGroupSize threads at once, and the conversion threads are shared out among them.
+std::atomic<int> g_host_calls{0};
+
 template <class Body>
 void parallel_ranges(size_t n, Body body) {
   const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-  const size_t threads = std::max<size_t>(1, std::min<size_t>({kConvertThreads, hw, n / 65536 + 1}));
+  const size_t share = std::max<size_t>(1, kConvertThreads / (size_t)std::max(1, g_host_calls.load()));
+  const size_t threads = std::max<size_t>(1, std::min<size_t>({share, hw, n / 65536 + 1}));
   const size_t per = (n + threads - 1) / threads;
   auto part = [&](size_t t) { body(std::min(n, per * t), std::min(n, per * (t + 1))); };
   std::vector<std::thread> pool;
@@ -353,6 +359,10 @@ int host_decode(fecl_ldpc* h, const double* llr, const double* syst_in, const do
   if (!llr) return fail(FECL_INVALID_ARGUMENT, "Input must contains parity");
   const Code& c = h->code;
   HIP_TRY(hipSetDevice(h->device));
+  struct InFlight {
+    InFlight() { ++g_host_calls; }
+    ~InFlight() { --g_host_calls; }
+  } in_flight;
   std::unique_ptr<Workspace> ws;
   if (int rc = acquire(h, &ws)) return rc;
   struct Back { fecl_ldpc* h; std::unique_ptr<Workspace>* ws; ~Back() { release(h, std::move(*ws)); } } back{h, &ws};
