@@ -1,5 +1,6 @@
 #include "ldpc_archive.h"
 
+#include <climits>
 #include <cstring>
 #include <stdexcept>
 
@@ -73,8 +74,11 @@ void put_matrix(Writer& w, const Csr& m, int64_t cols, bool first) {
 
 Csr get_matrix(Reader& r, int64_t* cols, bool first, bool* rowidx_seen) {
   if (first) r.class_info("SparseBitMatrix version");
-  *cols = (int64_t)r.get<uint64_t>();
+  const uint64_t c = r.get<uint64_t>();
+  if (c > (uint64_t)INT32_MAX) throw std::runtime_error("matrix too large for this library");
+  *cols = (int64_t)c;
   const uint64_t nnz = r.count(8);
+  if (nnz > (uint64_t)INT32_MAX) throw std::runtime_error("matrix too large for this library");
   std::vector<uint64_t> el(nnz);
   for (auto& e : el) e = r.get<uint64_t>();
   if (first) r.class_info("row index version");

@@ -361,6 +361,17 @@ def test_archive_load_rejects_damaged_bytes():
     bad_size[97] = 0x1f            # paritySize_ no longer the matrix's column count
     with pytest.raises(ValueError, match="sizes"):
         F.archive_load(bytes(bad_size))
+    # random byte changes: each archive is either rejected or read into a consistent structure
+    rng = np.random.default_rng(8)
+    for _ in range(1500):
+        d = bytearray(data)
+        for _ in range(int(rng.integers(1, 4))):
+            d[int(rng.integers(0, len(d)))] = int(rng.integers(0, 256))
+        try:
+            S, K, opts, _ = F.archive_load(bytes(d))
+        except ValueError:
+            continue
+        assert S.rows() < S.cols() and S.cols() - S.rows() <= K < S.cols() and S.size() == S.row_ptr[-1]
 
 
 def test_matlab_gateway_host_calls():
