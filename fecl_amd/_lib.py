@@ -73,7 +73,7 @@ def lib():
     global _LIB
     if _LIB is None:
         if not os.path.exists(LIB_PATH):
-            raise RuntimeError(f"{LIB_PATH} is not built: run __graft_entry__.build() or "
+            raise RuntimeError(f"{LIB_PATH} is not built: run "
                                "`make -C fecl_amd/csrc`")
         L = ctypes.CDLL(LIB_PATH)
         for name, args in SIGNATURES.items():

@@ -380,7 +380,7 @@ def test_matlab_gateway_host_calls():
     pattern, errors raised through mexErrMsgIdAndTxt, no arrays leaked."""
     import subprocess
     exe = os.path.join(HERE, "cpp", "mex_gateway_test")
-    assert os.path.exists(exe), "build it with `make -C tests/cpp` (done by __graft_entry__.build())"
+    assert os.path.exists(exe), "build it with `make -C tests/cpp`"
     r = subprocess.run([exe, "cpu"], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
 

@@ -12,7 +12,7 @@ EXE = os.path.join(HERE, "cpp", "facade_test")
 
 @pytest.mark.gpu
 def test_cpp_facade_reference_cases():
-    assert os.path.exists(EXE), "build it with `make -C tests/cpp` (done by __graft_entry__.build())"
+    assert os.path.exists(EXE), "build it with `make -C tests/cpp`"
     r = subprocess.run([EXE], capture_output=True, text=True, timeout=300)
     print(r.stdout, r.stderr)
     assert r.returncode == 0, r.stdout + r.stderr
@@ -24,7 +24,7 @@ def test_matlab_gateway_through_the_mex_test_double():
     DvbS2 matrix and from a dense Gallager matrix, sizes, encode/check/decode/soDecode against
     the C++ API, decoder options, work group size, puncturing, save/load, destroy."""
     exe = os.path.join(HERE, "cpp", "mex_gateway_test")
-    assert os.path.exists(exe), "build it with `make -C tests/cpp` (done by __graft_entry__.build())"
+    assert os.path.exists(exe), "build it with `make -C tests/cpp`"
     r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
     print(r.stdout, r.stderr)
     assert r.returncode == 0, r.stdout + r.stderr
