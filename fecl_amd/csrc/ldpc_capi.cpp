@@ -134,7 +134,9 @@ DevOpts make_opts(const fecl_ldpc* h) {
   float y[8];
   for (int i = 0; i < 8; ++i) y[i] = (float)std::log(1.0 + std::exp(-((double)(float)i / (double)2.0f)));
   for (int i = 0; i < 8; ++i) {
-    o.lin_y[i] = y[i];
+    // knot 7 enters only through the slope of segment 6: from x = 7 on FeCl's f is 0, which
+    // the kernels read as the (0, 0) entry 7
+    o.lin_y[i] = i < 7 ? y[i] : 0.0f;
     o.lin_s[i] = i < 7 ? y[i + 1] - y[i] : 0.0f;
   }
   return o;

@@ -69,8 +69,8 @@ struct DevOpts {
   int32_t algorithm;
   int32_t fused_syn;       // syndrome folded into the check kernel (scaling rows == 1)
   int32_t nt_rows;         // non-temporal message rows: bit 0 check kernel, bit 1 variable kernel
-  float lin_y[8];          // Linear table knots y[i] = log(1 + exp(-i/2)) (float)
-  float lin_s[8];          // slopes y[i+1] - y[i] (float)
+  float lin_y[8];          // Linear table knots y[i] = log(1 + exp(-i/2)) (float), entry 7 = 0
+  float lin_s[8];          // slopes y[i+1] - y[i] (float), entry 7 = 0
 };
 
 // Outputs of a finished batch, in tile layout (device buffers owned by the engine).

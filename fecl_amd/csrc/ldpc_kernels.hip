@@ -483,27 +483,27 @@ __global__ __launch_bounds__(kBlock) void cn_minsum_raw_kernel(DevCode c, DevSta
 //   Exact : the same form with f(z) = log(1 + e^-z), i.e. the exact box-plus
 //           -2 atanh(tanh(-a/2) tanh(-b/2)) in a form that stays finite in FP32.
 // The Linear table's knots y[q] and slopes s[q] sit in LDS (filled once per workgroup), so a
-// lookup is one ds_read_b64 instead of a chain of selects.
+// lookup is one ds_read_b64 instead of a chain of selects.  Entry 7 is (0, 0): FeCl's f is 0
+// from x = 7 on, so clamping x to 7.5 replaces the test; x - i is fract(x), exact for
+// 0 <= x < 8, and the product and sum round separately as FeCl's (y[i+1]-y[i])*(x-i)+y[i].
 __device__ static inline float lin_f(const float2* tab, float z) {
-  const float x = z * 2.0f;
-  const int i = min((int)x, 7);   // x >= 7 gives 0 below; the clamp keeps the index in the table
-  const float2 ys = tab[i];
-  const float dx = x - (float)i;
-  const float prod = ys.y * dx;
-  return x >= 7.0f ? 0.0f : prod + ys.x;
+  const float x = fminf(z * 2.0f, 7.5f);
+  const float2 ys = tab[(int)x];
+  const float prod = ys.y * __builtin_amdgcn_fractf(x);
+  return prod + ys.x;
 }
 __device__ static inline float exact_f(float z) { return __logf(1.0f + __expf(-z)); }
 
 template <int ALG>
 __device__ static inline float box_sum(const float2* tab, float a, float b) {
-  const float fa = fabsf(a), fb = fabsf(b);
-  const float m = (fb < fa) ? fb : fa;
+  const float m = fminf(fabsf(a), fabsf(b));
   const float s1 = fabsf(a + b), s2 = fabsf(a - b);
   float f1, f2;
   if (ALG == 1) { f1 = lin_f(tab, s1); f2 = lin_f(tab, s2); }
   else { f1 = exact_f(s1); f2 = exact_f(s2); }
-  const bool diff = (__float_as_uint(a) ^ __float_as_uint(b)) >> 31;
-  const float t = (diff ? m : -m) - f1;
+  // +m where the signs differ, -m where they agree (m's own sign bit is clear)
+  const uint32_t same = ~(__float_as_uint(a) ^ __float_as_uint(b)) & 0x80000000u;
+  const float t = __uint_as_float(__float_as_uint(m) | same) - f1;
   return t + f2;
 }
 
