@@ -477,11 +477,31 @@ __global__ __launch_bounds__(kBlock) void cn_minsum_raw_kernel(DevCode c, DevSta
 }
 
 // ------------------------------------------------------------------------------------------
-// Pairwise box-sums for the forward/backward check update (FeCl sign convention: the result
-// has magnitude ~min(|a|,|b|) and sign -sign(a)*sign(b)).
-//   Linear: LinearBoxSum, min - f(|a+b|) + f(|a-b|) with FeCl's 8-knot table f.
-//   Exact : the same form with f(z) = log(1 + e^-z), i.e. the exact box-plus
-//           -2 atanh(tanh(-a/2) tanh(-b/2)) in a form that stays finite in FP32.
+// Exact (BoxSum, src/detail/LlrMetrics.h:126-140): prior p = tanh(-x/2), product of the other
+// inputs' priors, post -log((1+p)/(1-p)).  Evaluated on the complement q = 1 - |p| with the
+// sign bits taken apart:
+//   q = 1 - tanh(|x|/2) = 2u/(1+u), u = e^-|x|;   1 - q = (1-q_a)(1-q_b)  =>  q = q_a + q_b (1-q_a);
+//   |out| = log((1+|p|)/(1-|p|)) = log((2-q)/q).
+// This is BoxSum's value without the cancellation of 1-p as |p| -> 1 and without tanh's
+// saturation, and it costs two transcendentals per input and two per output (a pairwise
+// log-domain box-sum needs four each).  |x| is clamped to 87 so that u stays a normal number:
+// that changes q by less than 2e-38, which moves no output unless every other input of the row
+// is as large, and such an output is still at least 87 - log(d-1).  Every q stays in [0, 1],
+// so (2-q)/q >= 1 up to rounding; the sign bit is cleared before the output's sign is set.
+__device__ static inline float exact_prior(float x) {
+  const float u = __expf(-fminf(fabsf(x), 87.0f));
+  return (2.0f * u) * __builtin_amdgcn_rcpf(1.0f + u);
+}
+__device__ static inline float exact_comb(float a, float b) { return a + b * (1.0f - a); }
+__device__ static inline float exact_post(float q, uint32_t sign) {
+  const float m = __logf((2.0f - q) * __builtin_amdgcn_rcpf(q));
+  return __uint_as_float((__float_as_uint(m) & 0x7fffffffu) | (sign & 0x80000000u));
+}
+
+// ------------------------------------------------------------------------------------------
+// Pairwise box-sums for the Linear check update (LinearBoxSum: FeCl sign convention, the
+// result has magnitude ~min(|a|,|b|) and sign -sign(a)*sign(b)): min - f(|a+b|) + f(|a-b|)
+// with FeCl's 8-knot table f.
 // The Linear table's knots y[q] and slopes s[q] sit in LDS (filled once per workgroup), so a
 // lookup is one ds_read_b64 instead of a chain of selects.  Entry 7 is (0, 0): FeCl's f is 0
 // from x = 7 on, so clamping x to 7.5 replaces the test; x - i is fract(x), exact for
@@ -492,15 +512,12 @@ __device__ static inline float lin_f(const float2* tab, float z) {
   const float prod = ys.y * __builtin_amdgcn_fractf(x);
   return prod + ys.x;
 }
-__device__ static inline float exact_f(float z) { return __logf(1.0f + __expf(-z)); }
-
 template <int ALG>
 __device__ static inline float box_sum(const float2* tab, float a, float b) {
+  static_assert(ALG == 1, "pairwise box-sums serve the Linear rule only");
   const float m = fminf(fabsf(a), fabsf(b));
   const float s1 = fabsf(a + b), s2 = fabsf(a - b);
-  float f1, f2;
-  if (ALG == 1) { f1 = lin_f(tab, s1); f2 = lin_f(tab, s2); }
-  else { f1 = exact_f(s1); f2 = exact_f(s2); }
+  const float f1 = lin_f(tab, s1), f2 = lin_f(tab, s2);
   // +m where the signs differ, -m where they agree (m's own sign bit is clear)
   const uint32_t same = ~(__float_as_uint(a) ^ __float_as_uint(b)) & 0x80000000u;
   const float t = __uint_as_float(__float_as_uint(m) | same) - f1;
@@ -509,7 +526,9 @@ __device__ static inline float box_sum(const float2* tab, float a, float b) {
 
 // Check-node update for Linear (ALG 1) and Exact (ALG 2 here) in FeCl's forward/backward
 // order: prefix P_j = ((x_0 # x_1) # ...) # x_{j-1}; suffix from x_{d-1} downwards;
-// out_{d-1} = P_{d-1}, out_j = P_j # S_j, out_0 = S_0, each times the scaling factor.
+// out_{d-1} = P_{d-1}, out_j = P_j # S_j, out_0 = S_0, each times the scaling factor.  Linear
+// combines LLRs pairwise; Exact combines the complements q of its priors the same way and
+// applies the post to each output.
 template <class F, int DMAX, int ALG>
 __global__ __launch_bounds__(kBlock) void cn_box_kernel(DevCode c, DevState s, DevOpts o) {
   constexpr int FPL = F::FPL, TW = F::TW, RW = F::RW;
@@ -560,18 +579,6 @@ __global__ __launch_bounds__(kBlock) void cn_box_kernel(DevCode c, DevState s, D
     }
   }
   if (!ln.lane_any) return;
-  // forward prefixes: pre[j] for j = 1 .. d-2, P = prefix over 0 .. d-2
-  float pre[DMAX][FPL];
-  float P[FPL];
-#pragma unroll
-  for (int k = 0; k < FPL; ++k) P[k] = x[0][k];
-#pragma unroll
-  for (int j = 1; j < DMAX - 1; ++j) {
-    if (j < d - 1) {
-#pragma unroll
-      for (int k = 0; k < FPL; ++k) { pre[j][k] = P[k]; P[k] = box_sum<ALG>(tab, P[k], x[j][k]); }
-    }
-  }
   auto store = [&](int j, float (&y)[FPL]) {
 #pragma unroll
     for (int k = 0; k < FPL; ++k) {
@@ -580,41 +587,114 @@ __global__ __launch_bounds__(kBlock) void cn_box_kernel(DevCode c, DevState s, D
     }
     F::store(mrow + (int64_t)j * RW, y);
   };
-  // out_{d-1} = P
+  float pre[DMAX][FPL];
+  float P[FPL];
+  if constexpr (ALG == 2) {
+    // q of every input, and the sign of the product of all priors (each prior has the sign
+    // of -x; the post flips it again): out_j's sign bit is (d & 1) ^ sx ^ sign(x_j)
+    float q[DMAX][FPL];
+    uint32_t sx[FPL];
 #pragma unroll
-  for (int j = DMAX - 1; j >= 1; --j) {
-    if (j == d - 1) {
-      float y[FPL];
+    for (int k = 0; k < FPL; ++k) sx[k] = (d & 1) ? 0x80000000u : 0u;
 #pragma unroll
-      for (int k = 0; k < FPL; ++k) y[k] = P[k];
-      store(j, y);
-    }
-  }
-  float S[FPL];
+    for (int j = 0; j < DMAX; ++j) {
+      if (j < d) {
 #pragma unroll
-  for (int j = DMAX - 1; j >= 1; --j) {
-    if (j == d - 1) {
-#pragma unroll
-      for (int k = 0; k < FPL; ++k) S[k] = x[j][k];
-    }
-  }
-#pragma unroll
-  for (int j = DMAX - 2; j >= 1; --j) {
-    if (j <= d - 2) {
-      float y[FPL];
-#pragma unroll
-      for (int k = 0; k < FPL; ++k) {
-        y[k] = box_sum<ALG>(tab, pre[j][k], S[k]);
-        S[k] = box_sum<ALG>(tab, S[k], x[j][k]);
+        for (int k = 0; k < FPL; ++k) {
+          q[j][k] = exact_prior(x[j][k]);
+          sx[k] ^= __float_as_uint(x[j][k]);
+        }
       }
-      store(j, y);
     }
-  }
-  {
+#pragma unroll
+    for (int k = 0; k < FPL; ++k) P[k] = q[0][k];
+#pragma unroll
+    for (int j = 1; j < DMAX - 1; ++j) {
+      if (j < d - 1) {
+#pragma unroll
+        for (int k = 0; k < FPL; ++k) { pre[j][k] = P[k]; P[k] = exact_comb(P[k], q[j][k]); }
+      }
+    }
+#pragma unroll
+    for (int j = DMAX - 1; j >= 1; --j) {
+      if (j == d - 1) {
+        float y[FPL];
+#pragma unroll
+        for (int k = 0; k < FPL; ++k) y[k] = exact_post(P[k], sx[k] ^ __float_as_uint(x[j][k]));
+        store(j, y);
+      }
+    }
+    float S[FPL];
+#pragma unroll
+    for (int j = DMAX - 1; j >= 1; --j) {
+      if (j == d - 1) {
+#pragma unroll
+        for (int k = 0; k < FPL; ++k) S[k] = q[j][k];
+      }
+    }
+#pragma unroll
+    for (int j = DMAX - 2; j >= 1; --j) {
+      if (j <= d - 2) {
+        float y[FPL];
+#pragma unroll
+        for (int k = 0; k < FPL; ++k) {
+          y[k] = exact_post(exact_comb(pre[j][k], S[k]), sx[k] ^ __float_as_uint(x[j][k]));
+          S[k] = exact_comb(S[k], q[j][k]);
+        }
+        store(j, y);
+      }
+    }
     float y[FPL];
 #pragma unroll
-    for (int k = 0; k < FPL; ++k) y[k] = S[k];
+    for (int k = 0; k < FPL; ++k) y[k] = exact_post(S[k], sx[k] ^ __float_as_uint(x[0][k]));
     store(0, y);
+  } else {
+    // forward prefixes: pre[j] for j = 1 .. d-2, P = prefix over 0 .. d-2
+#pragma unroll
+    for (int k = 0; k < FPL; ++k) P[k] = x[0][k];
+#pragma unroll
+    for (int j = 1; j < DMAX - 1; ++j) {
+      if (j < d - 1) {
+#pragma unroll
+        for (int k = 0; k < FPL; ++k) { pre[j][k] = P[k]; P[k] = box_sum<ALG>(tab, P[k], x[j][k]); }
+      }
+    }
+    // out_{d-1} = P
+#pragma unroll
+    for (int j = DMAX - 1; j >= 1; --j) {
+      if (j == d - 1) {
+        float y[FPL];
+#pragma unroll
+        for (int k = 0; k < FPL; ++k) y[k] = P[k];
+        store(j, y);
+      }
+    }
+    float S[FPL];
+#pragma unroll
+    for (int j = DMAX - 1; j >= 1; --j) {
+      if (j == d - 1) {
+#pragma unroll
+        for (int k = 0; k < FPL; ++k) S[k] = x[j][k];
+      }
+    }
+#pragma unroll
+    for (int j = DMAX - 2; j >= 1; --j) {
+      if (j <= d - 2) {
+        float y[FPL];
+#pragma unroll
+        for (int k = 0; k < FPL; ++k) {
+          y[k] = box_sum<ALG>(tab, pre[j][k], S[k]);
+          S[k] = box_sum<ALG>(tab, S[k], x[j][k]);
+        }
+        store(j, y);
+      }
+    }
+    {
+      float y[FPL];
+#pragma unroll
+      for (int k = 0; k < FPL; ++k) y[k] = S[k];
+      store(0, y);
+    }
   }
 }
 

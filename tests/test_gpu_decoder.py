@@ -6,7 +6,8 @@ Bar, per path:
     a-posteriori/extrinsic values, state and iteration counts.
   * min-sum / Linear with FP16 or BF16 message storage: bit-identical to the oracle's FP32
     restatement with the same storage rounding.
-  * Exact FP32 (stable box-plus, no tanh saturation) against FeCl's FP64 decoder: identical
+  * Exact FP32 (BoxSum's tanh product carried as q = 1 - |tanh(x/2)|, which neither cancels
+    nor saturates) against FeCl's FP64 decoder: identical
     hard decisions and iteration counts on these inputs, a-posteriori LLRs within 1e-4
     relative (|d| <= 1e-4 * max(1, |ref|)) where FeCl's value is finite.
 """
@@ -202,7 +203,8 @@ def test_exact_fp32_against_fp64(code):
     """Exact: FeCl evaluates BoxSum as -log((1+p)/(1-p)) with p a product of tanh(-x/2)
     (src/detail/LlrMetrics.h:126-140).  In FP64 that form loses relative accuracy as |p| -> 1
     (1-p cancels), so FeCl's own APP deviates from the exact box-plus value by up to ~4e-4 at
-    |APP| ~ 60.  The GPU evaluates the same value stably in FP32.  Bars:
+    |APP| ~ 60.  The GPU carries the complement q = 1 - |p| (product 1-q = (1-q_a)(1-q_b),
+    post log((2-q)/q)), which evaluates the same value without the cancellation.  Bars:
       * hard decisions and iteration counts identical to FeCl FP64;
       * APP within 2e-6 relative of the exact value (the oracle's FP64 stable evaluation);
       * APP within 1e-4 relative of FeCl FP64 plus FeCl's own distance from the exact value."""
