@@ -198,36 +198,65 @@ def test_check_kernel_forms_on_high_degree_codes(rate, words, sf, fmt, ofmt):
     assert_bit_exact(dec.soDecode(llr), ref, 4, oc.K, oc.N, oc.E)
 
 
+# Eb/N0 (dB) near each normal-frame rate's waterfall, so that some frames stop early and some
+# run out their iterations; short frames get 0.4 dB more
+_WATERFALL = {(1, 4): 1.1, (1, 3): 1.3, (2, 5): 1.5, (1, 2): 1.7, (3, 5): 2.4, (2, 3): 2.3, (3, 4): 2.6,
+              (4, 5): 2.9, (5, 6): 3.2, (8, 9): 3.8, (9, 10): 4.1}
+
+
+def assert_exact_close(oc, llr, H, iters, sf, bar):
+    """Exact FP32 on the GPU against FeCl FP64 (`ref`) and the exact box-plus value (`exact`,
+    the oracle's FP64 stable evaluation): identical hard decisions and iteration counts; APP
+    within `bar` relative (to max(1, |exact|)) of the exact value, and within 1e-4 relative of
+    FeCl FP64 plus FeCl's own distance from the exact value where FeCl's value is finite; the
+    same signs where FeCl saturates to +-inf."""
+    ref = oracle_run(oc, llr, F.Exact, iters, sf=sf, prec=64, threads=16)
+    exact = oracle_run(oc, llr, O.EXACT_STABLE, iters, sf=sf, prec=64, threads=16)
+    out = gpu_decoder(H, F.Exact, iters, sf=sf).soDecode(llr)
+    hard = gpu_decoder(H, F.Exact, iters, sf=sf).decode(llr)
+    assert np.array_equal(hard.reshape(-1, oc.K), ref["hard"])
+    assert np.array_equal(out["iterations"], ref["iters"])
+    r, x = ref["msg"].ravel(), exact["msg"].ravel()
+    g = out["msg"].astype(np.float64).ravel()
+    scale = np.maximum(1.0, np.abs(x))
+    assert (np.abs(g - x) / scale).max() <= bar
+    fin = np.isfinite(r)
+    own = np.abs(r[fin] - x[fin])
+    assert np.all(np.abs(g[fin] - r[fin]) <= own + 1e-4 * np.maximum(1.0, np.abs(r[fin])))
+    assert np.all(np.sign(g[~fin]) == np.sign(r[~fin]))
+
+
 @pytest.mark.parametrize("code", ["gallager", "dvbs2"])
 def test_exact_fp32_against_fp64(code):
     """Exact: FeCl evaluates BoxSum as -log((1+p)/(1-p)) with p a product of tanh(-x/2)
     (src/detail/LlrMetrics.h:126-140).  In FP64 that form loses relative accuracy as |p| -> 1
     (1-p cancels), so FeCl's own APP deviates from the exact box-plus value by up to ~4e-4 at
     |APP| ~ 60.  The GPU carries the complement q = 1 - |p| (product 1-q = (1-q_a)(1-q_b),
-    post log((2-q)/q)), which evaluates the same value without the cancellation.  Bars:
-      * hard decisions and iteration counts identical to FeCl FP64;
-      * APP within 2e-6 relative of the exact value (the oracle's FP64 stable evaluation);
-      * APP within 1e-4 relative of FeCl FP64 plus FeCl's own distance from the exact value."""
+    post log((2-q)/q)), which evaluates the same value without the cancellation.  Bars: hard
+    decisions and iteration counts identical to FeCl FP64; APP within 2e-6 relative of the
+    exact value, and within 1e-4 relative of FeCl FP64 plus FeCl's own distance from it."""
     oc, H = gallager() if code == "gallager" else dvbs2()
     if code == "gallager":
         llr = gallager_llr(oc.N, oc.K, 4, 2.0, 11)
     else:
         llr = dvbs2_llr(3, 1.0, seed=21)
-    llr = llr.astype(np.float32).astype(np.float64)
-    ref = oracle_run(oc, llr, F.Exact, 50, prec=64)
-    exact = oracle_run(oc, llr, O.EXACT_STABLE, 50, prec=64)
-    out = gpu_decoder(H, F.Exact, 50).soDecode(llr)
-    hard = gpu_decoder(H, F.Exact, 50).decode(llr)
-    assert np.array_equal(hard.reshape(-1, oc.K), ref["hard"])
-    assert np.array_equal(out["iterations"], ref["iters"])
-    r, x = ref["msg"].ravel(), exact["msg"].ravel()
-    g = out["msg"].astype(np.float64)
-    scale = np.maximum(1.0, np.abs(x))
-    assert (np.abs(g - x) / scale).max() <= 2e-6
-    fin = np.isfinite(r)
-    own = np.abs(r[fin] - x[fin])
-    assert np.all(np.abs(g[fin] - r[fin]) <= own + 1e-4 * np.maximum(1.0, np.abs(r[fin])))
-    assert np.all(np.sign(g[~fin]) == np.sign(r[~fin]))
+    assert_exact_close(oc, llr.astype(np.float32).astype(np.float64), H, 50, 1.0, 2e-6)
+
+
+@pytest.mark.parametrize("sf", [1.0, 0.75])
+def test_every_dvbs2_code_exact_against_fp64(sf):
+    """Exact FP32 on all 21 DVB-S2 codes, 3 frames each near the code's waterfall (the inputs
+    of test_every_dvbs2_code_bit_exact), 30 iterations, with and without a scaling factor:
+    hard decisions and iteration counts identical to FeCl FP64, APP within 1e-5 relative of
+    the exact box-plus value."""
+    from fecl_amd.ldpc import _DVBS2_RATES
+    for n, rates in sorted(_DVBS2_RATES.items()):
+        for rate in rates:
+            oc, H = dvbs2(n, rate)
+            ebn0 = _WATERFALL[rate] + (0.4 if n == 16200 else 0.0)
+            llr = dvbs2_llr(3, ebn0, seed=31 + rate[0] * 7 + rate[1], rate=rate, n=n)
+            print(f"code n={n} rate={rate[0]}/{rate[1]} Eb/N0={ebn0} sf={sf}", flush=True)
+            assert_exact_close(oc, llr.astype(np.float32).astype(np.float64), H, 30, sf, 1e-5)
 
 
 @pytest.mark.parametrize("words", [1, 4])
@@ -458,12 +487,6 @@ def test_decoder_from_a_structure_given_in_form():
     d1, d2 = F.Ldpc(D, opts), F.Ldpc.fromStructure(D, 16200 - D.rows(), opts)
     msg = np.random.default_rng(2).integers(0, 2, size=2 * d1.msgSize(), dtype=np.uint8)
     assert np.array_equal(d1.encode(msg), d2.encode(msg)) and d1.save() == d2.save()
-
-
-# Eb/N0 (dB) near each normal-frame rate's waterfall, so that some frames stop early and some
-# run out their iterations; short frames get 0.4 dB more
-_WATERFALL = {(1, 4): 1.1, (1, 3): 1.3, (2, 5): 1.5, (1, 2): 1.7, (3, 5): 2.4, (2, 3): 2.3, (3, 4): 2.6,
-              (4, 5): 2.9, (5, 6): 3.2, (8, 9): 3.8, (9, 10): 4.1}
 
 
 @pytest.mark.parametrize("alg,fmt,ofmt", [(F.Approximate, F.MsgFormat.F32, O.MSG_F32),
