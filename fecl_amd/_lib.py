@@ -44,6 +44,7 @@ SIGNATURES = {
     "fecl_ldpc_get_layout": [P, P, P],
     "fecl_ldpc_set_fused_syndrome": [P, I32],
     "fecl_ldpc_set_pass_frames": [P, I64],
+    "fecl_ldpc_set_compaction": [P, I32],
     "fecl_ldpc_set_cache_policy": [P, I32],
     "fecl_ldpc_set_vars_per_wave": [P, I32],
     "fecl_ldpc_get_vars_per_wave": [P, P],

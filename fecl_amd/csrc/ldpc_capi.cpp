@@ -174,41 +174,68 @@ Layout layout_of(const fecl_ldpc* h) {
   return lay;
 }
 
-// Sizes the workspace for `frames` frames and returns the device views of it.
-int prepare(fecl_ldpc* h, Workspace* ws, const Layout& lay, int64_t frames, bool soft, DevState* sp, DevOut* op) {
-  const Code& c = h->code;
+namespace {
+
+// The per-tile buffers of area a of a workspace (0: the first region's, 1: the spare).
+struct AreaRefs {
+  DevBuf *msg, *L, *hard, *phase, *iter, *fail, *flags, *out_hard, *out_ext, *out_app;
+};
+AreaRefs area_of(Workspace* ws, int a) {
+  if (a == 0)
+    return {&ws->msg, &ws->L, &ws->hard, &ws->phase, &ws->iter, &ws->fail, &ws->flags, &ws->out_hard, &ws->out_ext,
+            &ws->out_app};
+  Area& x = ws->spare;
+  return {&x.msg, &x.L, &x.hard, &x.phase, &x.iter, &x.fail, &x.flags, &x.out_hard, &x.out_ext, &x.out_app};
+}
+
+// Sizes an area for `tiles` tiles.
+int size_area(const Code& c, const Layout& lay, const AreaRefs& b, int64_t tiles, bool soft) {
   const int FPL = lay.fpl(), TW = lay.tw();
-  const int64_t tiles = (frames + TW - 1) / TW;
-  HIP_TRY(ws->msg.ensure((size_t)tiles * c.E * lay.rw() * 4));
-  HIP_TRY(ws->L.ensure((size_t)tiles * c.N * TW * 4));
-  HIP_TRY(ws->hard.ensure((size_t)tiles * c.N * FPL * 8));
-  HIP_TRY(ws->phase.ensure((size_t)tiles * TW));
-  HIP_TRY(ws->iter.ensure((size_t)tiles * TW * 4));
-  HIP_TRY(ws->fail.ensure((size_t)tiles * FPL * 8));
-  HIP_TRY(ws->flags.ensure((size_t)tiles * 8));
-  HIP_TRY(ws->active.ensure((size_t)h->iterations * 4));
-  HIP_TRY(ws->out_hard.ensure((size_t)tiles * c.K * FPL * 8));
+  HIP_TRY(b.msg->ensure((size_t)tiles * c.E * lay.rw() * 4));
+  HIP_TRY(b.L->ensure((size_t)tiles * c.N * TW * 4));
+  HIP_TRY(b.hard->ensure((size_t)tiles * c.N * FPL * 8));
+  HIP_TRY(b.phase->ensure((size_t)tiles * TW));
+  HIP_TRY(b.iter->ensure((size_t)tiles * TW * 4));
+  HIP_TRY(b.fail->ensure((size_t)tiles * FPL * 8));
+  HIP_TRY(b.flags->ensure((size_t)tiles * 8));
+  HIP_TRY(b.out_hard->ensure((size_t)tiles * c.K * FPL * 8));
   if (soft) {
-    HIP_TRY(ws->out_ext.ensure((size_t)tiles * c.N * TW * 4));
-    HIP_TRY(ws->out_app.ensure((size_t)tiles * c.K * TW * 4));
+    HIP_TRY(b.out_ext->ensure((size_t)tiles * c.N * TW * 4));
+    HIP_TRY(b.out_app->ensure((size_t)tiles * c.K * TW * 4));
   }
+  return FECL_OK;
+}
+
+// Device views of a region of `tiles` tiles at the start of an area sized for at least that many.
+void view_area(const AreaRefs& b, int64_t tiles, int32_t* active, bool soft, DevState* sp, DevOut* op) {
   DevState& s = *sp;
   s = DevState{};
-  s.msg = ws->msg.as<uint32_t>();
-  s.L = ws->L.as<float>();
-  s.hard = ws->hard.as<uint64_t>();
-  s.phase = ws->phase.as<uint8_t>();
-  s.iter = ws->iter.as<int32_t>();
-  s.fail = ws->fail.as<uint64_t>();
-  s.tile_flags = ws->flags.as<uint32_t>();
-  s.tile_iter = ws->flags.as<int32_t>() + tiles;
-  s.active = ws->active.as<int32_t>();
+  s.msg = b.msg->as<uint32_t>();
+  s.L = b.L->as<float>();
+  s.hard = b.hard->as<uint64_t>();
+  s.phase = b.phase->as<uint8_t>();
+  s.iter = b.iter->as<int32_t>();
+  s.fail = b.fail->as<uint64_t>();
+  s.tile_flags = b.flags->as<uint32_t>();
+  s.tile_iter = b.flags->as<int32_t>() + tiles;
+  s.active = active;
   s.tiles = (int32_t)tiles;
   DevOut& out = *op;
   out = DevOut{};
-  out.hard = ws->out_hard.as<uint64_t>();
-  out.ext = soft ? ws->out_ext.as<float>() : nullptr;
-  out.app = soft ? ws->out_app.as<float>() : nullptr;
+  out.hard = b.out_hard->as<uint64_t>();
+  out.ext = soft ? b.out_ext->as<float>() : nullptr;
+  out.app = soft ? b.out_app->as<float>() : nullptr;
+}
+
+}  // namespace
+
+// Sizes the workspace for `frames` frames and returns the device views of it.
+int prepare(fecl_ldpc* h, Workspace* ws, const Layout& lay, int64_t frames, bool soft, DevState* sp, DevOut* op) {
+  const int64_t tiles = (frames + lay.tw() - 1) / lay.tw();
+  const AreaRefs a = area_of(ws, 0);
+  if (int rc = size_area(h->code, lay, a, tiles, soft)) return rc;
+  HIP_TRY(ws->active.ensure((size_t)h->iterations * 4));
+  view_area(a, tiles, ws->active.as<int32_t>(), soft, sp, op);
   return FECL_OK;
 }
 
@@ -240,6 +267,22 @@ int run_batch(fecl_ldpc* h, Workspace* ws, hipStream_t st, const BatchIO& io, in
     }
   }
   int steps_run = 0;
+  // Compaction (eager decodes): once the live frames two steps back fit in three quarters of
+  // the current region's tiles or fewer, and the tile-steps this frees outweigh moving them,
+  // they move into the other area and later steps run on that region only.  The region left
+  // behind hands its finished frames' results to the caller first.  Regions shrink by a
+  // quarter or more each time, so area 1 needs three quarters of the first region's tiles.
+  const int TW = lay.tw();
+  const bool compact = h->compaction && !capture && !o.fused_syn && s.tiles >= 2;
+  if (compact) {
+    const int64_t spare = (3 * (int64_t)s.tiles + 3) / 4;
+    if (int rc = size_area(c, lay, area_of(ws, 1), spare, soft)) return rc;
+    for (DevBuf* b : {&ws->map[0], &ws->map[1], &ws->src_of}) HIP_TRY(b->ensure((size_t)spare * TW * 4));
+  }
+  DevState cur = s;
+  DevOut cur_out = out;
+  int cur_area = 0;
+  const int32_t* cur_map = nullptr;
 
   HIP_TRY(kernels_fill_i32(s.active, h->iterations, 0, st));
   uint8_t init = (uint8_t)((h->iterations == 1 ? PH_FINAL : PH_RUN) | (io.state ? PH_VNINIT : PH_FRESH));
@@ -250,11 +293,11 @@ int run_batch(fecl_ldpc* h, Workspace* ws, hipStream_t st, const BatchIO& io, in
   }
   for (int step = 0; step < h->iterations; ++step) {
     if (prof) HIP_TRY(hipEventRecord(ws->pev[step][0], st));
-    HIP_TRY(kernels_check(lay, h->dc, s, o, c.max_row_degree, st));
+    HIP_TRY(kernels_check(lay, h->dc, cur, o, c.max_row_degree, st));
     if (prof) HIP_TRY(hipEventRecord(ws->pev[step][1], st));
-    HIP_TRY(kernels_variable(lay, h->dc, s, out, c.max_var_degree, o.fused_syn, (o.nt_rows & 2) != 0, st));
+    HIP_TRY(kernels_variable(lay, h->dc, cur, cur_out, c.max_var_degree, o.fused_syn, (o.nt_rows & 2) != 0, st));
     if (prof) HIP_TRY(hipEventRecord(ws->pev[step][2], st));
-    HIP_TRY(kernels_syndrome_control(lay, h->dc, s, h->iterations, o.fused_syn, step, c.max_row_degree, st));
+    HIP_TRY(kernels_syndrome_control(lay, h->dc, cur, h->iterations, o.fused_syn, step, c.max_row_degree, st));
     if (prof) HIP_TRY(hipEventRecord(ws->pev[step][3], st));
     steps_run = step + 1;
     if (capture) continue;
@@ -264,10 +307,30 @@ int run_batch(fecl_ldpc* h, Workspace* ws, hipStream_t st, const BatchIO& io, in
     // while the last two steps run, so the device queue never drains on the host.
     if (step >= 2) {
       HIP_TRY(hipEventSynchronize(ws->ev[(step - 2) % 4]));
-      if (ws->host_active[step - 2] == 0) break;
+      const int64_t live = ws->host_active[step - 2];
+      if (live == 0) break;
+      // Live counts only fall, so the frames live now fit in `need` tiles.  Moving a frame
+      // costs about one frame-iteration of memory traffic; it is counted 1.5 times.
+      const int64_t need = (live + TW - 1) / TW;
+      const int64_t freed = cur.tiles - need, remaining = h->iterations - 1 - step;
+      if (compact && need <= 3 * (int64_t)cur.tiles / 4 && 2 * freed * remaining * TW > 3 * live &&
+          step + kCompactMinSteps < h->iterations) {
+        const int next = 1 - cur_area;
+        DevState to;
+        DevOut to_out;
+        view_area(area_of(ws, next), need, s.active, soft, &to, &to_out);
+        int32_t* next_map = ws->map[next].as<int32_t>();
+        HIP_TRY(kernels_compact(lay, h->dc, cur, to, cur_map, next_map, ws->src_of.as<int32_t>(), st));
+        HIP_TRY(kernels_results(lay, h->dc, cur, cur_out, frames, frame0, io.res, io.dbl_out, cur_map, st));
+        cur = to;
+        cur_out = to_out;
+        cur_area = next;
+        cur_map = next_map;
+        o.nt_rows = nt_rows_for(h, lay, cur.tiles);
+      }
     }
   }
-  HIP_TRY(kernels_results(lay, h->dc, s, out, frames, frame0, io.res, io.dbl_out, st));
+  HIP_TRY(kernels_results(lay, h->dc, cur, cur_out, frames, frame0, io.res, io.dbl_out, cur_map, st));
   if (prof) {
     // kernel durations from events on the launch stream: check, variable, syndrome+control
     HIP_TRY(hipStreamSynchronize(st));
@@ -1069,6 +1132,13 @@ int fecl_ldpc_set_vars_per_wave(fecl_ldpc* h, int32_t vars) {
 int fecl_ldpc_get_vars_per_wave(const fecl_ldpc* h, int32_t* vars) {
   if (int rc = check_handle(h)) return rc;
   if (vars) *vars = h->vpw;
+  return FECL_OK;
+}
+
+int fecl_ldpc_set_compaction(fecl_ldpc* h, int32_t enable) {
+  if (int rc = check_handle(h)) return rc;
+  std::lock_guard<std::mutex> lk(h->mu);
+  h->compaction = enable != 0;
   return FECL_OK;
 }
 

@@ -68,8 +68,22 @@ struct HostBuf {
   template <typename T> T* as() const { return static_cast<T*>(p); }
 };
 
+// The per-tile buffers of one area of tiles (message rows, channel values, transient hard
+// decisions, per-slot state, tile flags and the outputs of frames that finish there).
+struct Area {
+  DevBuf msg, L, hard, phase, iter, fail, flags, out_hard, out_ext, out_app;
+  void release() {
+    for (DevBuf* b : {&msg, &L, &hard, &phase, &iter, &fail, &flags, &out_hard, &out_ext, &out_app}) b->release();
+  }
+};
+
 struct Workspace {
   DevBuf msg, L, hard, phase, iter, fail, flags, active, out_hard, out_ext, out_app;
+  // Compaction: the batch's first region of tiles uses the buffers above; later regions
+  // alternate between `spare` (three quarters as many tiles) and those buffers.  map[a]: slot -> frame
+  // of the region in area a (area 0's first region needs none); src_of: the plan's scratch.
+  Area spare;
+  DevBuf map[2], src_of;
   DevBuf stage_in[2], stage_syst[2], stage_state[2], stage_out[2];   // host path, double-buffered
   HostBuf pin_in[2], pin_syst[2], pin_state[2], pin_out[2];          // host path: FP32, pinned
   int32_t* host_active = nullptr;
@@ -87,6 +101,8 @@ struct Workspace {
                       &stage_in[0], &stage_in[1], &stage_syst[0], &stage_syst[1], &stage_state[0], &stage_state[1],
                       &stage_out[0], &stage_out[1]})
       b->release();
+    spare.release();
+    for (DevBuf* b : {&map[0], &map[1], &src_of}) b->release();
     for (HostBuf* b : {&pin_in[0], &pin_in[1], &pin_syst[0], &pin_syst[1], &pin_state[0], &pin_state[1],
                        &pin_out[0], &pin_out[1]})
       b->release();
@@ -131,6 +147,9 @@ struct fecl_ldpc {
   // tile's fail word.
   bool fused_syndrome = false;
   int64_t pass_frames = 0;  // frames per pass through the iteration loop; 0 = kMaxPassFrames
+  // Compaction of the live frames into fewer tiles once a quarter of a region's tiles could go
+  // (eager decodes without the fused syndrome; see run_batch).
+  bool compaction = true;
   int32_t cache_policy = -1; // message-row cache hints: -1 by working set, else DevOpts::nt_rows bits
   std::mutex mu;
   std::vector<std::unique_ptr<fecl::engine::Workspace>> pool;
@@ -170,6 +189,8 @@ DevOpts make_opts(const fecl_ldpc* h);
 // Sizes ws for `frames` frames of h's layout and returns the device views.
 int prepare(fecl_ldpc* h, Workspace* ws, const Layout& lay, int64_t frames, bool soft, DevState* sp, DevOut* op);
 constexpr int64_t kMaxPassFrames = 8192;
+// A compaction is enqueued only while at least this many steps may remain.
+constexpr int kCompactMinSteps = 4;
 // Decodes frames [frame0, frame0 + frames) of io in one pass (see ldpc_capi.cpp).
 int run_batch(fecl_ldpc* h, Workspace* ws, hipStream_t st, const BatchIO& io, int64_t frames, int64_t frame0,
               bool capture);

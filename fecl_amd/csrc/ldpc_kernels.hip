@@ -1046,16 +1046,20 @@ __global__ __launch_bounds__(kBlock) void state_in_kernel(DevCode c, DevState s,
   }
 }
 
-// Hard decisions [tile][K][FPL] -> msg [F][K] bytes (0/1).
+// Results of a region of tiles go to the frames its slots hold: slot j holds frame map[j]
+// (-1: none), or frame j when map is null (the batch's first region).
+
+// Hard decisions [tile][K][FPL] -> msg [F][K] bytes (0/1); blockIdx.y is the slot.
 template <int FPL>
 __global__ __launch_bounds__(kBlock) void unpack_hard_kernel(const uint64_t* hard, int K, int64_t frames,
-                                                             uint8_t* msg, int64_t frame0) {
+                                                             uint8_t* msg, int64_t frame0, const int32_t* map) {
   constexpr int TW = kWave * FPL;
-  const int64_t f = blockIdx.y;
+  const int64_t j = blockIdx.y;
+  const int64_t f = map ? (int64_t)map[j] : j;
   const int v = blockIdx.x * kBlock + threadIdx.x;
-  if (v >= K || f >= frames) return;
-  const int64_t t = f / TW;
-  const int p = (int)(f % TW);
+  if (v >= K || f < 0 || f >= frames) return;
+  const int64_t t = j / TW;
+  const int p = (int)(j % TW);
   const uint64_t w = hard[(t * K + v) * FPL + (p % FPL)];
   msg[(frame0 + f) * K + v] = (uint8_t)((w >> (p / FPL)) & 1ull);
 }
@@ -1063,7 +1067,8 @@ __global__ __launch_bounds__(kBlock) void unpack_hard_kernel(const uint64_t* har
 // [tile][X][TW] float -> [F][count] (Out = float or double), x < count, through LDS.
 template <typename Out, int TW>
 __global__ __launch_bounds__(kBlock) void tiles_to_frames_kernel(const float* in, int64_t X, int64_t count,
-                                                                 int64_t frames, Out* out, int64_t frame0) {
+                                                                 int64_t frames, Out* out, int64_t frame0,
+                                                                 const int32_t* map) {
   __shared__ float tile[64][65];
   const int t = blockIdx.y;
   const int64_t x0 = (int64_t)blockIdx.x * 64;
@@ -1076,9 +1081,10 @@ __global__ __launch_bounds__(kBlock) void tiles_to_frames_kernel(const float* in
     __syncthreads();
     for (int i = threadIdx.x; i < 64 * 64; i += kBlock) {
       const int fl = i / 64, xl = i % 64;
-      const int64_t f = (int64_t)t * TW + fc + fl;
+      const int64_t j = (int64_t)t * TW + fc + fl;
+      const int64_t f = map ? (int64_t)map[j] : j;
       const int64_t x = x0 + xl;
-      if (f < frames && x < count) out[(frame0 + f) * count + x] = (Out)tile[xl][fl];
+      if (f >= 0 && f < frames && x < count) out[(frame0 + f) * count + x] = (Out)tile[xl][fl];
     }
     __syncthreads();
   }
@@ -1087,7 +1093,7 @@ __global__ __launch_bounds__(kBlock) void tiles_to_frames_kernel(const float* in
 // Message rows -> state [F][E] (Out = float or double).
 template <typename Out, class F>
 __global__ __launch_bounds__(kBlock) void state_out_kernel(DevCode c, DevState s, int64_t frames,
-                                                           Out* out, int64_t frame0) {
+                                                           Out* out, int64_t frame0, const int32_t* map) {
   constexpr int TW = F::TW, RW = F::RW, PW = F::PW;
   __shared__ float tile[64][65];
   const int t = blockIdx.y;
@@ -1108,9 +1114,10 @@ __global__ __launch_bounds__(kBlock) void state_out_kernel(DevCode c, DevState s
     __syncthreads();
     for (int i = threadIdx.x; i < 64 * 64; i += kBlock) {
       const int fl = i / 64, el = i % 64;
-      const int64_t f = (int64_t)t * TW + fc + fl;
+      const int64_t j = (int64_t)t * TW + fc + fl;
+      const int64_t f = map ? (int64_t)map[j] : j;
       const int e = e0 + el;
-      if (f < frames && e < c.E) out[(frame0 + f) * c.E + e] = (Out)tile[fl][el];
+      if (f >= 0 && f < frames && e < c.E) out[(frame0 + f) * c.E + e] = (Out)tile[fl][el];
     }
     __syncthreads();
   }
@@ -1168,10 +1175,148 @@ __global__ __launch_bounds__(kBlock) void extract_kernel(const uint64_t* hard, c
   if (blockIdx.x == 0 && threadIdx.x == 0) iters[i] = iter[p];
 }
 
-// Per-frame iteration counts -> iterations [F].
-__global__ void iters_out_kernel(const int32_t* iter, int64_t frames, int32_t* out, int64_t frame0) {
-  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (f < frames) out[frame0 + f] = iter[f];
+// Per-frame iteration counts -> iterations [F]; n slots (map) or frames.
+__global__ void iters_out_kernel(const int32_t* iter, int64_t n, int64_t frames, int32_t* out, int64_t frame0,
+                                 const int32_t* map) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const int64_t f = map ? (int64_t)map[j] : j;
+  if (f >= 0 && f < frames) out[frame0 + f] = iter[j];
+}
+
+// ------------------------------------------------------------------------------------------
+// Compaction: frames that converge leave their lanes idle, and a tile works until its last
+// frame is done, so late in a decode most tiles carry a few live frames.  The live frames of a
+// region of tiles move, in slot order, into the first slots of a smaller region; later steps
+// run on that region only.  Frames are independent, so every result is unchanged.
+
+// Plan (one workgroup): the live slots (RUN or FINAL) of the source region become slots
+// 0 .. n-1 of the destination; src_of[j] is slot j's source slot and map_out[j] its frame
+// (map_in[slot], or the slot itself when map_in is null).  Destination slots n .. dst_slots-1
+// get -1.  No write goes past dst_slots.
+constexpr int kPlanThreads = 1024;
+__global__ __launch_bounds__(kPlanThreads) void compact_plan_kernel(const uint8_t* phase, const int32_t* map_in,
+                                                                    int32_t slots, int32_t* src_of, int32_t* map_out,
+                                                                    int32_t dst_slots) {
+  __shared__ int32_t sums[kPlanThreads];
+  const int tid = threadIdx.x;
+  const int per = (slots + kPlanThreads - 1) / kPlanThreads;
+  const int i0 = min(slots, tid * per), i1 = min(slots, i0 + per);
+  int n = 0;
+  for (int i = i0; i < i1; ++i) {
+    const int b = phase[i] & PH_MASK;
+    n += (b == PH_RUN || b == PH_FINAL) ? 1 : 0;
+  }
+  sums[tid] = n;
+  __syncthreads();
+  for (int off = 1; off < kPlanThreads; off <<= 1) {   // inclusive scan of the counts
+    const int add = tid >= off ? sums[tid - off] : 0;
+    __syncthreads();
+    sums[tid] += add;
+    __syncthreads();
+  }
+  const int total = sums[kPlanThreads - 1];
+  int j = sums[tid] - n;
+  for (int i = i0; i < i1; ++i) {
+    const int b = phase[i] & PH_MASK;
+    if (b == PH_RUN || b == PH_FINAL) {
+      if (j < dst_slots) {
+        src_of[j] = i;
+        map_out[j] = map_in ? map_in[i] : i;
+      }
+      ++j;
+    }
+  }
+  for (int k = total + tid; k < dst_slots; k += kPlanThreads) {
+    src_of[k] = -1;
+    map_out[k] = -1;
+  }
+}
+
+// Message rows: destination word w of a (tile, edge) row gathers its PW elements (element p of
+// a row sits in word p / PW, bits 16 * (p % PW) up) from their source slots; empty slots get 0.
+// A workgroup covers kCompactEdges edges of one destination tile.
+constexpr int kCompactEdges = 16;
+template <class F>
+__global__ __launch_bounds__(kBlock) void compact_msg_kernel(DevCode c, const uint32_t* src, uint32_t* dst,
+                                                             const int32_t* src_of) {
+  constexpr int TW = F::TW, RW = F::RW, PW = F::PW;
+  const int tb = blockIdx.y;
+  const int e_begin = blockIdx.x * kCompactEdges;
+  for (int i = threadIdx.x; i < kCompactEdges * RW; i += kBlock) {
+    const int e = e_begin + i / RW, w = i % RW;
+    if (e >= c.E) break;
+    uint32_t word = 0;
+#pragma unroll
+    for (int h = 0; h < PW; ++h) {
+      const int from = src_of[(int64_t)tb * TW + w * PW + h];
+      if (from >= 0) {
+        const int ta = from / TW, pa = from % TW;
+        const uint32_t sw = src[((int64_t)ta * c.E + e) * RW + pa / PW];
+        const uint32_t x = PW == 1 ? sw : (sw >> (16 * (pa % PW))) & 0xffffu;
+        word |= x << (16 * h);
+      }
+    }
+    dst[((int64_t)tb * c.E + e) * RW + w] = word;
+  }
+}
+
+// Channel values [tile][N][TW]: kCompactEdges variables of one destination tile per workgroup.
+template <int TW>
+__global__ __launch_bounds__(kBlock) void compact_L_kernel(DevCode c, const float* src, float* dst,
+                                                           const int32_t* src_of) {
+  const int tb = blockIdx.y;
+  const int v_begin = blockIdx.x * kCompactEdges;
+  for (int i = threadIdx.x; i < kCompactEdges * TW; i += kBlock) {
+    const int v = v_begin + i / TW, p = i % TW;
+    if (v >= c.N) break;
+    const int from = src_of[(int64_t)tb * TW + p];
+    float x = 0.0f;
+    if (from >= 0) x = src[((int64_t)(from / TW) * c.N + v) * TW + from % TW];
+    dst[((int64_t)tb * c.N + v) * TW + p] = x;
+  }
+}
+
+// Phases and iteration counts of the destination slots (empty slots: PH_FREE).
+__global__ void compact_slots_kernel(const uint8_t* src_phase, const int32_t* src_iter, uint8_t* dst_phase,
+                                     int32_t* dst_iter, const int32_t* src_of, int32_t dst_slots) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= dst_slots) return;
+  const int from = src_of[j];
+  dst_phase[j] = from >= 0 ? src_phase[from] : (uint8_t)PH_FREE;
+  dst_iter[j] = from >= 0 ? src_iter[from] : 0;
+}
+
+// Tile flags of the destination region from its phases (one wavefront per tile), as the
+// control kernel leaves them at the end of a step; fail masks cleared.
+template <int FPL>
+__global__ __launch_bounds__(kWave) void retile_kernel(DevState s) {
+  constexpr int TW = kWave * FPL;
+  const int t = blockIdx.x;
+  const int lane = threadIdx.x;
+  bool any_live = false, any_run = false, all_run = true;
+  int iters[FPL];
+#pragma unroll
+  for (int k = 0; k < FPL; ++k) {
+    const int64_t p = (int64_t)t * TW + lane * FPL + k;
+    const uint8_t ph = s.phase[p];
+    const int base = ph & PH_MASK;
+    iters[k] = base == PH_RUN ? s.iter[p] : -1;
+    any_live |= base == PH_RUN || base == PH_FINAL;
+    any_run |= base == PH_RUN;
+    all_run &= ph == PH_RUN;
+  }
+  const int it_ref = __builtin_amdgcn_readfirstlane(iters[0]);
+#pragma unroll
+  for (int k = 0; k < FPL; ++k) all_run &= iters[k] == it_ref;
+  const bool uni = __ballot(!all_run) == 0;
+  const uint64_t blive = __ballot(any_live), brun = __ballot(any_run);
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < FPL; ++k) s.fail[t * FPL + k] = 0;
+    s.tile_flags[t] = (blive ? TF_LIVE : 0u) | (brun ? TF_RUN : 0u) | (uni ? TF_UNIFORM : 0u);
+    s.tile_iter[t] = it_ref;
+  }
 }
 
 __global__ void fill_i32_kernel(int32_t* p, int64_t n, int32_t v) {
@@ -1426,30 +1571,49 @@ static hipError_t launch_state_in(const DevCode& c, const DevState& s, const voi
 
 template <class F>
 static hipError_t launch_results(const DevCode& c, const DevState& s, const DevOut& out, int64_t frames,
-                                 int64_t frame0, const BatchResults& r, bool dbl, hipStream_t st) {
+                                 int64_t frame0, const BatchResults& r, bool dbl, const int32_t* map, hipStream_t st) {
   constexpr int FPL = F::FPL, TW = F::TW;
+  const int64_t n = map ? (int64_t)s.tiles * TW : frames;   // slots of the region, or frames
+  if (n <= 0) return hipSuccess;
   if (r.msg_bits)
-    hipLaunchKernelGGL((unpack_hard_kernel<FPL>), dim3((c.K + kBlock - 1) / kBlock, (unsigned)frames), dim3(kBlock), 0, st,
-                       out.hard, c.K, frames, r.msg_bits, frame0);
+    hipLaunchKernelGGL((unpack_hard_kernel<FPL>), dim3((c.K + kBlock - 1) / kBlock, (unsigned)n), dim3(kBlock), 0, st,
+                       out.hard, c.K, frames, r.msg_bits, frame0, map);
   if (r.iterations)
-    hipLaunchKernelGGL(iters_out_kernel, dim3((unsigned)((frames + 255) / 256)), dim3(256), 0, st, s.iter, frames,
-                       r.iterations, frame0);
+    hipLaunchKernelGGL(iters_out_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, s.iter, n, frames,
+                       r.iterations, frame0, map);
   auto to_frames = [&](const float* in, int64_t X, int64_t count, void* dst) {
     dim3 grid((unsigned)((count + 63) / 64), s.tiles);
     if (dbl) hipLaunchKernelGGL((tiles_to_frames_kernel<double, TW>), grid, dim3(kBlock), 0, st, in, X, count,
-                                frames, (double*)dst, frame0);
+                                frames, (double*)dst, frame0, map);
     else hipLaunchKernelGGL((tiles_to_frames_kernel<float, TW>), grid, dim3(kBlock), 0, st, in, X, count,
-                            frames, (float*)dst, frame0);
+                            frames, (float*)dst, frame0, map);
   };
   if (r.msg) to_frames(out.app, c.K, c.K, r.msg);
   if (r.parity) to_frames(out.ext, c.N, c.N, r.parity);
   if (r.syst) to_frames(out.ext, c.N, c.K, r.syst);
   if (r.state) {
     if (dbl) hipLaunchKernelGGL((state_out_kernel<double, F>), dim3((c.E + 63) / 64, s.tiles), dim3(kBlock), 0, st,
-                                c, s, frames, (double*)r.state, frame0);
+                                c, s, frames, (double*)r.state, frame0, map);
     else hipLaunchKernelGGL((state_out_kernel<float, F>), dim3((c.E + 63) / 64, s.tiles), dim3(kBlock), 0, st,
-                            c, s, frames, (float*)r.state, frame0);
+                            c, s, frames, (float*)r.state, frame0, map);
   }
+  return hipGetLastError();
+}
+
+template <class F>
+static hipError_t launch_compact(const DevCode& c, const DevState& from, const DevState& to, const int32_t* map_in,
+                                 int32_t* map_out, int32_t* src_of, hipStream_t st) {
+  constexpr int FPL = F::FPL, TW = F::TW;
+  const int32_t slots = from.tiles * TW, dst_slots = to.tiles * TW;
+  hipLaunchKernelGGL(compact_plan_kernel, dim3(1), dim3(kPlanThreads), 0, st, from.phase, map_in, slots, src_of,
+                     map_out, dst_slots);
+  hipLaunchKernelGGL((compact_msg_kernel<F>), dim3((c.E + kCompactEdges - 1) / kCompactEdges, to.tiles), dim3(kBlock),
+                     0, st, c, from.msg, to.msg, src_of);
+  hipLaunchKernelGGL((compact_L_kernel<TW>), dim3((c.N + kCompactEdges - 1) / kCompactEdges, to.tiles), dim3(kBlock),
+                     0, st, c, from.L, to.L, src_of);
+  hipLaunchKernelGGL(compact_slots_kernel, dim3((dst_slots + 255) / 256), dim3(256), 0, st, from.phase, from.iter,
+                     to.phase, to.iter, src_of, dst_slots);
+  hipLaunchKernelGGL((retile_kernel<FPL>), dim3(to.tiles), dim3(kWave), 0, st, to);
   return hipGetLastError();
 }
 
@@ -1554,8 +1718,14 @@ hipError_t kernels_extract(const Layout& lay, const DevCode& c, const DevState& 
 }
 
 hipError_t kernels_results(const Layout& lay, const DevCode& c, const DevState& s, const DevOut& out, int64_t frames,
-                           int64_t frame0, const BatchResults& r, bool dbl, hipStream_t st) {
-  FECL_DISPATCH(lay, launch_results<F>(c, s, out, frames, frame0, r, dbl, st));
+                           int64_t frame0, const BatchResults& r, bool dbl, const int32_t* map, hipStream_t st) {
+  FECL_DISPATCH(lay, launch_results<F>(c, s, out, frames, frame0, r, dbl, map, st));
+}
+
+hipError_t kernels_compact(const Layout& lay, const DevCode& c, const DevState& from, const DevState& to,
+                           const int32_t* map_in, int32_t* map_out, int32_t* src_of, hipStream_t st) {
+  if (from.tiles <= 0 || to.tiles <= 0 || (int64_t)from.tiles * lay.tw() > INT32_MAX) return hipErrorInvalidValue;
+  FECL_DISPATCH(lay, launch_compact<F>(c, from, to, map_in, map_out, src_of, st));
 }
 
 }  // namespace fecl

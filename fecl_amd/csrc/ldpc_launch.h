@@ -65,7 +65,16 @@ hipError_t kernels_admit(const Layout& lay, const DevCode& c, const DevState& s,
                          const int32_t* slots, int n, uint8_t init_phase, hipStream_t st);
 hipError_t kernels_extract(const Layout& lay, const DevCode& c, const DevState& s, const DevOut& out,
                            const int32_t* slots, int n, uint8_t* bits, int32_t* iters, hipStream_t st);
+// Results of a region of tiles; map[j] is the frame of slot j (-1: none), or null for the
+// batch's first region, whose slot j holds frame j.
 hipError_t kernels_results(const Layout& lay, const DevCode& c, const DevState& s, const DevOut& out, int64_t frames,
-                           int64_t frame0, const BatchResults& r, bool dbl, hipStream_t st);
+                           int64_t frame0, const BatchResults& r, bool dbl, const int32_t* map, hipStream_t st);
+// Moves the live frames of region `from` (slot j holds frame map_in[j], or j when map_in is
+// null) in slot order into the first slots of region `to`, which must have room for them:
+// message rows, channel values, phases and iteration counts; sets map_out (the frames of
+// `to`'s slots, -1 past the last) and `to`'s tile flags.  src_of: scratch of to.tiles * TW
+// entries.  Call at a step boundary (after the control kernel), without the fused syndrome.
+hipError_t kernels_compact(const Layout& lay, const DevCode& c, const DevState& from, const DevState& to,
+                           const int32_t* map_in, int32_t* map_out, int32_t* src_of, hipStream_t st);
 
 }  // namespace fecl

@@ -388,6 +388,10 @@ class Ldpc:
         """Decode batches in passes of at most `frames` frames (0 = 8192); results are the same."""
         check(lib().fecl_ldpc_set_pass_frames(self._h, int(frames)))
 
+    def setCompaction(self, enable):
+        """Move live frames into fewer tiles as frames converge (default on); same results."""
+        check(lib().fecl_ldpc_set_compaction(self._h, int(bool(enable))))
+
     def layout(self):
         """(words per lane, frames per tile) of the current device layout."""
         w, tw = ctypes.c_int32(), ctypes.c_int32()

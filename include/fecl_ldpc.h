@@ -211,6 +211,12 @@ int fecl_ldpc_get_layout(const fecl_ldpc* h, int32_t* words, int32_t* frames_per
  * messages and channel values can stay in the device's last-level cache between
  * iterations.  0 (default) = 8192 frames per pass. */
 int fecl_ldpc_set_pass_frames(fecl_ldpc* h, int64_t frames);
+/* Compaction (no counterpart in the reference; results are identical either way).  A tile
+ * of frames works until its last frame finishes, so once frames converge most tiles carry a
+ * few live frames.  With compaction on (default), an eager decode moves the live frames into
+ * three quarters as many tiles or fewer whenever they fit and enough iterations remain, and
+ * later iterations run on those tiles only; the batch's workspace grows by up to 75 %.  Hipgraph-captured decodes and the fused syndrome never compact. */
+int fecl_ldpc_set_compaction(fecl_ldpc* h, int32_t enable);
 /* Cache hints for the message rows (no counterpart in the reference; results are identical
  * for any value): bit 0 = non-temporal loads/stores in the check kernel, bit 1 = in the
  * variable kernel; -1 (default) = both once a batch's rows and channel values exceed 640 MiB,

@@ -1,0 +1,131 @@
+"""GPU: compaction of live frames into fewer tiles (run_batch in fecl_amd/csrc/ldpc_capi.cpp).
+
+Frames are independent (FeCl decodes each block on its own, BpDecoderImpl.cpp:38-130), so
+moving a frame's messages, channel values and state to another slot must change nothing: every
+output stays bit-identical to the oracle and to the decode without compaction.  The inputs mix
+frames that converge at different iterations with frames that never do, shuffled over the
+tiles, so that the live frames shrink to a half, a quarter, ... of the slots while every tile
+still holds one: the decoder then compacts more than once, and the later regions reuse the
+first region's buffers.
+"""
+import numpy as np
+import pytest
+
+import fecl_amd as F
+from oracle import oracle as O
+from tests.golden.vectors import dvbs2_llr, gallager_llr
+from tests.test_gpu_decoder import assert_bit_exact, gallager, gpu_decoder, oracle_run
+
+pytestmark = pytest.mark.gpu
+
+ITERS = 30
+
+
+def staggered_llr(oc, frames, seed):
+    """All-zero codewords at Eb/N0 from 6 dB (converge in 2-3 updates) down to -3 dB (never),
+    shuffled over the batch."""
+    rng = np.random.default_rng(seed)
+    points = [6.0, 4.0, 3.0, 2.5, 2.0, 1.5, -3.0]
+    share = [0.25, 0.2, 0.15, 0.1, 0.1, 0.1, 0.1]
+    counts = np.floor(np.array(share) * frames).astype(int)
+    counts[0] += frames - counts.sum()
+    parts = [gallager_llr(oc.N, oc.K, int(n), e, seed + 7 * i) for i, (n, e) in enumerate(zip(counts, points))]
+    llr = np.concatenate(parts)
+    return llr[rng.permutation(frames)].astype(np.float32).astype(np.float64)
+
+
+def compactions(iters, tw, iterations):
+    """Compactions the host's rule performs for these per-frame update counts.  A frame with c
+    check updates finishes in step c-1, so after step s the host knows the frames live after
+    step s-2 (c > s-1).  It compacts when they fit in three quarters of the current region's
+    tiles, the freed tile-steps outweigh 1.5 frame-iterations per live frame, and at least 4
+    steps are left."""
+    tiles = (len(iters) + tw - 1) // tw
+    n = 0
+    for step in range(2, iterations):
+        live = int(np.sum(iters > step - 1))
+        if live == 0:
+            break
+        need = (live + tw - 1) // tw
+        remaining = iterations - 1 - step
+        if need <= 3 * tiles // 4 and 2 * (tiles - need) * remaining * tw > 3 * live and step + 4 < iterations:
+            tiles, n = need, n + 1
+    return n
+
+
+@pytest.mark.parametrize("fmt,ofmt,words", [(F.MsgFormat.F32, O.MSG_F32, 1), (F.MsgFormat.F32, O.MSG_F32, 4),
+                                            (F.MsgFormat.F16, O.MSG_F16, 1), (F.MsgFormat.BF16, O.MSG_BF16, 2)])
+def test_compacted_decode_matches_the_oracle(fmt, ofmt, words):
+    """soDecode (APP, extrinsic, state, iteration counts) and decode bits bit-identical to the
+    oracle while the live frames move through several regions."""
+    oc, H = gallager()
+    frames = 1024
+    llr = staggered_llr(oc, frames, 40 + words)
+    ref = oracle_run(oc, llr, F.Approximate, ITERS, fmt=ofmt, threads=16)
+    dec = gpu_decoder(H, F.Approximate, ITERS, fmt=fmt, words=words)
+    tw = dec.layout()[1]
+    assert compactions(ref["iters"], tw, ITERS) >= 2
+    print(f"fmt={fmt} words={words} tw={tw} compactions={compactions(ref['iters'], tw, ITERS)}", flush=True)
+    assert_bit_exact(dec.soDecode(llr), ref, frames, oc.K, oc.N, oc.E)
+    bits = dec.decode(llr)
+    assert np.array_equal(np.asarray(bits).reshape(frames, oc.K), ref["hard"])
+
+
+@pytest.mark.parametrize("alg", [F.Linear, F.Exact])
+def test_compaction_changes_nothing(alg):
+    """Linear and Exact with scaling factors: every output with compaction equals the output
+    without it (Exact: the oracle is not bit-exact, so this is the check)."""
+    oc, H = gallager()
+    frames = 768
+    llr = staggered_llr(oc, frames, 50)
+    outs = []
+    for on in (True, False):
+        dec = gpu_decoder(H, alg, ITERS, sf=0.75, words=1)
+        dec.setCompaction(on)
+        o = dec.soDecode(llr)
+        o["bits"] = np.asarray(dec.decode(llr))
+        outs.append(o)
+    assert compactions(outs[0]["iterations"], 64, ITERS) >= 2
+    for name in ("msg", "syst", "parity", "state", "iterations", "bits"):
+        a, b = np.asarray(outs[0][name]), np.asarray(outs[1][name])
+        assert np.array_equal(a.view(np.uint8), b.view(np.uint8)), name
+
+
+def test_compaction_with_state_and_syst_inputs():
+    """A-priori state and systematic inputs (the state-seeded first variable pass) on a batch
+    that compacts: bit-identical to the oracle."""
+    oc, H = gallager()
+    frames = 512
+    llr = staggered_llr(oc, frames, 60)
+    rng = np.random.default_rng(61)
+    syst = rng.normal(0.0, 0.5, (frames, oc.K)).astype(np.float32).astype(np.float64)
+    state = rng.normal(0.0, 0.5, (frames, oc.E)).astype(np.float32).astype(np.float64)
+    ref = oracle_run(oc, llr, F.Approximate, ITERS, syst_in=syst, state_in=state, threads=16)
+    dec = gpu_decoder(H, F.Approximate, ITERS, words=1)
+    assert compactions(ref["iters"], 64, ITERS) >= 1
+    assert_bit_exact(dec.soDecode(llr, syst=syst, state=state), ref, frames, oc.K, oc.N, oc.E)
+
+
+def test_device_path_dvbs2_compaction_on_and_off():
+    """DVB-S2 r1/2 near the waterfall through the device path (FP16 messages): bits and
+    iteration counts with compaction equal those without it and the oracle's on a sample."""
+    import torch
+    from tests.test_gpu_decoder import dvbs2
+    oc, H = dvbs2()
+    frames = 512
+    llr = np.concatenate([dvbs2_llr(256, 1.7, seed=71), dvbs2_llr(256, 0.5, seed=72)])
+    llr = llr[np.random.default_rng(73).permutation(frames)].astype(np.float32)
+    x = torch.from_numpy(llr).cuda()
+    res = []
+    for on in (True, False):
+        dec = gpu_decoder(H, F.Approximate, 50, fmt=F.MsgFormat.F16, words=1)
+        dec.setCompaction(on)
+        bits, it = dec.decode(x, iterations=True)
+        torch.cuda.synchronize()
+        res.append((bits.cpu().numpy(), it.cpu().numpy()))
+    assert np.array_equal(res[0][0], res[1][0]) and np.array_equal(res[0][1], res[1][1])
+    assert compactions(res[0][1], 128, 50) >= 1
+    sample = np.arange(0, frames, 37)
+    ref = oracle_run(oc, llr[sample].astype(np.float64), F.Approximate, 50, fmt=O.MSG_F16, threads=16)
+    assert np.array_equal(res[0][1][sample], ref["iters"])
+    assert np.array_equal(res[0][0].reshape(frames, oc.K)[sample], ref["hard"])
