@@ -40,8 +40,9 @@ AGREEMENT_EBN0 = (0.8, 0.9, 1.0, 1.1, 1.2)   # Linear / Exact waterfall of r1/2
 PRECISION_WATERFALL = (1.45, 1.5, 1.55, 1.6, 1.65, 1.7)
 
 
-def make_batches(F, dec, H, frames, seed, device):
-    """Seeded info bits, library encoder, BPSK/AWGN on the GPU -> FP32 LLR batches."""
+def make_batches(F, dec, H, frames, seed, device, points=EBN0):
+    """Seeded info bits, library encoder, BPSK/AWGN on the GPU -> FP32 LLR batches, one per
+    Eb/N0 point."""
     import numpy as np
     import torch
     K, N = dec.msgSize(), dec.paritySize()
@@ -50,7 +51,7 @@ def make_batches(F, dec, H, frames, seed, device):
     rng = np.random.default_rng(seed)
     gen = torch.Generator(device=device)
     gen.manual_seed(seed)
-    for e in EBN0:
+    for e in points:
         info = rng.integers(0, 2, (frames, K), dtype=np.uint8)
         cw = dec.encode(info).reshape(frames, N)
         sigma2 = 1.0 / (2.0 * rate * 10.0 ** (e / 10.0))
@@ -536,6 +537,10 @@ def main():
     ap.add_argument("--offered-fps", type=float, default=200000.0)
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-forms", action="store_true", help="skip the Linear / FP16 / BF16 lines")
+    ap.add_argument("--ebn0", type=str, default="",
+                    help="comma-separated Eb/N0 (dB) of the batches (default: the headline points)")
+    ap.add_argument("--compaction", choices=["on", "off"], default="on",
+                    help="compaction of live frames into fewer tiles (results are the same)")
     ap.add_argument("--pass-frames", type=int, default=0, help="frames per decode pass (0 = library default)")
     ap.add_argument("--vpw", type=int, default=0, help="variables per wavefront (0 automatic; with --words >= 0)")
     args = ap.parse_args()
@@ -586,9 +591,11 @@ def main():
                 "probe_ms": {f"w{k[0]}v{k[1]}": round(v, 3) for k, v in times.items()}}
         print(f"[bench] autotune: {tune}", file=sys.stderr, flush=True)
     dec.setPassFrames(args.pass_frames)
+    dec.setCompaction(args.compaction == "on")
     words, frames_per_tile = dec.layout()
-    print(f"[bench] building {len(EBN0)} batches of {args.frames} frames", file=sys.stderr, flush=True)
-    batches = make_batches(F, dec, H, args.frames, args.seed, device)
+    points = tuple(float(x) for x in args.ebn0.split(",")) if args.ebn0 else EBN0
+    print(f"[bench] building {len(points)} batches of {args.frames} frames", file=sys.stderr, flush=True)
+    batches = make_batches(F, dec, H, args.frames, args.seed, device, points)
     torch.cuda.synchronize()
 
     stream = torch.cuda.current_stream(device)
@@ -665,7 +672,7 @@ def main():
         "config": {"workload": f"DVB-S2 n=64800 r1/2, {args.frames} frames per batch, "
                                f"{'min-sum' if args.alg == 'minsum' else args.alg}, {args.msg} messages, "
                                f"{args.iterations} max iterations with syndrome early exit, "
-                               f"Eb/N0 {list(EBN0)} dB",
+                               f"Eb/N0 {list(points)} dB",
                    "frames_per_step": args.frames, "iterations": args.iterations, "algorithm": args.alg,
                    "msg_format": args.msg, "code": "dvbs2 64800 1/2",
                    "lane_words": words, "frames_per_tile": frames_per_tile, "vars_per_wave": dec.varsPerWave(),
@@ -721,10 +728,10 @@ def main():
 
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
-        b = batches[2]  # Eb/N0 = 1.0 dB
+        cpu_batch = batches[min(2, len(batches) - 1)]  # Eb/N0 = 1.0 dB with the headline points
         nf = min(args.cpu_frames, args.frames)
-        llr_host = b["llr"][:nf].cpu().numpy().astype(np.float64)
-        info_host = b["info"][:nf].cpu().numpy()
+        llr_host = cpu_batch["llr"][:nf].cpu().numpy().astype(np.float64)
+        info_host = cpu_batch["info"][:nf].cpu().numpy()
         print(f"[bench] CPU baseline: {nf} frames on {threads} threads", file=sys.stderr, flush=True)
         cb = cpu_baseline(llr_host, info_host, args.iterations, threads)
         if args.alg == "minsum":
@@ -741,7 +748,7 @@ def main():
             result["fer_parity_sample"] = par
         result["cpu_baseline"] = {"value": round(nf * K / cb["seconds"] / 1e9, 6), "unit": "Gb/s",
                                   "cores": threads, "kind": cb["kind"],
-                                  "sample": f"{nf} frames of the Eb/N0=1.0 dB batch, FP64 min-sum, "
+                                  "sample": f"{nf} frames of the Eb/N0={cpu_batch['ebn0']} dB batch, FP64 min-sum, "
                                             f"{args.iterations} max iterations, {cb['seconds']:.2f} s wall",
                                   "cpu": _cpu_model()}
     if not args.no_serving:
