@@ -229,6 +229,23 @@ void view_area(const AreaRefs& b, int64_t tiles, int32_t* active, bool soft, Dev
 
 }  // namespace
 
+// Tiles of the largest pass run_passes makes of `frames` frames.
+int64_t pass_tiles(const fecl_ldpc* h, const Layout& lay, int64_t frames) {
+  const int64_t tw = lay.tw();
+  int64_t pass = h->pass_frames > 0 ? h->pass_frames : kMaxPassFrames;
+  pass = std::max<int64_t>(tw, (pass + tw - 1) / tw * tw);
+  return (std::min(pass, frames) + tw - 1) / tw;
+}
+
+int prepare_graph_compaction(fecl_ldpc* h, Workspace* ws, int64_t frames) {
+  const Layout lay = layout_of(h);
+  const int64_t tiles = pass_tiles(h, lay, frames);
+  if (int rc = size_area(h->code, lay, area_of(ws, 1), tiles, false)) return rc;
+  for (DevBuf* b : {&ws->map[0], &ws->map[1], &ws->src_of}) HIP_TRY(b->ensure((size_t)tiles * lay.tw() * 4));
+  HIP_TRY(ws->ctl.ensure(sizeof(CompactCtl)));
+  return FECL_OK;
+}
+
 // Sizes the workspace for `frames` frames and returns the device views of it.
 int prepare(fecl_ldpc* h, Workspace* ws, const Layout& lay, int64_t frames, bool soft, DevState* sp, DevOut* op) {
   const int64_t tiles = (frames + lay.tw() - 1) / lay.tw();
@@ -283,6 +300,21 @@ int run_batch(fecl_ldpc* h, Workspace* ws, hipStream_t st, const BatchIO& io, in
   DevOut cur_out = out;
   int cur_area = 0;
   const int32_t* cur_map = nullptr;
+  // Captured graphs: no host sees the live counts, so after every kGraphCompactEvery-th step
+  // the device decides and compacts in place (live frames through scratch into the region's
+  // first slots); a checkpoint that declines costs a few launches that return at once.
+  const size_t slot_bytes = (size_t)s.tiles * TW * 4;
+  const bool graph_compact = h->compaction && capture && !o.fused_syn && s.tiles >= 2 &&
+                             ws->spare.msg.bytes >= (size_t)s.tiles * c.E * lay.rw() * 4 &&
+                             ws->spare.L.bytes >= (size_t)s.tiles * c.N * TW * 4 &&
+                             ws->spare.phase.bytes >= (size_t)s.tiles * TW && ws->spare.iter.bytes >= slot_bytes &&
+                             ws->spare.flags.bytes >= (size_t)s.tiles * 8 && ws->map[0].bytes >= slot_bytes &&
+                             ws->map[1].bytes >= slot_bytes && ws->src_of.bytes >= slot_bytes &&
+                             ws->ctl.bytes >= sizeof(CompactCtl);
+  DevState scratch{};
+  DevOut scratch_out{};
+  if (graph_compact) view_area(area_of(ws, 1), s.tiles, s.active, false, &scratch, &scratch_out);
+  int checkpoint = 0;
 
   HIP_TRY(kernels_fill_i32(s.active, h->iterations, 0, st));
   uint8_t init = (uint8_t)((h->iterations == 1 ? PH_FINAL : PH_RUN) | (io.state ? PH_VNINIT : PH_FRESH));
@@ -300,7 +332,19 @@ int run_batch(fecl_ldpc* h, Workspace* ws, hipStream_t st, const BatchIO& io, in
     HIP_TRY(kernels_syndrome_control(lay, h->dc, cur, h->iterations, o.fused_syn, step, c.max_row_degree, st));
     if (prof) HIP_TRY(hipEventRecord(ws->pev[step][3], st));
     steps_run = step + 1;
-    if (capture) continue;
+    if (capture) {
+      if (graph_compact && (step + 1) % kGraphCompactEvery == 0 && step + kCompactMinSteps < h->iterations) {
+        int32_t* next_map = ws->map[checkpoint & 1].as<int32_t>();
+        CompactCtl* ctl = ws->ctl.as<CompactCtl>();
+        int32_t* src_of = ws->src_of.as<int32_t>();
+        HIP_TRY(kernels_compact_plan(lay, cur, cur_map, next_map, src_of, ctl, h->iterations - 1 - step, st));
+        HIP_TRY(kernels_results(lay, h->dc, cur, cur_out, frames, frame0, io.res, io.dbl_out, cur_map, ctl, st));
+        HIP_TRY(kernels_compact_in_place(lay, h->dc, cur, scratch, src_of, ctl, st));
+        cur_map = next_map;
+        ++checkpoint;
+      }
+      continue;
+    }
     HIP_TRY(hipMemcpyAsync(&ws->host_active[step], &s.active[step], 4, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipEventRecord(ws->ev[step % 4], st));
     // Stop enqueuing once every frame of the batch has finished: the count of step-2 is read
@@ -321,7 +365,7 @@ int run_batch(fecl_ldpc* h, Workspace* ws, hipStream_t st, const BatchIO& io, in
         view_area(area_of(ws, next), need, s.active, soft, &to, &to_out);
         int32_t* next_map = ws->map[next].as<int32_t>();
         HIP_TRY(kernels_compact(lay, h->dc, cur, to, cur_map, next_map, ws->src_of.as<int32_t>(), st));
-        HIP_TRY(kernels_results(lay, h->dc, cur, cur_out, frames, frame0, io.res, io.dbl_out, cur_map, st));
+        HIP_TRY(kernels_results(lay, h->dc, cur, cur_out, frames, frame0, io.res, io.dbl_out, cur_map, nullptr, st));
         cur = to;
         cur_out = to_out;
         cur_area = next;
@@ -330,7 +374,7 @@ int run_batch(fecl_ldpc* h, Workspace* ws, hipStream_t st, const BatchIO& io, in
       }
     }
   }
-  HIP_TRY(kernels_results(lay, h->dc, cur, cur_out, frames, frame0, io.res, io.dbl_out, cur_map, st));
+  HIP_TRY(kernels_results(lay, h->dc, cur, cur_out, frames, frame0, io.res, io.dbl_out, cur_map, nullptr, st));
   if (prof) {
     // kernel durations from events on the launch stream: check, variable, syndrome+control
     HIP_TRY(hipStreamSynchronize(st));
@@ -351,7 +395,7 @@ int run_passes(fecl_ldpc* h, Workspace* ws, hipStream_t st, const BatchIO& io, i
                bool capture) {
   const int64_t tw = layout_of(h).tw();
   int64_t pass = h->pass_frames > 0 ? h->pass_frames : kMaxPassFrames;
-  pass = std::max<int64_t>(tw, (pass + tw - 1) / tw * tw);
+  pass = std::max<int64_t>(tw, (pass + tw - 1) / tw * tw);   // as pass_tiles
   for (int64_t f = 0; f < frames; f += pass)
     if (int rc = run_batch(h, ws, st, io, std::min(pass, frames - f), frame0 + f, capture)) return rc;
   return FECL_OK;
@@ -1284,6 +1328,8 @@ int fecl_ldpc_graph_create(fecl_ldpc* const* handles, const int64_t* frames, int
     DevState s;
     DevOut o;
     if (int rc = prepare(h, ws.get(), layout_of(h), frames[i], false, &s, &o)) return rc;
+    if (h->compaction)
+      if (int rc = prepare_graph_compaction(h, ws.get(), frames[i])) return rc;
     g->ws.push_back(std::move(ws));
     hipStream_t st = nullptr;
     HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));

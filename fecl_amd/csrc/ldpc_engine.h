@@ -82,8 +82,10 @@ struct Workspace {
   // Compaction: the batch's first region of tiles uses the buffers above; later regions
   // alternate between `spare` (three quarters as many tiles) and those buffers.  map[a]: slot -> frame
   // of the region in area a (area 0's first region needs none); src_of: the plan's scratch.
+  // Captured graphs compact in place instead: `spare` (as many tiles) is the scratch, map[]
+  // alternates per checkpoint, and ctl holds the device's decision (CompactCtl).
   Area spare;
-  DevBuf map[2], src_of;
+  DevBuf map[2], src_of, ctl;
   DevBuf stage_in[2], stage_syst[2], stage_state[2], stage_out[2];   // host path, double-buffered
   HostBuf pin_in[2], pin_syst[2], pin_state[2], pin_out[2];          // host path: FP32, pinned
   int32_t* host_active = nullptr;
@@ -102,7 +104,7 @@ struct Workspace {
                       &stage_out[0], &stage_out[1]})
       b->release();
     spare.release();
-    for (DevBuf* b : {&map[0], &map[1], &src_of}) b->release();
+    for (DevBuf* b : {&map[0], &map[1], &src_of, &ctl}) b->release();
     for (HostBuf* b : {&pin_in[0], &pin_in[1], &pin_syst[0], &pin_syst[1], &pin_state[0], &pin_state[1],
                        &pin_out[0], &pin_out[1]})
       b->release();
@@ -191,6 +193,12 @@ int prepare(fecl_ldpc* h, Workspace* ws, const Layout& lay, int64_t frames, bool
 constexpr int64_t kMaxPassFrames = 8192;
 // A compaction is enqueued only while at least this many steps may remain.
 constexpr int kCompactMinSteps = 4;
+// Captured graphs: a device-decided compaction after every this many steps (4, 6 and 8 measured
+// on the uneven seven-rate batch: 3.06, 3.13 and 3.10 Gb/s).
+constexpr int kGraphCompactEvery = 6;
+// Sizes ws's in-place compaction buffers for a captured decode of `frames` frames (before
+// capture: nothing may allocate while a graph is recorded).
+int prepare_graph_compaction(fecl_ldpc* h, Workspace* ws, int64_t frames);
 // Decodes frames [frame0, frame0 + frames) of io in one pass (see ldpc_capi.cpp).
 int run_batch(fecl_ldpc* h, Workspace* ws, hipStream_t st, const BatchIO& io, int64_t frames, int64_t frame0,
               bool capture);

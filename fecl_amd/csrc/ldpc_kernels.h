@@ -73,6 +73,13 @@ struct DevOpts {
   float lin_s[8];          // slopes y[i+1] - y[i] (float), entry 7 = 0
 };
 
+// A compaction decided on the device (captured graphs): whether it moves frames, and how many
+// frames are live.  Kernels handed one do nothing unless `go` is set.
+struct CompactCtl {
+  int32_t go;
+  int32_t live;
+};
+
 // Outputs of a finished batch, in tile layout (device buffers owned by the engine).
 struct DevOut {
   uint64_t* hard;          // [tile][K][FPL]  decode hard decisions (L + sum ascending >= 0)

@@ -1052,8 +1052,10 @@ __global__ __launch_bounds__(kBlock) void state_in_kernel(DevCode c, DevState s,
 // Hard decisions [tile][K][FPL] -> msg [F][K] bytes (0/1); blockIdx.y is the slot.
 template <int FPL>
 __global__ __launch_bounds__(kBlock) void unpack_hard_kernel(const uint64_t* hard, int K, int64_t frames,
-                                                             uint8_t* msg, int64_t frame0, const int32_t* map) {
+                                                             uint8_t* msg, int64_t frame0, const int32_t* map,
+                                                             const CompactCtl* ctl) {
   constexpr int TW = kWave * FPL;
+  if (ctl && !ctl->go) return;
   const int64_t j = blockIdx.y;
   const int64_t f = map ? (int64_t)map[j] : j;
   const int v = blockIdx.x * kBlock + threadIdx.x;
@@ -1068,8 +1070,9 @@ __global__ __launch_bounds__(kBlock) void unpack_hard_kernel(const uint64_t* har
 template <typename Out, int TW>
 __global__ __launch_bounds__(kBlock) void tiles_to_frames_kernel(const float* in, int64_t X, int64_t count,
                                                                  int64_t frames, Out* out, int64_t frame0,
-                                                                 const int32_t* map) {
+                                                                 const int32_t* map, const CompactCtl* ctl) {
   __shared__ float tile[64][65];
+  if (ctl && !ctl->go) return;
   const int t = blockIdx.y;
   const int64_t x0 = (int64_t)blockIdx.x * 64;
   for (int fc = 0; fc < TW; fc += 64) {
@@ -1093,9 +1096,11 @@ __global__ __launch_bounds__(kBlock) void tiles_to_frames_kernel(const float* in
 // Message rows -> state [F][E] (Out = float or double).
 template <typename Out, class F>
 __global__ __launch_bounds__(kBlock) void state_out_kernel(DevCode c, DevState s, int64_t frames,
-                                                           Out* out, int64_t frame0, const int32_t* map) {
+                                                           Out* out, int64_t frame0, const int32_t* map,
+                                                           const CompactCtl* ctl) {
   constexpr int TW = F::TW, RW = F::RW, PW = F::PW;
   __shared__ float tile[64][65];
+  if (ctl && !ctl->go) return;
   const int t = blockIdx.y;
   const int e0 = blockIdx.x * 64;
   for (int fc = 0; fc < TW; fc += 64) {
@@ -1177,9 +1182,9 @@ __global__ __launch_bounds__(kBlock) void extract_kernel(const uint64_t* hard, c
 
 // Per-frame iteration counts -> iterations [F]; n slots (map) or frames.
 __global__ void iters_out_kernel(const int32_t* iter, int64_t n, int64_t frames, int32_t* out, int64_t frame0,
-                                 const int32_t* map) {
+                                 const int32_t* map, const CompactCtl* ctl) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
+  if (j >= n || (ctl && !ctl->go)) return;
   const int64_t f = map ? (int64_t)map[j] : j;
   if (f >= 0 && f < frames) out[frame0 + f] = iter[j];
 }
@@ -1194,12 +1199,33 @@ __global__ void iters_out_kernel(const int32_t* iter, int64_t n, int64_t frames,
 // 0 .. n-1 of the destination; src_of[j] is slot j's source slot and map_out[j] its frame
 // (map_in[slot], or the slot itself when map_in is null).  Destination slots n .. dst_slots-1
 // get -1.  No write goes past dst_slots.
+// With ctl (a captured graph, where no host sees the counts): the kernel decides.  It moves
+// the frames only when they fit in fewer tiles than hold live frames now, and the tile-steps
+// freed over the `remaining` steps outweigh moving each live frame twice (into scratch and
+// back, 1.5 frame-iterations a move).  Otherwise map_out[j] = map_in[j] for every slot and
+// src_of is left alone.  ctl->go and ctl->live record the outcome.
 constexpr int kPlanThreads = 1024;
 __global__ __launch_bounds__(kPlanThreads) void compact_plan_kernel(const uint8_t* phase, const int32_t* map_in,
                                                                     int32_t slots, int32_t* src_of, int32_t* map_out,
-                                                                    int32_t dst_slots) {
+                                                                    int32_t dst_slots, CompactCtl* ctl, int32_t tw,
+                                                                    int32_t remaining) {
   __shared__ int32_t sums[kPlanThreads];
+  __shared__ int32_t tiles_live;
   const int tid = threadIdx.x;
+  if (ctl) {
+    if (tid == 0) tiles_live = 0;
+    __syncthreads();
+    int mine = 0;
+    for (int t = tid; t * tw < slots; t += kPlanThreads) {
+      bool any = false;
+      for (int i = t * tw; i < t * tw + tw && i < slots; ++i) {
+        const int b = phase[i] & PH_MASK;
+        any |= b == PH_RUN || b == PH_FINAL;
+      }
+      mine += any ? 1 : 0;
+    }
+    if (mine) atomicAdd(&tiles_live, mine);
+  }
   const int per = (slots + kPlanThreads - 1) / kPlanThreads;
   const int i0 = min(slots, tid * per), i1 = min(slots, i0 + per);
   int n = 0;
@@ -1216,6 +1242,18 @@ __global__ __launch_bounds__(kPlanThreads) void compact_plan_kernel(const uint8_
     __syncthreads();
   }
   const int total = sums[kPlanThreads - 1];
+  if (ctl) {
+    const int64_t need = (total + tw - 1) / tw, freed = (int64_t)tiles_live - need;
+    const bool go = freed > 0 && 2 * freed * remaining * tw > 6 * (int64_t)total;
+    if (tid == 0) {
+      ctl->go = go ? 1 : 0;
+      ctl->live = total;
+    }
+    if (!go) {
+      for (int k = tid; k < slots; k += kPlanThreads) map_out[k] = map_in ? map_in[k] : k;
+      return;
+    }
+  }
   int j = sums[tid] - n;
   for (int i = i0; i < i1; ++i) {
     const int b = phase[i] & PH_MASK;
@@ -1237,11 +1275,19 @@ __global__ __launch_bounds__(kPlanThreads) void compact_plan_kernel(const uint8_
 // a row sits in word p / PW, bits 16 * (p % PW) up) from their source slots; empty slots get 0.
 // A workgroup covers kCompactEdges edges of one destination tile.
 constexpr int kCompactEdges = 16;
+// With ctl: nothing unless ctl->go, destination tiles past the live frames are skipped, and a
+// null src_of means slot j's source is slot j (the copy back from scratch).
+__device__ static inline int compact_source(const int32_t* src_of, const CompactCtl* ctl, int64_t j) {
+  if (ctl && j >= ctl->live) return -1;
+  return src_of ? src_of[j] : (int)j;
+}
+
 template <class F>
 __global__ __launch_bounds__(kBlock) void compact_msg_kernel(DevCode c, const uint32_t* src, uint32_t* dst,
-                                                             const int32_t* src_of) {
+                                                             const int32_t* src_of, const CompactCtl* ctl) {
   constexpr int TW = F::TW, RW = F::RW, PW = F::PW;
   const int tb = blockIdx.y;
+  if (ctl && (!ctl->go || (int64_t)tb * TW >= ctl->live)) return;
   const int e_begin = blockIdx.x * kCompactEdges;
   for (int i = threadIdx.x; i < kCompactEdges * RW; i += kBlock) {
     const int e = e_begin + i / RW, w = i % RW;
@@ -1249,7 +1295,7 @@ __global__ __launch_bounds__(kBlock) void compact_msg_kernel(DevCode c, const ui
     uint32_t word = 0;
 #pragma unroll
     for (int h = 0; h < PW; ++h) {
-      const int from = src_of[(int64_t)tb * TW + w * PW + h];
+      const int from = compact_source(src_of, ctl, (int64_t)tb * TW + w * PW + h);
       if (from >= 0) {
         const int ta = from / TW, pa = from % TW;
         const uint32_t sw = src[((int64_t)ta * c.E + e) * RW + pa / PW];
@@ -1264,13 +1310,14 @@ __global__ __launch_bounds__(kBlock) void compact_msg_kernel(DevCode c, const ui
 // Channel values [tile][N][TW]: kCompactEdges variables of one destination tile per workgroup.
 template <int TW>
 __global__ __launch_bounds__(kBlock) void compact_L_kernel(DevCode c, const float* src, float* dst,
-                                                           const int32_t* src_of) {
+                                                           const int32_t* src_of, const CompactCtl* ctl) {
   const int tb = blockIdx.y;
+  if (ctl && (!ctl->go || (int64_t)tb * TW >= ctl->live)) return;
   const int v_begin = blockIdx.x * kCompactEdges;
   for (int i = threadIdx.x; i < kCompactEdges * TW; i += kBlock) {
     const int v = v_begin + i / TW, p = i % TW;
     if (v >= c.N) break;
-    const int from = src_of[(int64_t)tb * TW + p];
+    const int from = compact_source(src_of, ctl, (int64_t)tb * TW + p);
     float x = 0.0f;
     if (from >= 0) x = src[((int64_t)(from / TW) * c.N + v) * TW + from % TW];
     dst[((int64_t)tb * c.N + v) * TW + p] = x;
@@ -1279,10 +1326,11 @@ __global__ __launch_bounds__(kBlock) void compact_L_kernel(DevCode c, const floa
 
 // Phases and iteration counts of the destination slots (empty slots: PH_FREE).
 __global__ void compact_slots_kernel(const uint8_t* src_phase, const int32_t* src_iter, uint8_t* dst_phase,
-                                     int32_t* dst_iter, const int32_t* src_of, int32_t dst_slots) {
+                                     int32_t* dst_iter, const int32_t* src_of, int32_t dst_slots,
+                                     const CompactCtl* ctl) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= dst_slots) return;
-  const int from = src_of[j];
+  if (j >= dst_slots || (ctl && !ctl->go)) return;
+  const int from = compact_source(src_of, ctl, j);
   dst_phase[j] = from >= 0 ? src_phase[from] : (uint8_t)PH_FREE;
   dst_iter[j] = from >= 0 ? src_iter[from] : 0;
 }
@@ -1290,8 +1338,9 @@ __global__ void compact_slots_kernel(const uint8_t* src_phase, const int32_t* sr
 // Tile flags of the destination region from its phases (one wavefront per tile), as the
 // control kernel leaves them at the end of a step; fail masks cleared.
 template <int FPL>
-__global__ __launch_bounds__(kWave) void retile_kernel(DevState s) {
+__global__ __launch_bounds__(kWave) void retile_kernel(DevState s, const CompactCtl* ctl) {
   constexpr int TW = kWave * FPL;
+  if (ctl && !ctl->go) return;
   const int t = blockIdx.x;
   const int lane = threadIdx.x;
   bool any_live = false, any_run = false, all_run = true;
@@ -1571,33 +1620,48 @@ static hipError_t launch_state_in(const DevCode& c, const DevState& s, const voi
 
 template <class F>
 static hipError_t launch_results(const DevCode& c, const DevState& s, const DevOut& out, int64_t frames,
-                                 int64_t frame0, const BatchResults& r, bool dbl, const int32_t* map, hipStream_t st) {
+                                 int64_t frame0, const BatchResults& r, bool dbl, const int32_t* map,
+                                 const CompactCtl* ctl, hipStream_t st) {
   constexpr int FPL = F::FPL, TW = F::TW;
   const int64_t n = map ? (int64_t)s.tiles * TW : frames;   // slots of the region, or frames
   if (n <= 0) return hipSuccess;
   if (r.msg_bits)
     hipLaunchKernelGGL((unpack_hard_kernel<FPL>), dim3((c.K + kBlock - 1) / kBlock, (unsigned)n), dim3(kBlock), 0, st,
-                       out.hard, c.K, frames, r.msg_bits, frame0, map);
+                       out.hard, c.K, frames, r.msg_bits, frame0, map, ctl);
   if (r.iterations)
     hipLaunchKernelGGL(iters_out_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, s.iter, n, frames,
-                       r.iterations, frame0, map);
+                       r.iterations, frame0, map, ctl);
   auto to_frames = [&](const float* in, int64_t X, int64_t count, void* dst) {
     dim3 grid((unsigned)((count + 63) / 64), s.tiles);
     if (dbl) hipLaunchKernelGGL((tiles_to_frames_kernel<double, TW>), grid, dim3(kBlock), 0, st, in, X, count,
-                                frames, (double*)dst, frame0, map);
+                                frames, (double*)dst, frame0, map, ctl);
     else hipLaunchKernelGGL((tiles_to_frames_kernel<float, TW>), grid, dim3(kBlock), 0, st, in, X, count,
-                            frames, (float*)dst, frame0, map);
+                            frames, (float*)dst, frame0, map, ctl);
   };
   if (r.msg) to_frames(out.app, c.K, c.K, r.msg);
   if (r.parity) to_frames(out.ext, c.N, c.N, r.parity);
   if (r.syst) to_frames(out.ext, c.N, c.K, r.syst);
   if (r.state) {
     if (dbl) hipLaunchKernelGGL((state_out_kernel<double, F>), dim3((c.E + 63) / 64, s.tiles), dim3(kBlock), 0, st,
-                                c, s, frames, (double*)r.state, frame0, map);
+                                c, s, frames, (double*)r.state, frame0, map, ctl);
     else hipLaunchKernelGGL((state_out_kernel<float, F>), dim3((c.E + 63) / 64, s.tiles), dim3(kBlock), 0, st,
-                            c, s, frames, (float*)r.state, frame0, map);
+                            c, s, frames, (float*)r.state, frame0, map, ctl);
   }
   return hipGetLastError();
+}
+
+// Moves the frames of slots src_of[j] of `from` into slots j of `to` (ctl: see compact_source).
+template <class F>
+static void launch_move(const DevCode& c, const DevState& from, const DevState& to, const int32_t* src_of,
+                        const CompactCtl* ctl, hipStream_t st) {
+  constexpr int TW = F::TW;
+  const int32_t dst_slots = to.tiles * TW;
+  hipLaunchKernelGGL((compact_msg_kernel<F>), dim3((c.E + kCompactEdges - 1) / kCompactEdges, to.tiles), dim3(kBlock),
+                     0, st, c, from.msg, to.msg, src_of, ctl);
+  hipLaunchKernelGGL((compact_L_kernel<TW>), dim3((c.N + kCompactEdges - 1) / kCompactEdges, to.tiles), dim3(kBlock),
+                     0, st, c, from.L, to.L, src_of, ctl);
+  hipLaunchKernelGGL(compact_slots_kernel, dim3((dst_slots + 255) / 256), dim3(256), 0, st, from.phase, from.iter,
+                     to.phase, to.iter, src_of, dst_slots, ctl);
 }
 
 template <class F>
@@ -1606,14 +1670,19 @@ static hipError_t launch_compact(const DevCode& c, const DevState& from, const D
   constexpr int FPL = F::FPL, TW = F::TW;
   const int32_t slots = from.tiles * TW, dst_slots = to.tiles * TW;
   hipLaunchKernelGGL(compact_plan_kernel, dim3(1), dim3(kPlanThreads), 0, st, from.phase, map_in, slots, src_of,
-                     map_out, dst_slots);
-  hipLaunchKernelGGL((compact_msg_kernel<F>), dim3((c.E + kCompactEdges - 1) / kCompactEdges, to.tiles), dim3(kBlock),
-                     0, st, c, from.msg, to.msg, src_of);
-  hipLaunchKernelGGL((compact_L_kernel<TW>), dim3((c.N + kCompactEdges - 1) / kCompactEdges, to.tiles), dim3(kBlock),
-                     0, st, c, from.L, to.L, src_of);
-  hipLaunchKernelGGL(compact_slots_kernel, dim3((dst_slots + 255) / 256), dim3(256), 0, st, from.phase, from.iter,
-                     to.phase, to.iter, src_of, dst_slots);
-  hipLaunchKernelGGL((retile_kernel<FPL>), dim3(to.tiles), dim3(kWave), 0, st, to);
+                     map_out, dst_slots, (CompactCtl*)nullptr, TW, 0);
+  launch_move<F>(c, from, to, src_of, nullptr, st);
+  hipLaunchKernelGGL((retile_kernel<FPL>), dim3(to.tiles), dim3(kWave), 0, st, to, (const CompactCtl*)nullptr);
+  return hipGetLastError();
+}
+
+template <class F>
+static hipError_t launch_compact_in_place(const DevCode& c, const DevState& s, const DevState& scratch,
+                                          const int32_t* src_of, const CompactCtl* ctl, hipStream_t st) {
+  constexpr int FPL = F::FPL;
+  launch_move<F>(c, s, scratch, src_of, ctl, st);    // live frames, in slot order, into scratch
+  launch_move<F>(c, scratch, s, nullptr, ctl, st);   // back into the region's first slots
+  hipLaunchKernelGGL((retile_kernel<FPL>), dim3(s.tiles), dim3(kWave), 0, st, s, ctl);
   return hipGetLastError();
 }
 
@@ -1718,14 +1787,30 @@ hipError_t kernels_extract(const Layout& lay, const DevCode& c, const DevState& 
 }
 
 hipError_t kernels_results(const Layout& lay, const DevCode& c, const DevState& s, const DevOut& out, int64_t frames,
-                           int64_t frame0, const BatchResults& r, bool dbl, const int32_t* map, hipStream_t st) {
-  FECL_DISPATCH(lay, launch_results<F>(c, s, out, frames, frame0, r, dbl, map, st));
+                           int64_t frame0, const BatchResults& r, bool dbl, const int32_t* map, const CompactCtl* ctl,
+                           hipStream_t st) {
+  FECL_DISPATCH(lay, launch_results<F>(c, s, out, frames, frame0, r, dbl, map, ctl, st));
 }
 
 hipError_t kernels_compact(const Layout& lay, const DevCode& c, const DevState& from, const DevState& to,
                            const int32_t* map_in, int32_t* map_out, int32_t* src_of, hipStream_t st) {
   if (from.tiles <= 0 || to.tiles <= 0 || (int64_t)from.tiles * lay.tw() > INT32_MAX) return hipErrorInvalidValue;
   FECL_DISPATCH(lay, launch_compact<F>(c, from, to, map_in, map_out, src_of, st));
+}
+
+hipError_t kernels_compact_plan(const Layout& lay, const DevState& s, const int32_t* map_in, int32_t* map_out,
+                                int32_t* src_of, CompactCtl* ctl, int32_t remaining, hipStream_t st) {
+  const int64_t slots = (int64_t)s.tiles * lay.tw();
+  if (s.tiles <= 0 || slots > INT32_MAX) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(compact_plan_kernel, dim3(1), dim3(kPlanThreads), 0, st, s.phase, map_in, (int32_t)slots, src_of,
+                     map_out, (int32_t)slots, ctl, (int32_t)lay.tw(), remaining);
+  return hipGetLastError();
+}
+
+hipError_t kernels_compact_in_place(const Layout& lay, const DevCode& c, const DevState& s, const DevState& scratch,
+                                    const int32_t* src_of, const CompactCtl* ctl, hipStream_t st) {
+  if (s.tiles <= 0 || scratch.tiles < s.tiles) return hipErrorInvalidValue;
+  FECL_DISPATCH(lay, launch_compact_in_place<F>(c, s, scratch, src_of, ctl, st));
 }
 
 }  // namespace fecl

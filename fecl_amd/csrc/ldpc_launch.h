@@ -67,8 +67,10 @@ hipError_t kernels_extract(const Layout& lay, const DevCode& c, const DevState& 
                            const int32_t* slots, int n, uint8_t* bits, int32_t* iters, hipStream_t st);
 // Results of a region of tiles; map[j] is the frame of slot j (-1: none), or null for the
 // batch's first region, whose slot j holds frame j.
+// ctl (nullable): the results are written only if ctl->go.
 hipError_t kernels_results(const Layout& lay, const DevCode& c, const DevState& s, const DevOut& out, int64_t frames,
-                           int64_t frame0, const BatchResults& r, bool dbl, const int32_t* map, hipStream_t st);
+                           int64_t frame0, const BatchResults& r, bool dbl, const int32_t* map, const CompactCtl* ctl,
+                           hipStream_t st);
 // Moves the live frames of region `from` (slot j holds frame map_in[j], or j when map_in is
 // null) in slot order into the first slots of region `to`, which must have room for them:
 // message rows, channel values, phases and iteration counts; sets map_out (the frames of
@@ -76,5 +78,15 @@ hipError_t kernels_results(const Layout& lay, const DevCode& c, const DevState& 
 // entries.  Call at a step boundary (after the control kernel), without the fused syndrome.
 hipError_t kernels_compact(const Layout& lay, const DevCode& c, const DevState& from, const DevState& to,
                            const int32_t* map_in, int32_t* map_out, int32_t* src_of, hipStream_t st);
+// Compaction decided on the device (captured graphs), in place in one region: the plan sets
+// ctl (go, live) from the phases, the steps left (`remaining`) and the tiles holding live
+// frames, and writes map_out (a copy of map_in when it declines) and src_of (s.tiles * TW
+// entries each).  Between the plan and the move the caller hands the region's finished
+// frames' results over with the same ctl.  The move gathers the live frames into `scratch`
+// (as many tiles as s) and copies them back into s's first slots, then resets s's tile flags.
+hipError_t kernels_compact_plan(const Layout& lay, const DevState& s, const int32_t* map_in, int32_t* map_out,
+                                int32_t* src_of, CompactCtl* ctl, int32_t remaining, hipStream_t st);
+hipError_t kernels_compact_in_place(const Layout& lay, const DevCode& c, const DevState& s, const DevState& scratch,
+                                    const int32_t* src_of, const CompactCtl* ctl, hipStream_t st);
 
 }  // namespace fecl

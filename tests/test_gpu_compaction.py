@@ -129,3 +129,55 @@ def test_device_path_dvbs2_compaction_on_and_off():
     ref = oracle_run(oc, llr[sample].astype(np.float64), F.Approximate, 50, fmt=O.MSG_F16, threads=16)
     assert np.array_equal(res[0][1][sample], ref["iters"])
     assert np.array_equal(res[0][0].reshape(frames, oc.K)[sample], ref["hard"])
+
+
+def graph_compactions(iters, tw, iterations):
+    """In-place compactions a captured decode makes: after every sixth step (while 4 steps
+    remain) the device counts the frames still live (c > s+1 updates) and the tiles holding
+    them, and moves them into the first slots when that frees a tile and the freed tile-steps
+    outweigh two moves of 1.5 frame-iterations per live frame."""
+    slots = list(range(len(iters)))
+    n = 0
+    for step in range(5, iterations, 6):
+        if step + 4 >= iterations:
+            break
+        live = [f for f in slots if f >= 0 and iters[f] > step + 1]
+        tiles_live = len({j // tw for j, f in enumerate(slots) if f >= 0 and iters[f] > step + 1})
+        freed = tiles_live - (len(live) + tw - 1) // tw
+        if freed > 0 and 2 * freed * (iterations - 1 - step) * tw > 6 * len(live):
+            slots, n = live + [-1] * (len(slots) - len(live)), n + 1
+    return n
+
+
+def test_graph_compacts_on_the_device():
+    """One hipGraph over two batches that converge at staggered iterations (FP32 and FP16
+    messages, 64 and 128 frames per tile): the device decides at every sixth step and compacts
+    in place.  Replayed with new inputs copied in eagerly between launches, every replay's bits
+    and iteration counts equal the oracle's and the eager decode's."""
+    import torch
+    oc, H = gallager()
+    specs = [(F.MsgFormat.F32, O.MSG_F32, 1024), (F.MsgFormat.F16, O.MSG_F16, 768)]
+    decs = [gpu_decoder(H, F.Approximate, ITERS, fmt=fmt, words=1) for fmt, _, _ in specs]
+    eager = [gpu_decoder(H, F.Approximate, ITERS, fmt=fmt, words=1) for fmt, _, _ in specs]
+    tensors = [torch.zeros(frames * oc.N, dtype=torch.float32, device="cuda") for _, _, frames in specs]
+    g = F.BatchGraph(decs, tensors)
+    for rep in range(3):
+        refs = []
+        for i, (fmt, ofmt, frames) in enumerate(specs):
+            llr = staggered_llr(oc, frames, 80 + 10 * rep + i)
+            tensors[i].copy_(torch.from_numpy(llr.astype(np.float32).ravel()))
+            refs.append(oracle_run(oc, llr, F.Approximate, ITERS, fmt=ofmt, want=("hard", "iters"), threads=16))
+            n = graph_compactions(refs[-1]["iters"], decs[i].layout()[1], ITERS)
+            print(f"replay {rep} batch {i}: {frames} frames, in-place compactions {n}", flush=True)
+            assert n >= 1
+        torch.cuda.synchronize()
+        g.launch()
+        torch.cuda.synchronize()
+        for i, (fmt, ofmt, frames) in enumerate(specs):
+            got = g.msg[i].cpu().numpy().reshape(frames, oc.K)
+            assert np.array_equal(got, refs[i]["hard"]), (rep, i)
+            assert np.array_equal(g.iterations[i].cpu().numpy(), refs[i]["iters"]), (rep, i)
+            bits, it = eager[i].decode(tensors[i], iterations=True)
+            torch.cuda.synchronize()
+            assert np.array_equal(bits.cpu().numpy().reshape(frames, oc.K), got), (rep, i)
+            assert np.array_equal(it.cpu().numpy(), refs[i]["iters"]), (rep, i)
