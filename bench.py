@@ -416,6 +416,78 @@ def run_agreement(args, device):
     return out
 
 
+def run_plain(args, device):
+    """The reference's own CPU-runnable case: DVB-S2 n=64800 r1/2, 16 frames at Eb/N0 = 1.0 dB,
+    FP64 channel LLRs, Exact, 50 max iterations with syndrome early termination.  The GPU takes
+    the LLRs through FeCl's decode hook (FP64 in host memory, bits out, PCIe included) and, for
+    scale, from device memory; stock FeCl decodes the same LLRs on the host cores.  Agreement:
+    hard decisions and iteration counts against FeCl's FP64 decoder (the oracle's bit-identical
+    restatement), APP error against the exact box-plus value."""
+    import numpy as np
+    import torch
+    import fecl_amd as F
+    from oracle import oracle as O
+    frames, iters, ebn0, reps = 16, 50, 1.0, 20
+    threads = min(16, os.cpu_count() or 1)
+    H = F.DvbS2.matrix(64800, 0.5)
+    dec = F.Ldpc(H, F.DecoderOptions().algorithm(F.Exact).iterations(iters))
+    K, N = dec.msgSize(), dec.paritySize()
+    rng = np.random.default_rng(args.seed + 21)
+    info = rng.integers(0, 2, (frames, K), dtype=np.uint8)
+    cw = dec.encode(info).reshape(frames, N)
+    sigma2 = 1.0 / (2.0 * (K / N) * 10.0 ** (ebn0 / 10.0))
+    llr = -2.0 * ((1.0 - 2.0 * cw) + rng.normal(0.0, np.sqrt(sigma2), cw.shape)) / sigma2
+    bits = dec.decode(llr)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        bits = dec.decode(llr)
+    host_s = (time.perf_counter() - t0) / reps
+    x = torch.from_numpy(llr.astype(np.float32)).to(device)
+    dec.decode(x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        dbits, dit = dec.decode(x, iterations=True)
+    torch.cuda.synchronize()
+    dev_s = (time.perf_counter() - t0) / reps
+    out = {"metric": "decoded info Gb/s, DVB-S2 r1/2 Exact, 16 frames of FP64 LLRs (the reference's CPU case)",
+           "workload": f"DVB-S2 n=64800 r1/2, {frames} frames, Eb/N0 {ebn0} dB, Exact, {iters} max iterations, "
+                       "FP64 LLRs from host memory",
+           "gpu_host_path_Gb_per_s": round(frames * K / host_s / 1e9, 4), "gpu_host_path_ms": round(host_s * 1e3, 3),
+           "gpu_device_resident_Gb_per_s": round(frames * K / dev_s / 1e9, 4),
+           "fer_gpu": float(np.mean(np.any(np.asarray(bits).reshape(frames, K) != info, axis=1)))}
+    cpu = None
+    exe = os.path.join(ROOT, "oracle", "_ref", "fecl_bench")
+    if os.path.exists(exe):
+        with tempfile.TemporaryDirectory() as tmp:
+            path = os.path.join(tmp, "llr.bin")
+            with open(path, "wb") as f:
+                np.array(llr.shape, np.int64).tofile(f)
+                llr.astype(np.float64).tofile(f)
+            try:
+                r = subprocess.run([exe, "dvbs2:64800:0.5", path, "0", str(iters), str(threads)],
+                                   capture_output=True, text=True, timeout=600, check=True)
+                cpu = {"seconds": json.loads(r.stdout.strip().splitlines()[-1])["seconds"], "kind": "reference"}
+            except Exception as e:
+                print(f"[bench] stock FeCl unavailable ({e}); using the C restatement", file=sys.stderr, flush=True)
+    code = O.dvbs2(64800, 1, 2)
+    t0 = time.perf_counter()
+    ref = O.decode(code, llr, O.EXACT, iters, prec=64, want=("hard", "iters", "msg"), threads=threads)
+    if cpu is None:
+        cpu = {"seconds": time.perf_counter() - t0, "kind": "port"}
+    out["cpu_baseline"] = {"value": round(frames * K / cpu["seconds"] / 1e9, 6), "unit": "Gb/s", "cores": threads,
+                           "kind": cpu["kind"], "seconds": round(cpu["seconds"], 3), "cpu": _cpu_model()}
+    exact = O.decode(code, llr.astype(np.float32).astype(np.float64), O.EXACT_STABLE, iters, prec=64,
+                     want=("msg",), threads=threads)["msg"]
+    app = np.asarray(dec.soDecode(llr, outputs=("msg",))["msg"]).reshape(frames, K)
+    out["agreement"] = {
+        "hard_identical_to_fecl_fp64": bool(np.array_equal(np.asarray(bits).reshape(frames, K), ref["hard"])),
+        "iterations_identical_to_fecl_fp64": bool(np.array_equal(dit.cpu().numpy(), ref["iters"])),
+        "iterations": [int(v) for v in ref["iters"]],
+        "app_max_rel_error_vs_exact_box_plus": float((np.abs(app - exact) / np.maximum(1.0, np.abs(exact))).max())}
+    return out
+
+
 def copy_reference(device, gib=1, reps=5):
     """A plain device-to-device copy of gib GiB (the PMC byte-counter calibration kernel)."""
     import torch
@@ -528,7 +600,8 @@ def main():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--words", type=int, default=-1,
                     help="lane layout: 1, 2, 4 words per lane per edge; 0 automatic; -1 autotune")
-    ap.add_argument("--config", choices=["batch", "uneven", "serving", "precision", "agreement", "calibrate", "copy"],
+    ap.add_argument("--config", choices=["batch", "uneven", "serving", "precision", "agreement", "calibrate", "copy",
+                                         "plain"],
                     default="batch")
     ap.add_argument("--precision-frames", type=int, default=4096)
     ap.add_argument("--agreement-frames", type=int, default=1024)
@@ -575,6 +648,9 @@ def main():
         return
     if args.config == "copy":
         print(json.dumps(copy_reference(device)), flush=True)
+        return
+    if args.config == "plain":
+        print(json.dumps(run_plain(args, device)), flush=True)
         return
     alg = {"minsum": F.Approximate, "linear": F.Linear, "exact": F.Exact}[args.alg]
     fmt = {"f32": F.MsgFormat.F32, "f16": F.MsgFormat.F16, "bf16": F.MsgFormat.BF16}[args.msg]
