@@ -261,7 +261,17 @@ int prepare(fecl_ldpc* h, Workspace* ws, const Layout& lay, int64_t frames, bool
 // that finish early make the remaining steps cheap no-ops) and nothing waits on the host.
 int run_batch(fecl_ldpc* h, Workspace* ws, hipStream_t st, const BatchIO& io, int64_t frames, int64_t frame0,
               bool capture) {
-  const Layout lay = layout_of(h);
+  // A small eager batch with the automatic layout takes fewer words per lane when that cuts its
+  // padded slots (whole tiles) by more than 10 %: every iteration moves whole tiles, so 16
+  // frames in a 256-frame tile would move 16 times the bytes they need.  Results do not depend
+  // on the layout.  Pinned layouts (set_layout, autotune) and captured graphs, whose
+  // workspaces are sized before capture, keep the handle's layout.
+  Layout lay = layout_of(h);
+  if (!capture && !h->words_pinned) {
+    auto padded = [&](int w) { const int64_t tw = 64LL * w * lay.pw(); return (frames + tw - 1) / tw * tw; };
+    while (lay.words > 1 && padded(lay.words / 2) * 11 < padded(lay.words) * 10) lay.words /= 2;
+    if (!vpw_fits(lay, h->code.max_var_degree)) lay.vpw = 1;
+  }
   const Code& c = h->code;
   const bool soft = io.res.msg || io.res.syst || io.res.parity;
   DevState s;
