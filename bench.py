@@ -752,6 +752,7 @@ def main():
                    "frames_per_step": args.frames, "iterations": args.iterations, "algorithm": args.alg,
                    "msg_format": args.msg, "code": "dvbs2 64800 1/2",
                    "lane_words": words, "frames_per_tile": frames_per_tile, "vars_per_wave": dec.varsPerWave(),
+                   "compaction": args.compaction,
                    "autotune": tune,
                    "pass_frames": args.pass_frames},
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
