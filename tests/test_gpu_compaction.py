@@ -181,3 +181,48 @@ def test_graph_compacts_on_the_device():
             torch.cuda.synchronize()
             assert np.array_equal(bits.cpu().numpy().reshape(frames, oc.K), got), (rep, i)
             assert np.array_equal(it.cpu().numpy(), refs[i]["iters"]), (rep, i)
+
+
+@pytest.mark.parametrize("alg", [F.Linear, F.Exact])
+def test_graph_compaction_linear_exact_equal_eager(alg):
+    """Linear and Exact with scaling factor 0.75 in one hipGraph (in-place compaction on the
+    device) against eager decodes (compaction between regions on the host's counts): bits and
+    iteration counts equal on two replays."""
+    import torch
+    oc, H = gallager()
+    frames = 1024
+    dec = gpu_decoder(H, alg, ITERS, sf=0.75, words=1)
+    eager = gpu_decoder(H, alg, ITERS, sf=0.75, words=1)
+    x = torch.zeros(frames * oc.N, dtype=torch.float32, device="cuda")
+    g = F.BatchGraph([dec], [x])
+    for rep in range(2):
+        llr = staggered_llr(oc, frames, 90 + rep)
+        x.copy_(torch.from_numpy(llr.astype(np.float32).ravel()))
+        torch.cuda.synchronize()
+        g.launch()
+        torch.cuda.synchronize()
+        bits, it = eager.decode(x, iterations=True)
+        torch.cuda.synchronize()
+        n = graph_compactions(it.cpu().numpy(), 64, ITERS)
+        print(f"alg={alg} replay {rep}: in-place compactions {n}", flush=True)
+        assert n >= 1
+        assert np.array_equal(g.msg[0].cpu().numpy(), bits.cpu().numpy()), rep
+        assert np.array_equal(g.iterations[0].cpu().numpy(), it.cpu().numpy()), rep
+
+
+def test_short_frame_compaction_matches_the_oracle():
+    """DVB-S2 n=16200 r1/2, 512 frames mixing converging and non-converging Eb/N0 over 8 tiles
+    of 64: soDecode outputs bit-identical to the oracle through the compactions."""
+    from tests.test_gpu_decoder import dvbs2
+    oc, H = dvbs2(16200, (1, 2))
+    rng = np.random.default_rng(95)
+    parts = [dvbs2_llr(n, e, seed=96 + i, n=16200) for i, (n, e) in enumerate(((200, 3.0), (150, 2.2), (100, 1.8), (62, -2.0)))]
+    llr = np.concatenate(parts)
+    llr = llr[rng.permutation(len(llr))].astype(np.float32).astype(np.float64)
+    frames = llr.shape[0]
+    ref = oracle_run(oc, llr, F.Approximate, ITERS, threads=16)
+    n = compactions(ref["iters"], 64, ITERS)
+    print(f"short frames: compactions {n}", flush=True)
+    assert n >= 1
+    dec = gpu_decoder(H, F.Approximate, ITERS, words=1)
+    assert_bit_exact(dec.soDecode(llr), ref, frames, oc.K, oc.N, oc.E)
