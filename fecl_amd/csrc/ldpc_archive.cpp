@@ -89,9 +89,14 @@ Csr get_matrix(Reader& r, int64_t* cols, bool first, bool* rowidx_seen) {
     *rowidx_seen = true;
   }
   Csr m;
+  // FeCl's rows are consecutive ranges of the element list ({begin, end} with begin = the
+  // previous row's end, the first at 0), so the expanded matrix never exceeds nnz entries
+  uint64_t next = 0;
   for (uint64_t i = 0; i < rows; ++i) {
     const uint64_t b = r.get<uint64_t>(), e = r.get<uint64_t>();
     if (b > e || e > nnz) throw std::runtime_error("row index outside the matrix");
+    if (b != next) throw std::runtime_error("row ranges are not consecutive");
+    next = e;
     for (uint64_t k = b; k < e; ++k) {
       if (el[k] >= (uint64_t)*cols) throw std::runtime_error("column index outside the matrix");
       m.idx.push_back((int32_t)el[k]);

@@ -260,7 +260,7 @@ int prepare(fecl_ldpc* h, Workspace* ws, const Layout& lay, int64_t frames, bool
 // capture: the launches are being recorded into a hipGraph, so every step is enqueued (frames
 // that finish early make the remaining steps cheap no-ops) and nothing waits on the host.
 int run_batch(fecl_ldpc* h, Workspace* ws, hipStream_t st, const BatchIO& io, int64_t frames, int64_t frame0,
-              bool capture) {
+              bool capture, const DevOpts* base) {
   // A small eager batch with the automatic layout takes fewer words per lane when that cuts its
   // padded slots (whole tiles) by more than 10 %: every iteration moves whole tiles, so 16
   // frames in a 256-frame tile would move 16 times the bytes they need.  Results do not depend
@@ -283,7 +283,7 @@ int run_batch(fecl_ldpc* h, Workspace* ws, hipStream_t st, const BatchIO& io, in
     HIP_TRY(hipHostMalloc((void**)&ws->host_active, (size_t)h->iterations * 4, hipHostMallocDefault));
     ws->host_active_cap = h->iterations;
   }
-  DevOpts o = make_opts(h);
+  DevOpts o = base ? *base : make_opts(h);
   o.nt_rows = nt_rows_for(h, lay, s.tiles);
   const bool prof = h->profiling && !capture;
   if (prof) {
@@ -402,12 +402,24 @@ int run_batch(fecl_ldpc* h, Workspace* ws, hipStream_t st, const BatchIO& io, in
 }
 
 int run_passes(fecl_ldpc* h, Workspace* ws, hipStream_t st, const BatchIO& io, int64_t frames, int64_t frame0,
-               bool capture) {
+               bool capture, const DevOpts* base) {
   const int64_t tw = layout_of(h).tw();
   int64_t pass = h->pass_frames > 0 ? h->pass_frames : kMaxPassFrames;
   pass = std::max<int64_t>(tw, (pass + tw - 1) / tw * tw);   // as pass_tiles
   for (int64_t f = 0; f < frames; f += pass)
-    if (int rc = run_batch(h, ws, st, io, std::min(pass, frames - f), frame0 + f, capture)) return rc;
+    if (int rc = run_batch(h, ws, st, io, std::min(pass, frames - f), frame0 + f, capture, base)) return rc;
+  return FECL_OK;
+}
+
+int copy_opts(const fecl_ldpc* h, OptsCopy* out) {
+  out->opts = make_opts(h);
+  const size_t n = h->sf.size();
+  HIP_TRY(out->sf.ensure(n * sizeof(double)));
+  HIP_TRY(out->sf32.ensure(n * sizeof(float)));
+  HIP_TRY(hipMemcpy(out->sf.p, h->d_sf.p, n * sizeof(double), hipMemcpyDeviceToDevice));
+  HIP_TRY(hipMemcpy(out->sf32.p, h->d_sf32.p, n * sizeof(float), hipMemcpyDeviceToDevice));
+  out->opts.sf = out->sf.as<double>();
+  out->opts.sf32 = out->sf32.as<float>();
   return FECL_OK;
 }
 
@@ -457,6 +469,21 @@ void parallel_ranges(size_t n, Body body) {
 void to_float(const double* src, float* dst, size_t n) {
   parallel_ranges(n, [=](size_t a, size_t b) {
     for (size_t i = a; i < b; ++i) dst[i] = (float)src[i];
+  });
+}
+
+// A-priori inputs on the first K bits (soDecode): dst[f][v] = (float)((double)(float)llr[f][v]
+// + syst[f][v]) for v < K, (float)llr[f][v] otherwise.  FeCl's decoder instantiated with a
+// float LLR type stores the channel value as float, then adds the double a-priori value and
+// rounds once (src/detail/BpDecoder/BpDecoderImpl.cpp:75-80); rounding syst to float first
+// would round twice.
+void to_float_plus(const double* llr, const double* syst, float* dst, int64_t frames, int64_t N, int64_t K) {
+  parallel_ranges((size_t)frames * N, [=](size_t a, size_t b) {
+    for (size_t i = a; i < b; ++i) {
+      const size_t f = i / (size_t)N, v = i % (size_t)N;
+      const float x = (float)llr[i];
+      dst[i] = v < (size_t)K ? (float)((double)x + syst[f * (size_t)K + v]) : x;
+    }
   });
 }
 
@@ -520,10 +547,6 @@ int host_decode(fecl_ldpc* h, const double* llr, const double* syst_in, const do
   for (int b = 0; b < 2; ++b) {
     HIP_TRY(ws->stage_in[b].ensure((size_t)chunk * c.N * 4));
     HIP_TRY(ws->pin_in[b].ensure((size_t)chunk * c.N * 4));
-    if (syst_in) {
-      HIP_TRY(ws->stage_syst[b].ensure((size_t)chunk * c.K * 4));
-      HIP_TRY(ws->pin_syst[b].ensure((size_t)chunk * c.K * 4));
-    }
     if (state_in) {
       HIP_TRY(ws->stage_state[b].ensure((size_t)chunk * c.E * 4));
       HIP_TRY(ws->pin_state[b].ensure((size_t)chunk * c.E * 4));
@@ -547,13 +570,14 @@ int host_decode(fecl_ldpc* h, const double* llr, const double* syst_in, const do
       const int b = (int)(k & 1);
       const int64_t f0 = start(k), n = size(k);
       if (k >= 2) HIP_TRY(hipEventSynchronize(ws->copied[b]));
-      to_float(llr + f0 * c.N, ws->pin_in[b].as<float>(), (size_t)n * c.N);
-      if (syst_in) to_float(syst_in + f0 * c.K, ws->pin_syst[b].as<float>(), (size_t)n * c.K);
+      // the a-priori values are added here, in double, and the sum rounded once
+      if (syst_in)
+        to_float_plus(llr + f0 * c.N, syst_in + f0 * c.K, ws->pin_in[b].as<float>(), n, c.N, c.K);
+      else
+        to_float(llr + f0 * c.N, ws->pin_in[b].as<float>(), (size_t)n * c.N);
       if (state_in) to_float(state_in + f0 * c.E, ws->pin_state[b].as<float>(), (size_t)n * c.E);
       if (k >= 2) HIP_TRY(hipStreamWaitEvent(cs, ws->used[b], 0));
       HIP_TRY(hipMemcpyAsync(ws->stage_in[b].p, ws->pin_in[b].p, (size_t)n * c.N * 4, hipMemcpyHostToDevice, cs));
-      if (syst_in)
-        HIP_TRY(hipMemcpyAsync(ws->stage_syst[b].p, ws->pin_syst[b].p, (size_t)n * c.K * 4, hipMemcpyHostToDevice, cs));
       if (state_in)
         HIP_TRY(hipMemcpyAsync(ws->stage_state[b].p, ws->pin_state[b].p, (size_t)n * c.E * 4, hipMemcpyHostToDevice, cs));
       HIP_TRY(hipEventRecord(ws->copied[b], cs));
@@ -613,7 +637,7 @@ int host_decode(fecl_ldpc* h, const double* llr, const double* syst_in, const do
     char* ob = ws->stage_out[b].as<char>();
     BatchIO io{};
     io.llr = ws->stage_in[b].p;
-    io.syst = syst_in ? ws->stage_syst[b].p : nullptr;
+    io.syst = nullptr;   // already added to the channel values on the host
     io.state = state_in ? ws->stage_state[b].p : nullptr;
     io.dbl_in = false;
     io.dbl_out = false;
@@ -1303,6 +1327,7 @@ int fecl_ldpc_autotune(fecl_ldpc* h, int64_t frames, int32_t probe_iterations, i
 struct fecl_ldpc_graph {
   std::vector<fecl_ldpc*> handles;
   std::vector<std::unique_ptr<Workspace>> ws;
+  std::vector<std::unique_ptr<OptsCopy>> opts;   // each branch's scaling tables, as at creation
   std::vector<hipStream_t> streams;
   std::vector<hipEvent_t> events;
   hipStream_t capture = nullptr;
@@ -1345,6 +1370,9 @@ int fecl_ldpc_graph_create(fecl_ldpc* const* handles, const int64_t* frames, int
     if (h->compaction)
       if (int rc = prepare_graph_compaction(h, ws.get(), frames[i])) return rc;
     g->ws.push_back(std::move(ws));
+    std::unique_ptr<OptsCopy> oc(new OptsCopy());
+    if (int rc = copy_opts(h, oc.get())) return rc;
+    g->opts.push_back(std::move(oc));
     hipStream_t st = nullptr;
     HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
     g->streams.push_back(st);
@@ -1362,7 +1390,7 @@ int fecl_ldpc_graph_create(fecl_ldpc* const* handles, const int64_t* frames, int
     io.llr = llr[i];
     io.res.msg_bits = msg[i];
     io.res.iterations = iterations ? iterations[i] : nullptr;
-    rc = run_passes(handles[i], g->ws[i].get(), g->streams[i], io, frames[i], 0, /*capture=*/true);
+    rc = run_passes(handles[i], g->ws[i].get(), g->streams[i], io, frames[i], 0, /*capture=*/true, &g->opts[i]->opts);
     if (rc != FECL_OK) break;
     e = hipEventRecord(g->events[i + 1], g->streams[i]);
     if (e == hipSuccess) e = hipStreamWaitEvent(g->capture, g->events[i + 1], 0);

@@ -86,8 +86,8 @@ struct Workspace {
   // alternates per checkpoint, and ctl holds the device's decision (CompactCtl).
   Area spare;
   DevBuf map[2], src_of, ctl;
-  DevBuf stage_in[2], stage_syst[2], stage_state[2], stage_out[2];   // host path, double-buffered
-  HostBuf pin_in[2], pin_syst[2], pin_state[2], pin_out[2];          // host path: FP32, pinned
+  DevBuf stage_in[2], stage_state[2], stage_out[2];   // host path, double-buffered
+  HostBuf pin_in[2], pin_state[2], pin_out[2];          // host path: FP32, pinned
   int32_t* host_active = nullptr;
   int host_active_cap = 0;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -100,12 +100,12 @@ struct Workspace {
   hipEvent_t decoded[2] = {nullptr, nullptr}, drained[2] = {nullptr, nullptr};
   ~Workspace() {
     for (DevBuf* b : {&msg, &L, &hard, &phase, &iter, &fail, &flags, &active, &out_hard, &out_ext, &out_app,
-                      &stage_in[0], &stage_in[1], &stage_syst[0], &stage_syst[1], &stage_state[0], &stage_state[1],
+                      &stage_in[0], &stage_in[1], &stage_state[0], &stage_state[1],
                       &stage_out[0], &stage_out[1]})
       b->release();
     spare.release();
     for (DevBuf* b : {&map[0], &map[1], &src_of, &ctl}) b->release();
-    for (HostBuf* b : {&pin_in[0], &pin_in[1], &pin_syst[0], &pin_syst[1], &pin_state[0], &pin_state[1],
+    for (HostBuf* b : {&pin_in[0], &pin_in[1], &pin_state[0], &pin_state[1],
                        &pin_out[0], &pin_out[1]})
       b->release();
     if (host_active) (void)hipHostFree(host_active);
@@ -178,6 +178,19 @@ struct BatchIO {
   bool dbl_out;
 };
 
+// The handle's kernel options with the scaling tables in device memory of its own, for users
+// that keep running on the options a handle had when they were made (captured graphs, the
+// server): fecl_ldpc_set_options rewrites or reallocates the handle's tables.
+struct OptsCopy {
+  DevBuf sf, sf32;
+  DevOpts opts{};
+  ~OptsCopy() {
+    sf.release();
+    sf32.release();
+  }
+};
+int copy_opts(const fecl_ldpc* h, OptsCopy* out);
+
 int check_handle(const fecl_ldpc* h);
 // DevOpts::nt_rows for a batch of `tiles` tiles of h's layout: the handle's policy, or with
 // the automatic policy non-temporal rows in both kernels once the batch's rows and channel
@@ -199,12 +212,13 @@ constexpr int kGraphCompactEvery = 6;
 // Sizes ws's in-place compaction buffers for a captured decode of `frames` frames (before
 // capture: nothing may allocate while a graph is recorded).
 int prepare_graph_compaction(fecl_ldpc* h, Workspace* ws, int64_t frames);
-// Decodes frames [frame0, frame0 + frames) of io in one pass (see ldpc_capi.cpp).
+// Decodes frames [frame0, frame0 + frames) of io in one pass (see ldpc_capi.cpp).  base
+// (nullable): kernel options to use instead of make_opts(h) (their scaling tables).
 int run_batch(fecl_ldpc* h, Workspace* ws, hipStream_t st, const BatchIO& io, int64_t frames, int64_t frame0,
-              bool capture);
+              bool capture, const DevOpts* base = nullptr);
 // Decodes frames [frame0, frame0 + frames) of io in consecutive passes of h->pass_frames.
 int run_passes(fecl_ldpc* h, Workspace* ws, hipStream_t st, const BatchIO& io, int64_t frames, int64_t frame0,
-               bool capture);
+               bool capture, const DevOpts* base = nullptr);
 
 }  // namespace engine
 }  // namespace fecl

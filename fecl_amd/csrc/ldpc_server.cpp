@@ -64,6 +64,8 @@ struct PinnedBuf {
 struct Pool {
   fecl_ldpc* h = nullptr;
   Layout lay;
+  OptsCopy oc;              // the handle's options at creation, scaling tables included
+  int32_t iterations = 1;
   DevOpts opts{};
   Workspace ws;
   DevState s{};
@@ -222,7 +224,7 @@ int enqueue_round(fecl_ldpc_server* sv, Events* ev, int64_t e, const std::vector
       int64_t n = 0;
       for (const Admit& a : admits[pi]) n += a.n;
       if (!n) continue;
-      const uint8_t init = (uint8_t)((P.h->iterations == 1 ? PH_FINAL : PH_RUN) | PH_FRESH);
+      const uint8_t init = (uint8_t)((P.iterations == 1 ? PH_FINAL : PH_RUN) | PH_FRESH);
       HIP_TRY(kernels_admit(P.lay, P.h->dc, P.s, P.d_stage[b].as<float>(), P.d_admit[b].as<int32_t>(), (int)n, init,
                             sv->st));
     }
@@ -234,7 +236,7 @@ int enqueue_round(fecl_ldpc_server* sv, Events* ev, int64_t e, const std::vector
     HIP_TRY(kernels_check(P.lay, P.h->dc, P.s, P.opts, P.h->code.max_row_degree, sv->st));
     HIP_TRY(kernels_variable(P.lay, P.h->dc, P.s, P.out, P.h->code.max_var_degree, P.opts.fused_syn,
                              (P.opts.nt_rows & 2) != 0, sv->st));
-    HIP_TRY(kernels_syndrome_control(P.lay, P.h->dc, P.s, P.h->iterations, P.opts.fused_syn, -1,
+    HIP_TRY(kernels_syndrome_control(P.lay, P.h->dc, P.s, P.iterations, P.opts.fused_syn, -1,
                                      P.h->code.max_row_degree, sv->st));
   }
   for (auto& pp : sv->pools) {   // every pool, so that each round's phases are complete
@@ -413,7 +415,9 @@ int fecl_ldpc_server_create(fecl_ldpc* const* handles, int32_t count, const int6
     std::unique_ptr<Pool> P(new Pool());
     P->h = h;
     P->lay = layout_of(h);
-    P->opts = make_opts(h);
+    if (int rc = copy_opts(h, &P->oc)) return rc;
+    P->opts = P->oc.opts;
+    P->iterations = h->iterations;
     const int64_t tw = P->lay.tw();
     int64_t want = (slots && slots[i] > 0) ? slots[i] : (memory_bytes / unset) / slot_bytes(h, P->lay);
     int64_t tiles = std::max<int64_t>(1, want / tw);

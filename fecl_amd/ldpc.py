@@ -650,10 +650,14 @@ class BatchGraph:
         if len(decoders) != len(llr) or not decoders:
             raise ValueError("one LLR tensor per decoder")
         self._decoders = list(decoders)
-        self.llr = [x.contiguous().view(-1) for x in llr]
-        for x in self.llr:
+        for x in llr:
             if not x.is_cuda or x.dtype != torch.float32:
                 raise ValueError("LLRs must be CUDA float32 tensors")
+            # the graph reads the tensor's memory at every launch: a private contiguous copy
+            # would never see the caller's later writes
+            if not x.is_contiguous():
+                raise ValueError("LLR tensors must be contiguous (the graph re-reads their memory)")
+        self.llr = [x.view(-1) for x in llr]
         self.frames = [d._frames(x.numel()) for d, x in zip(self._decoders, self.llr)]
         self.msg = [torch.empty(f * d.msgSize(), dtype=torch.uint8, device=x.device)
                     for f, d, x in zip(self.frames, self._decoders, self.llr)]

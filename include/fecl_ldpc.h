@@ -215,7 +215,12 @@ int fecl_ldpc_set_pass_frames(fecl_ldpc* h, int64_t frames);
  * of frames works until its last frame finishes, so once frames converge most tiles carry a
  * few live frames.  With compaction on (default), an eager decode moves the live frames into
  * three quarters as many tiles or fewer whenever they fit and enough iterations remain, and
- * later iterations run on those tiles only; the batch's workspace grows by up to 75 %.  Hipgraph-captured decodes and the fused syndrome never compact. */
+ * later iterations run on those tiles only; the batch's workspace grows by up to 75 %.
+ * Captured decodes (fecl_ldpc_graph_create) compact in place instead: after every sixth
+ * step, while at least four remain, a device kernel decides whether the live frames fit in
+ * fewer tiles and moves them into the region's first tiles through a scratch area; such a
+ * graph holds one more area of the pass's tiles (message rows, channel values, per-slot
+ * state) plus three slot-sized maps.  The fused syndrome never compacts. */
 int fecl_ldpc_set_compaction(fecl_ldpc* h, int32_t enable);
 /* Cache hints for the message rows (no counterpart in the reference; results are identical
  * for any value): bit 0 = non-temporal loads/stores in the check kernel, bit 1 = in the
@@ -245,8 +250,8 @@ int fecl_ldpc_graph_destroy(fecl_ldpc_graph* g);
  * slot's footprint.  A worker thread repeats: admit queued frames into free slots, run one
  * check/variable/syndrome step on every pool holding live frames, retire finished frames.
  * Requests arriving while others run join at the next step; each frame's result equals a
- * batch decode of it (frames share no state).  Handles must outlive the server and keep
- * their options while it runs.
+ * batch decode of it (frames share no state).  Handles must outlive the server; each pool
+ * decodes with the options (and layout) its handle had when the server was created.
  *   submit: host LLRs [frames][N] (FP32) of code `code`; msg [frames][K] (and iterations
  *           [frames], nullable) receive the results; all three stay valid until wait()
  *           reports done.

@@ -361,6 +361,16 @@ def test_archive_load_rejects_damaged_bytes():
     bad_size[97] = 0x1f            # paritySize_ no longer the matrix's column count
     with pytest.raises(ValueError, match="sizes"):
         F.archive_load(bytes(bad_size))
+    # rows are consecutive {begin, end} ranges of the element list: a row that starts anywhere
+    # else (here row 1 spans rows 0 and 1, so the ranges overlap) is rejected, which also caps
+    # the expanded matrix at the stored element count
+    rows = np.array([0, 5, 5, 10], dtype="<u8").tobytes()
+    at = data.find(rows)
+    assert at > 0 and data.count(rows) == 1
+    overlap = bytearray(data)
+    overlap[at + 16:at + 24] = np.array([0], dtype="<u8").tobytes()
+    with pytest.raises(ValueError, match="consecutive"):
+        F.archive_load(bytes(overlap))
     # random byte changes: each archive is either rejected or read into a consistent structure
     rng = np.random.default_rng(8)
     for _ in range(1500):

@@ -272,6 +272,28 @@ def test_sodecode_state_and_syst_inputs(words):
     assert_bit_exact(out, ref, 2, oc.K, oc.N, oc.E)
 
 
+def test_host_sodecode_adds_fp64_apriori_once():
+    """FeCl's decoder with a float LLR type stores the channel value as float, adds the double
+    a-priori value and rounds once (src/detail/BpDecoder/BpDecoderImpl.cpp:75-80).  The host
+    soDecode path gets FP64 syst values that are not FP32-representable, among them 2^-24 +
+    2^-50 on a channel value of 1.0 (one rounding gives 1 + 2^-23, two roundings give 1.0):
+    every output equals the oracle's, which adds them the same way."""
+    oc, H = gallager()
+    llr = gallager_llr(oc.N, oc.K, 3, 1.0, 14).astype(np.float32).astype(np.float64)
+    syst = np.random.default_rng(97).normal(0, 1, (3, oc.K))
+    llr[0, :8] = 1.0
+    syst[0, :8] = 2.0 ** -24 + 2.0 ** -50
+    assert np.any(syst.astype(np.float32).astype(np.float64) != syst)
+    for iters in (1, 6):
+        ref = oracle_run(oc, llr, F.Approximate, iters, syst_in=syst)
+        out = gpu_decoder(H, F.Approximate, iters).soDecode(llr, syst=syst)
+        assert_bit_exact(out, ref, 3, oc.K, oc.N, oc.E)
+        # the first 8 a-posteriori values with one iteration carry 1 + 2^-23, not 1.0
+        if iters == 1:
+            twice = oracle_run(oc, llr, F.Approximate, 1, syst_in=syst.astype(np.float32).astype(np.float64))
+            assert not np.array_equal(twice["msg"][0, :8], ref["msg"][0, :8])
+
+
 def test_host_buffer_path_in_several_chunks():
     """FeCl's hooks pass host FP64 buffers: 1300 frames go through device staging in chunks of
     512 whose input copies overlap the previous chunk's decode; with a-priori state and
@@ -363,6 +385,34 @@ def test_batch_graph_multirate_replays():
             eager, it = decs[i].decode(tensors[i], iterations=True)
             torch.cuda.synchronize()
             assert np.array_equal(eager.cpu().numpy().reshape(frames, K), got)
+
+
+def test_batch_graph_keeps_its_options_after_set_options():
+    """A captured graph decodes with the options its handles had at creation: changing a
+    handle's options afterwards (here from one scaling row to a per-iteration table, which
+    reallocates the handle's device table) leaves every replay equal to the oracle run with the
+    original options, while eager decodes on the handle follow the new ones."""
+    import torch
+    oc, H = dvbs2(16200, (1, 2))
+    frames = 70
+    dec = gpu_decoder(H, F.Approximate, 20, sf=0.75)
+    x = torch.zeros(frames * oc.N, dtype=torch.float32, device="cuda")
+    g = F.BatchGraph([dec], [x])
+    new_sf = [0.5, 0.625, 0.75, 0.875, 1.0] * 4
+    dec.setDecoderOptions(F.DecoderOptions().algorithm(F.Approximate).iterations(20).scalingFactor({0: new_sf}))
+    for rep in range(2):
+        llr = dvbs2_llr(frames, 2.0 + 0.2 * rep, seed=60 + rep, rate=(1, 2), n=16200).astype(np.float32)
+        x.copy_(torch.from_numpy(llr.ravel()))
+        torch.cuda.synchronize()
+        g.launch()
+        torch.cuda.synchronize()
+        ref = oracle_run(oc, llr.astype(np.float64), F.Approximate, 20, sf=0.75, want=("hard", "iters"))
+        assert np.array_equal(g.msg[0].cpu().numpy().reshape(frames, oc.K), ref["hard"]), rep
+        assert np.array_equal(g.iterations[0].cpu().numpy(), ref["iters"]), rep
+        now = oracle_run(oc, llr.astype(np.float64), F.Approximate, 20, sf=new_sf,
+                         want=("hard", "iters"))
+        _, it = dec.decode(llr.astype(np.float64), iterations=True)
+        assert np.array_equal(it, now["iters"]), rep
 
 
 def test_device_encoder_matches_host_for_all_dvbs2_codes():
