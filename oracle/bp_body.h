@@ -21,9 +21,15 @@ static REAL FN(lin_y)[8];
 static void FN(lin_init)(void) {
   for (int i = 0; i < 8; ++i) FN(lin_y)[i] = (REAL)log(1.0 + exp(-((double)(REAL)i / (double)(REAL)2.0)));
 }
+/* Which side of the table's cutoff (x >= 7, where f drops from y[7] = log(1 + e^-3.5) to 0)
+ * every lookup of the current frame fell on, folded into a 64-bit FNV-1a hash in lookup order
+ * (a diagnostic: two decodes of a frame that run the same lookups agree on every side iff
+ * their hashes agree, up to hash collisions). */
+static __thread uint64_t FN(lin_side);
 /* Piecewise-linear log(1+exp(-x)): x *= 2; 0 beyond the last knot; interpolate. */
 static inline REAL FN(lin_f)(REAL x) {
   x *= (REAL)2.0;
+  FN(lin_side) = (FN(lin_side) ^ (uint64_t)(x >= (REAL)7)) * 0x100000001b3ull;
   if (x >= (REAL)7) return 0;
   size_t i = (size_t)x;
   return (FN(lin_y)[i + 1] - FN(lin_y)[i]) * (x - (REAL)i) + FN(lin_y)[i];
@@ -134,6 +140,7 @@ typedef struct {
   const FN(code_t)* cd;
   const double* llr; const double* syst_in; const double* state_in;
   uint8_t* hard; double* msg; double* parity; double* syst; double* state; int32_t* iters;
+  uint64_t* lin_side;
   int64_t f0, f1;
 } FN(job_t);
 
@@ -146,6 +153,7 @@ static void* FN(run_frames)(void* arg) {
   REAL* c = (REAL*)malloc(sizeof(REAL) * E);
   REAL* buf = (REAL*)malloc(sizeof(REAL) * E);
   for (int64_t f = jb->f0; f < jb->f1; ++f) {
+    FN(lin_side) = 0xcbf29ce484222325ull;
     const double* llr = jb->llr + f * N;
     for (int64_t v = 0; v < N; ++v) L[v] = (REAL)llr[v];
     if (jb->syst_in)
@@ -166,6 +174,7 @@ static void* FN(run_frames)(void* arg) {
     }
     FN(check_update)(cd, cd->iterations - 1, c, buf);
     if (jb->iters) jb->iters[f] = count;
+    if (jb->lin_side) jb->lin_side[f] = FN(lin_side);
     if (jb->hard) {  /* decode output: app = L, then += c ascending (BpDecoderImpl.cpp:63-69) */
       for (int64_t v = 0; v < N; ++v) app[v] = L[v];
       for (int64_t e = 0; e < E; ++e) app[cd->col[e]] += c[e];
@@ -187,7 +196,7 @@ static void* FN(run_frames)(void* arg) {
 static int FN(decode)(const FN(code_t)* cd, const double* llr, int64_t frames,
                       const double* syst_in, const double* state_in,
                       uint8_t* hard, double* msg, double* parity, double* syst, double* state,
-                      int32_t* iters, int nthreads) {
+                      int32_t* iters, uint64_t* lin_side, int nthreads) {
   FN(lin_init)();
   if (nthreads < 1) nthreads = 1;
   if (nthreads > 256) nthreads = 256;
@@ -198,7 +207,7 @@ static int FN(decode)(const FN(code_t)* cd, const double* llr, int64_t frames,
   int64_t step = (frames + nthreads - 1) / nthreads;
   for (int i = 0; i < nthreads; ++i) {
     int64_t f0 = i * step, f1 = (i + 1) * step < frames ? (i + 1) * step : frames;
-    FN(job_t) jb = {cd, llr, syst_in, state_in, hard, msg, parity, syst, state, iters, f0, f1 > f0 ? f1 : f0};
+    FN(job_t) jb = {cd, llr, syst_in, state_in, hard, msg, parity, syst, state, iters, lin_side, f0, f1 > f0 ? f1 : f0};
     jobs[i] = jb;
   }
   /* chunks 1.. on worker threads, chunk 0 on the caller (src/Codec.h:221-237 fans out alike) */

@@ -135,6 +135,16 @@ int orc_bp_decode(const int32_t* row_ptr, const int32_t* col, int64_t M, int64_t
                   const double* syst_in, const double* state_in,
                   uint8_t* hard, double* msg, double* parity, double* syst, double* state,
                   int32_t* iters, int nthreads) {
+  return orc_bp_decode_diag(row_ptr, col, M, N, K, llr, frames, alg, iterations, sf, sf_rows, sf_cols, prec,
+                            msg_fmt, syst_in, state_in, hard, msg, parity, syst, state, iters, NULL, nthreads);
+}
+
+int orc_bp_decode_diag(const int32_t* row_ptr, const int32_t* col, int64_t M, int64_t N, int64_t K,
+                       const double* llr, int64_t frames, int alg, int iterations,
+                       const double* sf, int sf_rows, int sf_cols, int prec, int msg_fmt,
+                       const double* syst_in, const double* state_in,
+                       uint8_t* hard, double* msg, double* parity, double* syst, double* state,
+                       int32_t* iters, uint64_t* lin_side, int nthreads) {
   if (iterations < 1 || frames < 0 || sf_rows < 1 || alg < 0 || alg > 3) return -1;
   if (prec != 32 && prec != 64) return -1;
   if (prec == 64 && msg_fmt != ORC_MSG_F32) return -1;
@@ -146,8 +156,10 @@ int orc_bp_decode(const int32_t* row_ptr, const int32_t* col, int64_t M, int64_t
   const int64_t E = row_ptr[M];
   if (prec == 64) {
     code_t_f64 cd = {row_ptr, col, M, N, K, E, alg, iterations, msg_fmt, sf, sf_rows, sf_cols};
-    return decode_f64(&cd, llr, frames, syst_in, state_in, hard, msg, parity, syst, state, iters, nthreads);
+    return decode_f64(&cd, llr, frames, syst_in, state_in, hard, msg, parity, syst, state, iters, lin_side,
+                      nthreads);
   }
   code_t_f32 cd = {row_ptr, col, M, N, K, E, alg, iterations, msg_fmt, sf, sf_rows, sf_cols};
-  return decode_f32(&cd, llr, frames, syst_in, state_in, hard, msg, parity, syst, state, iters, nthreads);
+  return decode_f32(&cd, llr, frames, syst_in, state_in, hard, msg, parity, syst, state, iters, lin_side,
+                    nthreads);
 }

@@ -64,6 +64,19 @@ int orc_bp_decode(const int32_t* row_ptr, const int32_t* col, int64_t M, int64_t
                   uint8_t* hard, double* msg, double* parity, double* syst, double* state,
                   int32_t* iters, int nthreads);
 
+/* orc_bp_decode plus a diagnostic of the Linear rule (alg ORC_LINEAR): lin_side[f]
+ * (nullable) receives a 64-bit FNV-1a hash of the side of the table's cutoff (x >= 7, i.e.
+ * |a +- b| >= 3.5, where FeCl's correction drops from 0.0298 to 0) that each table lookup of
+ * frame f fell on, in lookup order.  A float and a double decode of a frame that take the same
+ * number of iterations make the same lookups in the same order, so their hashes differ exactly
+ * when some lookup fell on different sides (up to hash collisions). */
+int orc_bp_decode_diag(const int32_t* row_ptr, const int32_t* col, int64_t M, int64_t N, int64_t K,
+                       const double* llr, int64_t frames, int alg, int iterations,
+                       const double* sf, int sf_rows, int sf_cols, int prec, int msg_fmt,
+                       const double* syst_in, const double* state_in,
+                       uint8_t* hard, double* msg, double* parity, double* syst, double* state,
+                       int32_t* iters, uint64_t* lin_side, int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -32,6 +32,8 @@ def lib():
         L.orc_quantize.restype = ctypes.c_float
         L.orc_bp_decode.argtypes = [P, P, i64, i64, i64, P, i64, i32, i32, P, i32, i32, i32, i32,
                                     P, P, P, P, P, P, P, P, i32]
+        L.orc_bp_decode_diag.argtypes = [P, P, i64, i64, i64, P, i64, i32, i32, P, i32, i32, i32, i32,
+                                         P, P, P, P, P, P, P, P, P, i32]
         _LIB = L
     return _LIB
 
@@ -109,16 +111,18 @@ def decode(code, llr, alg, iterations, sf=None, prec=64, msg_fmt=MSG_F32, syst_i
         out["state"] = np.zeros((F, code.E))
     if "iters" in want:
         out["iters"] = np.zeros(F, np.int32)
+    if "lin_side" in want:   # Linear only: hash of the cutoff side of every table lookup
+        out["lin_side"] = np.zeros(F, np.uint64)
     if syst_in is not None:
         syst_in = np.ascontiguousarray(syst_in, dtype=np.float64).reshape(F, code.K)
     if state_in is not None:
         state_in = np.ascontiguousarray(state_in, dtype=np.float64).reshape(F, code.E)
     threads = threads or min(16, os.cpu_count() or 1)
-    rc = lib().orc_bp_decode(_p(code.row_ptr), _p(code.col), code.M, code.N, code.K, _p(llr), F,
-                             alg, iterations, _p(sf), sf.shape[0], sf.shape[1], prec, msg_fmt,
-                             _p(syst_in), _p(state_in), _p(out.get("hard")), _p(out.get("msg")),
-                             _p(out.get("parity")), _p(out.get("syst")), _p(out.get("state")),
-                             _p(out.get("iters")), threads)
+    rc = lib().orc_bp_decode_diag(_p(code.row_ptr), _p(code.col), code.M, code.N, code.K, _p(llr), F,
+                                  alg, iterations, _p(sf), sf.shape[0], sf.shape[1], prec, msg_fmt,
+                                  _p(syst_in), _p(state_in), _p(out.get("hard")), _p(out.get("msg")),
+                                  _p(out.get("parity")), _p(out.get("syst")), _p(out.get("state")),
+                                  _p(out.get("iters")), _p(out.get("lin_side")), threads)
     if rc != 0:
         raise ValueError("oracle rejected the arguments")
     return out

@@ -193,3 +193,39 @@ def test_rejects_bad_arguments():
     bad = O.Code(np.array([0, 1], np.int32), np.array([0], np.int32), 2, 1)
     with pytest.raises(ValueError):
         O.decode(bad, np.zeros((1, 2)), O.APPROXIMATE, 5)
+
+
+def _fnv_sides(sides):
+    h = 0xcbf29ce484222325
+    for s in sides:
+        h = ((h ^ int(s)) * 0x100000001b3) & 0xFFFFFFFFFFFFFFFF
+    return h
+
+
+def test_linear_cutoff_side_hash():
+    """The Linear diagnostic (lookup-side hash): on a degree-3 row with one iteration, the check
+    update makes six table lookups in FeCl's order, at |x0 + x1|, |x0 - x1| (forward), then
+    |x0 + x2|, |x0 - x2| and |x2 + x1|, |x2 - x1| (backward); each lookup records whether
+    2|a +- b| >= 7.  The hash equals FNV-1a over those six bits computed here, in both
+    precisions; an input that is not FP32-representable and sits just below the cutoff in FP64
+    (x1 = 2.5 - 2^-40, so x0 + x1 = 3.5 - 2^-40) rounds onto it in FP32 (2.5), which flips the
+    first lookup's side and so the hash."""
+    code = single_row(3)
+
+    def side_hash(x, prec):
+        out = O.decode(code, np.asarray(x, np.float64)[None, :], O.LINEAR, 1, prec=prec, want=("lin_side",))
+        return int(out["lin_side"][0])
+
+    def expected(x0, x1, x2):
+        pairs = [(x0, x1), (x0, x2), (x2, x1)]
+        sides = [2.0 * abs(s) >= 7.0 for a, b in pairs for s in (a + b, a - b)]
+        return _fnv_sides(sides)
+
+    for x in ([1.0, 2.5, -0.5], [1.0, 2.0, 4.0], [-3.0, 0.25, 6.5], [0.1, 0.2, 0.3]):
+        assert side_hash(x, 64) == side_hash(x, 32) == expected(*x)
+    x = [1.0, 2.5 - 2.0 ** -40, -0.5]
+    assert side_hash(x, 64) == expected(*x)
+    assert side_hash(x, 32) == expected(1.0, 2.5, -0.5) != side_hash(x, 64)
+    # other algorithms make no lookups: the hash stays at FNV's offset basis
+    out = O.decode(code, np.asarray([1.0, 2.0, 3.0])[None, :], O.APPROXIMATE, 1, want=("lin_side",))
+    assert int(out["lin_side"][0]) == 0xcbf29ce484222325
