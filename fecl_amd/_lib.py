@@ -38,6 +38,7 @@ SIGNATURES = {
     "fecl_ldpc_sodecode": [P, P, P, P, I64, P, P, P, P, P],
     "fecl_ldpc_decode_device": [P, P, I64, P, P, P],
     "fecl_ldpc_sodecode_device": [P, P, P, P, I64, P, P, P, P, P, P],
+    "fecl_ldpc_decode_device_noalloc": [P, P, I64, P, P, P],
     "fecl_ldpc_profile": [P, I32],
     "fecl_ldpc_profile_read": [P, P, P],
     "fecl_ldpc_set_layout": [P, I32],

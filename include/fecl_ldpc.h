@@ -178,6 +178,20 @@ int fecl_ldpc_sodecode_device(fecl_ldpc* h, const float* parity_in, const float*
                               float* syst_out, float* parity_out, float* state_out,
                               int32_t* iterations, void* stream);
 
+/* fecl_ldpc_decode_device without allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device or host memory during the call (code recorded into a
+ * hipGraph, serving loops that forbid allocation).  The batch's workspace is a fixed 128 MiB
+ * array compiled into the library (one per device, held from the first call until the process
+ * ends); the call makes no hipMalloc/hipFree/hipHostMalloc and creates no stream or event.
+ * Same results as fecl_ldpc_decode_device.  It runs every one of the handle's `iterations`
+ * steps (no host reads the live count; steps after the last frame finishes return at once),
+ * with one 32-bit word per lane per edge (64-frame tiles for FP32 messages, 128 for 16-bit),
+ * no compaction and no passes: a batch whose workspace exceeds the array is refused with
+ * FECL_INVALID_ARGUMENT (DVB-S2 n = 64800: 64 frames with FP32 messages, 128 with 16-bit).
+ * Calls that overlap on one device (other streams or threads) race on the array. */
+int fecl_ldpc_decode_device_noalloc(fecl_ldpc* h, const float* llr, int64_t frames, uint8_t* msg,
+                                    int32_t* iterations, void* stream);
+
 /* Kernel timing (no counterpart in the reference; instrumentation for benchmarks).
  * While enabled, each decode records HIP events on its launch stream around the check-node,
  * variable-node and syndrome/control launches of every step and waits for the batch to finish.
