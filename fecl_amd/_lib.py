@@ -52,6 +52,9 @@ SIGNATURES = {
     "fecl_ldpc_autotune": [P, I64, I32, P, P],
     "fecl_ldpc_graph_create": [P, P, I32, P, P, P, P],
     "fecl_ldpc_graph_launch": [P, P],
+    "fecl_ldpc_graph_check": [P],
+    "fecl_ldpc_graph_is_persistent": [P, P],
+    "fecl_ldpc_set_graph_persistent": [P, I32],
     "fecl_ldpc_graph_destroy": [P],
     "fecl_ldpc_server_create": [P, I32, P, I64, P],
     "fecl_ldpc_server_slots": [P, I32, P],

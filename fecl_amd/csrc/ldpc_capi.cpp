@@ -1333,12 +1333,14 @@ struct fecl_ldpc_graph {
   hipStream_t capture = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  PersistBatch* persist = nullptr;   // the persistent form's workspaces and control block
   ~fecl_ldpc_graph() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     for (auto e : events) (void)hipEventDestroy(e);
     for (auto st : streams) (void)hipStreamDestroy(st);
     if (capture) (void)hipStreamDestroy(capture);
+    if (persist) persist_destroy(persist);
   }
 };
 
@@ -1356,6 +1358,31 @@ int fecl_ldpc_graph_create(fecl_ldpc* const* handles, const int64_t* frames, int
     if (handles[i]->device != dev) return fail(FECL_INVALID_ARGUMENT, "all codes must live on one device");
   }
   HIP_TRY(hipStreamCreateWithFlags(&g->capture, hipStreamNonBlocking));
+  Layout common;
+  if (persist_eligible(handles, count, &common)) {
+    // one persistent kernel for every code (ldpc_persist.hip), preceded by the counter reset
+    // and each code's ingest
+    std::vector<DevOpts> opts;
+    for (int32_t i = 0; i < count; ++i) {
+      g->handles.push_back(handles[i]);
+      std::unique_ptr<OptsCopy> oc(new OptsCopy());
+      if (int rc = copy_opts(handles[i], oc.get())) return rc;
+      opts.push_back(oc->opts);
+      g->opts.push_back(std::move(oc));
+    }
+    if (int rc = persist_create(handles, frames, count, llr, msg, iterations, common, opts.data(), &g->persist))
+      return rc;
+    HIP_TRY(hipStreamBeginCapture(g->capture, hipStreamCaptureModeThreadLocal));
+    const int rc = persist_enqueue(g->persist, handles, g->capture);
+    hipGraph_t graph = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(g->capture, &graph);
+    g->graph = graph;
+    if (rc != FECL_OK) return rc;
+    HIP_TRY(e2);
+    HIP_TRY(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+    *out = g.release();
+    return FECL_OK;
+  }
   hipEvent_t fork = nullptr;
   HIP_TRY(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
   g->events.push_back(fork);
@@ -1409,6 +1436,28 @@ int fecl_ldpc_graph_create(fecl_ldpc* const* handles, const int64_t* frames, int
 int fecl_ldpc_graph_launch(fecl_ldpc_graph* g, void* stream) {
   if (!g) return fail(FECL_INVALID_ARGUMENT, "null graph");
   HIP_TRY(hipGraphLaunch(g->exec, (hipStream_t)stream));
+  return FECL_OK;
+}
+
+int fecl_ldpc_graph_check(fecl_ldpc_graph* g) {
+  if (!g) return fail(FECL_INVALID_ARGUMENT, "null graph");
+  if (!g->persist) return FECL_OK;
+  int32_t error = 0;
+  if (int rc = persist_error(g->persist, &error)) return rc;
+  if (error) return fail(FECL_DEVICE_ERROR, "persistent decode stopped on its watchdog");
+  return FECL_OK;
+}
+
+int fecl_ldpc_graph_is_persistent(const fecl_ldpc_graph* g, int32_t* persistent) {
+  if (!g || !persistent) return fail(FECL_INVALID_ARGUMENT, "invalid arguments");
+  *persistent = g->persist ? 1 : 0;
+  return FECL_OK;
+}
+
+int fecl_ldpc_set_graph_persistent(fecl_ldpc* h, int32_t enable) {
+  if (int rc = check_handle(h)) return rc;
+  std::lock_guard<std::mutex> lk(h->mu);
+  h->persistent = enable != 0;
   return FECL_OK;
 }
 

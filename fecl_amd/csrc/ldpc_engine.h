@@ -152,6 +152,11 @@ struct fecl_ldpc {
   // Compaction of the live frames into fewer tiles once a quarter of a region's tiles could go
   // (eager decodes without the fused syndrome; see run_batch).
   bool compaction = true;
+  // Captured multi-code graphs run as one persistent kernel (ldpc_persist.hip) when every
+  // handle of the graph asks for it and is eligible; otherwise (default) as recorded per-step
+  // launches, which are faster on the seven-rate batch (the persistent kernel holds every
+  // phase's registers at once, 2 waves per SIMD).
+  bool persistent = false;
   int32_t cache_policy = -1; // message-row cache hints: -1 by working set, else DevOpts::nt_rows bits
   std::mutex mu;
   std::vector<std::unique_ptr<fecl::engine::Workspace>> pool;
@@ -219,6 +224,19 @@ int run_batch(fecl_ldpc* h, Workspace* ws, hipStream_t st, const BatchIO& io, in
 // Decodes frames [frame0, frame0 + frames) of io in consecutive passes of h->pass_frames.
 int run_passes(fecl_ldpc* h, Workspace* ws, hipStream_t st, const BatchIO& io, int64_t frames, int64_t frame0,
                bool capture, const DevOpts* base = nullptr);
+
+// Persistent multi-code decode (ldpc_persist.hip).  eligible: whether these handles can share
+// one persistent launch, and their common layout.  create: sizes the batch's workspaces and
+// control block (before capture).  enqueue: one decode (counter reset, ingests, the kernel),
+// capturable.  error: whether the last finished launch ended on its watchdog.
+struct PersistBatch;
+bool persist_eligible(fecl_ldpc* const* handles, int32_t count, Layout* common);
+int persist_create(fecl_ldpc* const* handles, const int64_t* frames, int32_t count, const float* const* llr,
+                   uint8_t* const* msg, int32_t* const* iterations, const Layout& lay, const DevOpts* opts,
+                   PersistBatch** out);
+int persist_enqueue(PersistBatch* b, fecl_ldpc* const* handles, hipStream_t st);
+int persist_error(PersistBatch* b, int32_t* error);
+void persist_destroy(PersistBatch* b);
 
 }  // namespace engine
 }  // namespace fecl

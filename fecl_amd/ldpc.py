@@ -392,6 +392,11 @@ class Ldpc:
         """Move live frames into fewer tiles as frames converge (default on); same results."""
         check(lib().fecl_ldpc_set_compaction(self._h, int(bool(enable))))
 
+    def setGraphPersistent(self, enable):
+        """Whether BatchGraphs made with this decoder may use the persistent kernel (default
+        off; used when every decoder of the graph asks for it and is eligible); same results."""
+        check(lib().fecl_ldpc_set_graph_persistent(self._h, int(bool(enable))))
+
     def layout(self):
         """(words per lane, frames per tile) of the current device layout."""
         w, tw = ctypes.c_int32(), ctypes.c_int32()
@@ -677,6 +682,18 @@ class BatchGraph:
         import torch
         st = stream if stream is not None else torch.cuda.current_stream(self.llr[0].device).cuda_stream
         check(lib().fecl_ldpc_graph_launch(self._g, ctypes.c_void_p(st)))
+
+    @property
+    def persistent(self):
+        """True when the graph runs the persistent multi-code kernel."""
+        v = ctypes.c_int32()
+        check(lib().fecl_ldpc_graph_is_persistent(self._g, ctypes.byref(v)))
+        return bool(v.value)
+
+    def check(self):
+        """Raises if the last finished launch of a persistent graph stopped on its watchdog
+        (call after synchronising the launch stream)."""
+        check(lib().fecl_ldpc_graph_check(self._g))
 
     def __del__(self):
         g = getattr(self, "_g", None)

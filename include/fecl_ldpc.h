@@ -247,16 +247,31 @@ int fecl_ldpc_autotune(fecl_ldpc* h, int64_t frames, int32_t probe_iterations, i
 /* Multi-code batch in one launch (no counterpart in the reference: FeCl decodes one code per
  * call).  Captures, for each i < count, a full decode of frames[i] frames of code handles[i]
  * (device LLRs llr[i] [frames[i]][N_i] FP32 -> device msg[i] [frames[i]][K_i] bytes and
- * iterations[i] [frames[i]], nullable) into one hipGraph whose branches run concurrently.
- * Each launch decodes whatever the input buffers hold at that time.  Every step of every
- * branch is recorded; frames that finish early leave the remaining steps as cheap no-ops.
- * The options (and layout) of each handle are those at creation time. */
+ * iterations[i] [frames[i]], nullable) into one hipGraph.  Each launch decodes whatever the
+ * input buffers hold at that time.  The options (and layout) of each handle are those at
+ * creation time.
+ *
+ * Two forms, with the same results.  Persistent (chosen when every handle asked for it with
+ * fecl_ldpc_set_graph_persistent and all are min-sum without scaling factors in one message
+ * format and lane layout): a counter reset, each code's ingest, and ONE persistent
+ * kernel whose workgroups take check, variable and syndrome items of all codes from one work
+ * queue, with per-tile counters in place of kernel boundaries; converged tiles drop out, and
+ * live frames are compacted into the other of two areas at device-decided checkpoints
+ * (workspace: two areas of each code's tiles).  Otherwise (the default, faster on the DVB-S2
+ * seven-rate batch): every step of every code is recorded as its own launches on concurrent
+ * branches; frames that finish early leave the remaining steps as cheap no-ops.  fecl_ldpc_graph_check (after a launch has finished)
+ * reports FECL_DEVICE_ERROR if a persistent launch stopped on its watchdog (a wait that did not
+ * end within seconds); fecl_ldpc_graph_is_persistent tells which form a graph has. */
 typedef struct fecl_ldpc_graph fecl_ldpc_graph;
 int fecl_ldpc_graph_create(fecl_ldpc* const* handles, const int64_t* frames, int32_t count,
                            const float* const* llr, uint8_t* const* msg, int32_t* const* iterations,
                            fecl_ldpc_graph** out);
 int fecl_ldpc_graph_launch(fecl_ldpc_graph* g, void* stream);
+int fecl_ldpc_graph_check(fecl_ldpc_graph* g);
+int fecl_ldpc_graph_is_persistent(const fecl_ldpc_graph* g, int32_t* persistent);
 int fecl_ldpc_graph_destroy(fecl_ldpc_graph* g);
+/* 1: graphs made with this handle may use the persistent form (default 0: per-step form). */
+int fecl_ldpc_set_graph_persistent(fecl_ldpc* h, int32_t enable);
 
 /* Continuous-batching decode server (no counterpart in the reference).  Each of the `count`
  * codes gets a pool of device frame slots: slots[i] frames, or, where slots is NULL or
