@@ -30,10 +30,15 @@ METRIC = "decoded info Gb/s at FER parity vs FP64 (DVB-S2 r1/2); p50/p99 latency
 
 # HBM bytes moved per algorithmic byte by the dominant kernels, from rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE runs of this workload (all frames live), each counter scaled by what
-# it reads for a plain 1 GiB copy (FETCH x2.000, WRITE x1.000).  Keyed by (msg format, lane words).
-PMC_TRAFFIC_RATIO = {("f32", 4): {"vn_kernel": 1.0030, "cn_kernel": 1.0096},
+# it reads for copy kernels of known size (FETCH x2.000, WRITE x1.000).  Keyed by (msg format,
+# lane words); a layout without an entry reports traffic null.
+PMC_TRAFFIC_RATIO = {("f32", 4): {"vn_kernel": 1.0031, "cn_kernel": 1.0096},
                      ("f32", 2): {"vn_kernel": 1.0109, "cn_kernel": 1.0093},
-                     ("f16", 2): {"vn_kernel": 1.0132, "cn_kernel": 1.0192}}
+                     ("f32", 1): {"vn_kernel": 0.9977, "cn_kernel": 1.0091},
+                     ("f16", 2): {"vn_kernel": 1.0132, "cn_kernel": 1.0192},
+                     ("f16", 1): {"vn_kernel": 1.0268, "cn_kernel": 1.0185},
+                     ("bf16", 2): {"vn_kernel": 1.0132, "cn_kernel": 1.0192},
+                     ("bf16", 1): {"vn_kernel": 1.0268, "cn_kernel": 1.0185}}
 
 # Extra Eb/N0 points (dB) of the precision study, across the waterfall of min-sum on DVB-S2 r1/2.
 AGREEMENT_EBN0 = (0.8, 0.9, 1.0, 1.1, 1.2)   # Linear / Exact waterfall of r1/2
@@ -312,7 +317,7 @@ def run_precision(args, device):
     import torch
     import fecl_amd as F
     from oracle import oracle as O
-    threads = min(16, os.cpu_count() or 1)
+    threads = host_threads()[0]
     oc = O.dvbs2(64800, 1, 2)
     H = F.DvbS2.matrix(64800, 0.5)
     fmts = {"f32": F.MsgFormat.F32, "f16": F.MsgFormat.F16, "bf16": F.MsgFormat.BF16}
@@ -392,7 +397,7 @@ def run_agreement(args, device):
     import torch
     import fecl_amd as F
     from oracle import oracle as O
-    threads = min(16, os.cpu_count() or 1)
+    threads = host_threads()[0]
     oc = O.dvbs2(64800, 1, 2)
     H = F.DvbS2.matrix(64800, 0.5)
     frames = args.agreement_frames
@@ -453,7 +458,7 @@ def run_plain(args, device):
     import fecl_amd as F
     from oracle import oracle as O
     frames, iters, ebn0, reps = 16, 50, 1.0, 20
-    threads = min(16, os.cpu_count() or 1)
+    threads = host_threads()[0]
     H = F.DvbS2.matrix(64800, 0.5)
     dec = F.Ldpc(H, F.DecoderOptions().algorithm(F.Exact).iterations(iters))
     K, N = dec.msgSize(), dec.paritySize()
@@ -513,6 +518,85 @@ def run_plain(args, device):
     return out
 
 
+# Best stream-probe setting per mode on MI355X (`bench.py --config copy` sweeps them all):
+# (order/hints flags, workgroups per CU)
+CEILING_SETTINGS = {"read": (3, 2), "write": (2, 8), "update": (1, 2), "copy": (1, 2)}
+
+
+def stream_ceiling(device, gib=2, reps=5, settings=None):
+    """Achievable HBM bandwidth on this GPU: hand-written streaming kernels
+    (fecl_amd/libfecl_stream.so, 16-byte words) copying, updating in place, reading and writing
+    gib GiB, swept over grid-stride and per-block-chunk order, with and without non-temporal
+    hints, at 2 to 16 workgroups of 256 threads per CU (or only `settings[mode]`); best of
+    `reps` per setting, timed with HIP events on the launch stream.  GB/s counts the bytes read
+    plus written.  The full sweep adds scattered 1 KiB rows updated in place over 8 GiB.
+    Returns the best setting of each mode and the whole sweep."""
+    import ctypes
+    import torch
+    lib = ctypes.CDLL(os.path.join(ROOT, "fecl_amd", "libfecl_stream.so"))
+    lib.fecl_stream_probe.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int,
+                                      ctypes.c_int, ctypes.c_void_p]
+    cus = torch.cuda.get_device_properties(device).multi_processor_count
+    nbytes = gib << 30
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device=device)
+    b = torch.empty_like(a)
+    st = torch.cuda.current_stream(device)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    sweep, best = {}, {}
+    # "update" is the copy with source and destination the same buffer: every word is read and
+    # written back in place, as the decoder's check and variable passes update their rows
+    for mode, name, moved in ((0, "copy", 2 * nbytes), (0, "update", 2 * nbytes), (1, "read", nbytes),
+                              (2, "write", nbytes)):
+        dst = a if name == "update" else b
+        grid = [settings[name]] if settings else [(f, p) for f in range(4) for p in (2, 4, 8, 16)]
+        for flags, per_cu in grid:
+            t = 1e30
+            for _ in range(reps + 1):
+                e0.record(st)
+                rc = lib.fecl_stream_probe(mode, a.data_ptr(), dst.data_ptr(), nbytes, flags, per_cu * cus,
+                                           st.cuda_stream)
+                e1.record(st)
+                e1.synchronize()
+                if rc:
+                    raise RuntimeError(f"stream probe failed ({rc})")
+                t = min(t, e0.elapsed_time(e1))
+            gbs = round(moved / t / 1e6, 1)
+            key = f"{name}/{'chunk' if flags & 2 else 'stride'}/{'nt' if flags & 1 else 'plain'}/{per_cu}"
+            sweep[key] = gbs
+            if gbs > best.get(name, (0.0, ""))[0]:
+                best[name] = (gbs, key)
+    del a, b
+    torch.cuda.empty_cache()
+    if settings is None:
+        # scattered 1 KiB rows updated in place over 8 GiB (32x the Infinity Cache), the access
+        # pattern of the variable pass
+        sb = 8 << 30
+        c = torch.zeros(sb // 4, dtype=torch.float32, device=device)
+        for flags in (0, 1):
+            for per_cu in (2, 4, 8, 16):
+                waves = per_cu * cus * 4
+                moved = 2 * 8192 * max(1, (sb // 1024) // (8 * waves)) * waves
+                t = 1e30
+                for _ in range(reps + 1):
+                    e0.record(st)
+                    rc = lib.fecl_stream_probe(3, None, c.data_ptr(), sb, flags, per_cu * cus, st.cuda_stream)
+                    e1.record(st)
+                    e1.synchronize()
+                    if rc:
+                        raise RuntimeError(f"stream probe failed ({rc})")
+                    t = min(t, e0.elapsed_time(e1))
+                gbs = round(moved / t / 1e6, 1)
+                key = f"scatter_update/rows1KiB/{'nt' if flags & 1 else 'plain'}/{per_cu}"
+                sweep[key] = gbs
+                if gbs > best.get("scatter_update", (0.0, ""))[0]:
+                    best["scatter_update"] = (gbs, key)
+        del c
+        torch.cuda.empty_cache()
+    return {"bytes": nbytes, "best_GBs": {k: v[0] for k, v in best.items()},
+            "best_setting": {k: v[1] for k, v in best.items()},
+            "sweep_GBs": sweep, "key": "mode/order/hints/workgroups per CU"}
+
+
 def copy_reference(device, gib=1, reps=5):
     """A plain device-to-device copy of gib GiB (the PMC byte-counter calibration kernel)."""
     import torch
@@ -563,6 +647,69 @@ def time_form(F, H, alg, fmt, iterations, batches, frames, K, steps, device):
             "lane_words": dec.layout()[0], "steps": steps,
             "cn_GBs": round(its * 2 * E * w / (ms[0] / 1e3) / 1e9, 1) if ms[0] > 0 else None,
             "vn_GBs": round(vn_updates * (2 * E * w + 4 * N) / (ms[1] / 1e3) / 1e9, 1) if ms[1] > 0 else None}
+
+
+def host_threads():
+    """Threads for the host legs: the CPUs this process may run on (sched_getaffinity), capped
+    by the cgroup CPU quota when one is set and at 16 (a one-GPU box's CPU share)."""
+    aff = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        q, per = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            quota = int(q) / int(per)
+    except (OSError, ValueError):
+        pass
+    n = aff if quota is None else min(aff, max(1, int(quota)))
+    return min(n, 16), {"affinity_cpus": aff, "cgroup_cpu_quota": quota, "cap": 16}
+
+
+def run_waterfall(F, H, device, threads, iterations, seed, frames=2048, points=(1.55, 1.6)):
+    """FER on the r1/2 waterfall of the GPU's FP32, FP16 and BF16 message forms (min-sum) and of
+    FeCl's FP64 decoder (the oracle's bit-identical restatement, host cores) on the same FP64
+    channel LLRs (the GPU takes them rounded to FP32).  The gap of each form to FP64 in dB at
+    each point is its log10-FER excess over FP64 divided by the FP64 curve's log10-FER slope
+    between the points (None where a FER is 0)."""
+    import numpy as np
+    import torch
+    from oracle import oracle as O
+    oc = O.dvbs2(64800, 1, 2)
+    fmts = {"f32": F.MsgFormat.F32, "f16": F.MsgFormat.F16, "bf16": F.MsgFormat.BF16}
+    decs = {k: F.Ldpc(H, F.DecoderOptions().algorithm(F.Approximate).iterations(iterations).msgFormat(v))
+            for k, v in fmts.items()}
+    K, N = decs["f32"].msgSize(), decs["f32"].paritySize()
+    rng = np.random.default_rng(seed + 13)
+    rows = []
+    for e in points:
+        info = rng.integers(0, 2, (frames, K), dtype=np.uint8)
+        cw = decs["f32"].encode(info).reshape(frames, N)
+        sigma2 = 1.0 / (2.0 * 0.5 * 10.0 ** (e / 10.0))
+        llr = -2.0 * ((1.0 - 2.0 * cw) + rng.normal(0.0, np.sqrt(sigma2), cw.shape)) / sigma2
+        t0 = time.perf_counter()
+        ref = O.decode(oc, llr, O.APPROXIMATE, iterations, prec=64, want=("hard",), threads=threads)["hard"]
+        row = {"ebn0": e, "frames": frames, "fer_fp64": float(np.mean(np.any(ref != info, axis=1))),
+               "fp64_seconds": round(time.perf_counter() - t0, 2)}
+        d_llr = torch.from_numpy(llr.astype(np.float32)).to(device)
+        for k, dec in decs.items():
+            hard = dec.decode(d_llr).view(frames, K).cpu().numpy()
+            row[f"fer_{k}"] = float(np.mean(np.any(hard != info, axis=1)))
+            row[f"frames_differing_from_fp64_{k}"] = int(np.sum(np.any(hard != ref, axis=1)))
+        rows.append(row)
+        print(f"[bench] waterfall {row}", file=sys.stderr, flush=True)
+        del d_llr, llr, cw, ref
+    gaps = {k: [None] * len(rows) for k in fmts}
+    f0, f1 = rows[0]["fer_fp64"], rows[-1]["fer_fp64"]
+    slope = None
+    if f0 > f1 > 0:
+        slope = (np.log10(f1) - np.log10(f0)) / (points[-1] - points[0])
+        for k in fmts:
+            for i, r in enumerate(rows):
+                if r[f"fer_{k}"] > 0 and r["fer_fp64"] > 0:
+                    gaps[k][i] = round(float((np.log10(r[f"fer_{k}"]) - np.log10(r["fer_fp64"])) / -slope), 4)
+    return {"workload": f"DVB-S2 n=64800 r1/2, min-sum, {iterations} max iterations, {frames} frames per Eb/N0, "
+                        "FP64 channel LLRs", "host_threads": threads, "points": rows,
+            "fp64_log10_fer_slope_per_db": None if slope is None else round(float(slope), 3),
+            "gap_db_vs_fp64": gaps}
 
 
 def fp64_parity(llr_host, info_host, iters, threads, gpu_hard):
@@ -634,6 +781,7 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=2048)
     ap.add_argument("--offered-fps", type=float, default=200000.0)
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-waterfall", action="store_true", help="skip the FP16/BF16 vs FP64 waterfall FER leg")
     ap.add_argument("--no-forms", action="store_true", help="skip the Linear / FP16 / BF16 lines")
     ap.add_argument("--ebn0", type=str, default="",
                     help="comma-separated Eb/N0 (dB) of the batches (default: the headline points)")
@@ -672,7 +820,9 @@ def main():
         print(json.dumps(run_agreement(args, device)), flush=True)
         return
     if args.config == "copy":
-        print(json.dumps(copy_reference(device)), flush=True)
+        out = copy_reference(device)
+        out["streaming_kernels"] = stream_ceiling(device)
+        print(json.dumps(out), flush=True)
         return
     if args.config == "plain":
         print(json.dumps(run_plain(args, device)), flush=True)
@@ -792,6 +942,18 @@ def main():
                      "cn_GBs": round(cn_gbs, 1), "vn_GBs": round(vn_gbs, 1)},
     }
 
+    # the achievable ceiling for the dominant kernel's read/write mix: the best hand-written
+    # streaming read and write rates on this GPU, combined in the kernel's proportions
+    ceil = stream_ceiling(device, gib=1, reps=3, settings=CEILING_SETTINGS)["best_GBs"]
+    rd = E * w + (4 * N if dom == "vn_kernel" else 0)
+    wr = E * w
+    mix = (rd + wr) / (rd / ceil["read"] + wr / ceil["write"])
+    result["roofline"]["measured_ceiling"] = {
+        "read_GBs": ceil["read"], "write_GBs": ceil["write"], "update_in_place_GBs": ceil["update"],
+        "copy_GBs": ceil["copy"], "mix_bound_GBs": round(mix, 1), "frac_of_mix_bound": round(dom_gbs / mix, 4),
+        "note": "streaming kernels of fecl_amd/libfecl_stream.so; mix bound = the kernel's bytes read and "
+                "written at the best read-only and write-only rates"}
+
     if not args.no_forms:
         # the same batches through the other forms of the path (one timed decode of each batch)
         forms = {}
@@ -803,6 +965,15 @@ def main():
             forms[name] = time_form(F, H, a2, f2, args.iterations, batches, args.frames, K, len(batches), device)
             print(f"[bench] {name}: {forms[name]}", file=sys.stderr, flush=True)
         result["other_forms"] = forms
+        # the precision form on its own 4096-frame batches (same channel points)
+        p_batches = make_batches(F, dec, H, 4096, args.seed + 1, device, points)
+        torch.cuda.synchronize()
+        result["precision_4096_frames"] = {
+            name: time_form(F, H, F.Approximate, f2, args.iterations, p_batches, 4096, K, len(p_batches), device)
+            for name, f2 in (("minsum_f16", F.MsgFormat.F16), ("minsum_bf16", F.MsgFormat.BF16))}
+        print(f"[bench] 4096-frame precision form: {result['precision_4096_frames']}", file=sys.stderr, flush=True)
+        del p_batches
+        torch.cuda.empty_cache()
         # FeCl's hooks hand over host FP64 buffers: the same decode from host memory, transfers
         # included (pageable numpy arrays, as a FeCl caller holds them)
         llr_h = batches[0]["llr"].cpu().numpy().astype(np.float64)
@@ -829,7 +1000,7 @@ def main():
         print(f"[bench] host soDecode path: {result['host_buffer_path_sodecode']}", file=sys.stderr, flush=True)
 
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
+        threads, cores_info = host_threads()
         cpu_batch = batches[min(2, len(batches) - 1)]  # Eb/N0 = 1.0 dB with the headline points
         nf = min(args.cpu_frames, args.frames)
         llr_host = cpu_batch["llr"][:nf].cpu().numpy().astype(np.float64)
@@ -852,7 +1023,9 @@ def main():
                                   "cores": threads, "kind": cb["kind"],
                                   "sample": f"{nf} frames of the Eb/N0={cpu_batch['ebn0']} dB batch, FP64 min-sum, "
                                             f"{args.iterations} max iterations, {cb['seconds']:.2f} s wall",
-                                  "cpu": _cpu_model()}
+                                  "cpu": _cpu_model(), "host_cpus": cores_info}
+        if not args.no_waterfall:
+            result["waterfall_fer_vs_fp64"] = run_waterfall(F, H, device, threads, args.iterations, args.seed)
     if not args.no_serving:
         # the serving pools start from a device heap without the batch bench's buffers (decoder
         # workspaces, input batches, torch's cached blocks)
