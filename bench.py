@@ -213,8 +213,9 @@ def run_uneven(args):
 
 
 class ServingSetup:
-    """Seven DVB-S2 normal-rate decoders (min-sum, FP16 messages) and per-rate host pools of
-    FP32 LLR frames at Eb/N0 within +-0.3 dB of each rate's ~1e-3 FER point."""
+    """Seven DVB-S2 normal-rate decoders (min-sum, FP16 messages) and per-rate pools of FP32 LLR
+    frames at Eb/N0 within +-0.3 dB of each rate's ~1e-3 FER point, in device memory (as a
+    receiver's demapper on the same GPU leaves them) and in page-locked host memory."""
 
     POOL = 768
 
@@ -225,7 +226,7 @@ class ServingSetup:
         gen = torch.Generator(device=device)
         gen.manual_seed(args.seed + 1)
         self.rates = list(RATE_POINTS)
-        self.decs, self.pools, self.infos, self.pinned = [], [], [], []
+        self.decs, self.pools, self.dev, self.infos, self.pinned = [], [], [], [], []
         for rate in self.rates:
             dec = F.Ldpc(F.DvbS2.matrix(64800, rate),
                          F.DecoderOptions().algorithm(F.Approximate).iterations(args.iterations)
@@ -234,15 +235,30 @@ class ServingSetup:
             idx = np.arange(self.POOL) % 256
             ebn0 = torch.from_numpy(RATE_POINTS[rate] + rng.uniform(-0.3, 0.3, self.POOL)).to(device)
             llr = awgn_llr_gpu(torch.from_numpy(cw[idx]).to(device), ebn0, dec.msgSize() / dec.paritySize(), gen)
-            # page-locked host memory: the server copies admitted frames straight from it
+            self.dev.append(llr)
             host = llr.cpu().pin_memory()
             self.pinned.append(host)
             self.pools.append(host.numpy())
             self.infos.append(info[idx])
             self.decs.append(dec)
+        torch.cuda.synchronize()
+
+    def eager_frames_per_s(self, reps=3):
+        """Each rate's pool decoded by one eager decode() call from device memory: frames per
+        second over the seven calls (the serving mix has equal frames per rate)."""
+        import torch
+        for d, x in zip(self.decs, self.dev):
+            d.decode(x)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            for d, x in zip(self.decs, self.dev):
+                d.decode(x)
+        torch.cuda.synchronize()
+        return reps * self.POOL * len(self.decs) / (time.perf_counter() - t0)
 
 
-def run_serving(F, setup, args, duration_s, offered_fps, seed):
+def run_serving(F, setup, args, duration_s, offered_fps, seed, device_inputs=True):
     """Serving: Poisson-arriving requests of U[1,256] frames, rate uniform over the 7 rates;
     continuous batching in the library's server.  Arrivals are generated by this thread at
     their scheduled times for duration_s; every result is then collected and checked."""
@@ -250,6 +266,7 @@ def run_serving(F, setup, args, duration_s, offered_fps, seed):
     rng = np.random.default_rng(seed)
     slots = [args.serve_slots] * len(setup.rates)
     sv = F.Server(setup.decs, slots=slots)
+    sv.setRoundSteps(args.serve_round_steps)
     lam = offered_fps / 128.5     # requests per second (mean request 128.5 frames)
     tickets = []
     t_start = time.perf_counter()
@@ -265,7 +282,8 @@ def run_serving(F, setup, args, duration_s, offered_fps, seed):
         ri = int(rng.integers(0, len(setup.rates)))
         f = int(rng.integers(1, 257))
         o = int(rng.integers(0, setup.POOL - f + 1))
-        tickets.append((ri, o, f, sv.submit(ri, setup.pools[ri][o:o + f])))
+        src = setup.dev[ri][o:o + f] if device_inputs else setup.pools[ri][o:o + f]
+        tickets.append((ri, o, f, sv.submit(ri, src)))
     t_sub_end = time.perf_counter()
     offered_frames = sum(f for _, _, f, _ in tickets)
     lats, errs, frames, bits = [], 0, 0, 0
@@ -283,26 +301,39 @@ def run_serving(F, setup, args, duration_s, offered_fps, seed):
     st = sv.stats()
     sv.close()
     wall = t_end - t_start
-    return {"offered_frames_per_s": offered_fps,
+    return {"offered_frames_per_s": offered_fps, "inputs": "device memory" if device_inputs else "pinned host memory",
             "arrival_frames_per_s": round(offered_frames / (t_sub_end - t_start), 1),
             "arrival_seconds": round(t_sub_end - t_start, 2), "requests": len(tickets), "frames": frames,
             "throughput_frames_per_s": round(frames / wall, 1), "info_Gb_per_s": round(bits / wall / 1e9, 4),
             "latency_ms": {"p50": round(float(np.percentile(lats, 50)) * 1e3, 2),
                            "p99": round(float(np.percentile(lats, 99)) * 1e3, 2), "unit": "per request"},
-            "fer": round(errs / max(1, frames), 5), "server_rounds": st["steps"], "slots_per_rate": slots[0]}
+            "fer": round(errs / max(1, frames), 5), "server_steps": st["steps"], "slots_per_rate": slots[0],
+            "steps_per_round": args.serve_round_steps}
 
 
 def serving_report(F, args, device, duration_s):
-    """The offered load (default 200k frames/s), then 70 % of the throughput it sustained, so
-    that the second latency figure is that of an unsaturated server."""
+    """The offered load (default 200k frames/s) with requests' LLRs in device memory and in
+    page-locked host memory; then 50, 70 and 90 % of the throughput the device-memory run
+    sustained, for the latency of an unsaturated server; and the eager rate on the same mix."""
     setup = ServingSetup(F, args, device)
+    eager = setup.eager_frames_per_s()
     sat = run_serving(F, setup, args, duration_s, args.offered_fps, args.seed + 2)
-    load = round(0.7 * sat["throughput_frames_per_s"], -2)
-    under = run_serving(F, setup, args, duration_s, load, args.seed + 3)
+    print(f"[bench] serving at offered load: {sat}", file=sys.stderr, flush=True)
+    host = run_serving(F, setup, args, duration_s, args.offered_fps, args.seed + 5, device_inputs=False)
+    print(f"[bench] serving from host memory: {host}", file=sys.stderr, flush=True)
+    loads = {}
+    for i, frac in enumerate((0.5, 0.7, 0.9)):
+        load = round(frac * sat["throughput_frames_per_s"], -2)
+        loads[f"{int(frac * 100)}pct"] = run_serving(F, setup, args, duration_s, load, args.seed + 3 + 10 * i)
+        print(f"[bench] serving at {frac:.0%} of sustained: {loads[f'{int(frac * 100)}pct']}", file=sys.stderr,
+              flush=True)
     return {"workload": "Poisson arrivals, requests of U[1,256] frames, rate uniform over DVB-S2 normal "
                         "{1/4,1/2,3/5,2/3,3/4,5/6,9/10}, Eb/N0 within +-0.3 dB of each rate's ~1e-3 FER point, "
                         f"min-sum, f16 messages, {args.iterations} max iterations",
-            "at_offered": sat, "at_70pct_of_sustained": under}
+            "eager_same_mix_frames_per_s": round(eager, 1),
+            "sustained_vs_eager": round(sat["throughput_frames_per_s"] / eager, 4),
+            "at_offered": sat, "at_offered_host_inputs": host,
+            "latency_at_fraction_of_sustained": loads}
 
 
 def run_precision(args, device):
@@ -779,6 +810,7 @@ def main():
     ap.add_argument("--agreement-frames", type=int, default=1024)
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--serve-slots", type=int, default=2048)
+    ap.add_argument("--serve-round-steps", type=int, default=1, help="server steps per round")
     ap.add_argument("--offered-fps", type=float, default=200000.0)
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-waterfall", action="store_true", help="skip the FP16/BF16 vs FP64 waterfall FER leg")

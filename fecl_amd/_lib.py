@@ -61,6 +61,7 @@ SIGNATURES = {
     "fecl_ldpc_server_submit": [P, I32, P, I64, P, P, P],
     "fecl_ldpc_server_wait": [P, ctypes.c_uint64, ctypes.c_double, P, P],
     "fecl_ldpc_server_stats": [P, P, P, P],
+    "fecl_ldpc_server_set_round_steps": [P, I32],
     "fecl_ldpc_server_destroy": [P],
     "fecl_ldpc_trim": [P],
 }

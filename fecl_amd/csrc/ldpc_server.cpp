@@ -2,9 +2,11 @@
 //
 // FeCl has no serving layer: Codec::decode blocks on one batch (src/Codec.h:210-238).  Here
 // each code owns a pool of device frame slots whose message state stays resident across
-// iterations.  A worker thread runs rounds, each one step for every pool:
-//   admit   queued frames into free slots (their LLRs are copied into the slots' tile rows)
-//   step    check-node, variable-node (+ outputs of finishing frames), syndrome/control
+// iterations.  A worker thread runs rounds, each a few steps for every pool:
+//   admit   queued frames into free slots (their LLRs are copied into the slots' tile rows;
+//           LLRs already in device memory are read in place)
+//   step    check-node, variable-node (+ outputs of finishing frames), syndrome/control,
+//           repeated steps_per_round times with no host work in between
 //   retire  frames that reached DONE: read their hard bits and iteration counts, free the slots
 // with the device kept one round ahead of the host's bookkeeping.  Requests that arrive while
 // others are on the GPU join at a round boundary, and a frame's arithmetic is exactly that of
@@ -30,11 +32,16 @@ namespace engine {
 
 using Clock = std::chrono::steady_clock;
 
+// Steps per round unless fecl_ldpc_server_set_round_steps says otherwise, and the limit.
+constexpr int kDefaultRoundSteps = 1;
+constexpr int kMaxRoundSteps = 64;
+
 struct Request {
   uint64_t id = 0;
   int pool = 0;
-  const float* llr = nullptr;   // host [frames][N]
+  const float* llr = nullptr;   // [frames][N], host or device memory
   bool pinned = false;          // llr is page-locked: admission copies it straight to the device
+  bool device = false;          // llr is device memory: admission kernels read it in place
   int64_t frames = 0;
   uint8_t* msg = nullptr;       // host [frames][K]
   int32_t* iters = nullptr;     // host [frames] or null
@@ -45,13 +52,15 @@ struct Request {
 
 struct PinnedBuf {
   void* p = nullptr;
+  void* dev = nullptr;   // the same memory as the device addresses it (kernels read it in place)
   size_t bytes = 0;
   hipError_t ensure(size_t n) {
     if (n <= bytes) return hipSuccess;
     if (p) (void)hipHostFree(p);
-    p = nullptr;
+    p = dev = nullptr;
     bytes = 0;
-    hipError_t e = hipHostMalloc(&p, n, hipHostMallocDefault);
+    hipError_t e = hipHostMalloc(&p, n, hipHostMallocMapped);
+    if (e == hipSuccess) e = hipHostGetDevicePointer(&dev, p, 0);
     if (e == hipSuccess) bytes = n;
     return e;
   }
@@ -71,20 +80,20 @@ struct Pool {
   DevState s{};
   DevOut out{};
   int64_t nslots = 0;
-  int admit_cap = 0;
-  // free slots, lowest first: live frames stay packed into few tiles, and tiles without a
-  // live frame cost the kernels one flag load
-  std::priority_queue<int32_t, std::vector<int32_t>, std::greater<int32_t>> free_slots;
+  // free slots of each tile and the tile's occupied slots (see choose_admissions)
+  std::vector<std::vector<int32_t>> tile_free;
+  std::vector<int32_t> tile_used;
   std::vector<std::pair<Request*, int64_t>> owner;  // slot -> (request, frame)
   int64_t live = 0;
   std::deque<Request*> queue;
   std::vector<int64_t> admit_round;                 // slot -> round its current frame was admitted in
-  DevBuf d_stage[2], d_admit[2], d_ext_slots, d_ext_bits, d_ext_iters;
+  DevBuf d_stage[2], d_admit[2], d_ext_bits[2], d_ext_iters[2];
   // host buffers, two of each: the worker fills or reads one while the device uses the other
   PinnedBuf h_stage[2], h_admit[2], h_ext_slots[2], h_ext_bits[2], h_ext_iters[2], h_phase[2];
   std::vector<std::pair<Request*, int64_t>> ext_owner[2];   // retired entries awaiting their results
   ~Pool() {
-    for (DevBuf* b : {&d_stage[0], &d_stage[1], &d_admit[0], &d_admit[1], &d_ext_slots, &d_ext_bits, &d_ext_iters})
+    for (DevBuf* b : {&d_stage[0], &d_stage[1], &d_admit[0], &d_admit[1], &d_ext_bits[0], &d_ext_bits[1],
+                      &d_ext_iters[0], &d_ext_iters[1]})
       b->release();
   }
 };
@@ -105,8 +114,10 @@ struct fecl_ldpc_server {
   int error = FECL_OK;
   std::string error_msg;
   int64_t frames_done = 0, steps = 0, queued_frames = 0;
-  hipStream_t st = nullptr;      // admissions' kernels, steps, retirements
+  int steps_per_round = kDefaultRoundSteps;   // read by the worker at each round (under mu)
+  hipStream_t st = nullptr;      // admissions' kernels, steps, retirements' extracts
   hipStream_t copy = nullptr;    // admissions' host-to-device copies, overlapping the steps
+  hipStream_t out = nullptr;     // phases and results to the host, overlapping the steps
   int device = 0;
   std::thread worker;
   ~fecl_ldpc_server() {
@@ -117,15 +128,19 @@ struct fecl_ldpc_server {
     work_cv.notify_all();
     if (worker.joinable()) worker.join();
     (void)hipSetDevice(device);
-    if (st) (void)hipStreamSynchronize(st);
-    if (copy) (void)hipStreamSynchronize(copy);
+    for (hipStream_t x : {st, copy, out})
+      if (x) (void)hipStreamSynchronize(x);
     pools.clear();
-    if (st) (void)hipStreamDestroy(st);
-    if (copy) (void)hipStreamDestroy(copy);
+    for (hipStream_t x : {st, copy, out})
+      if (x) (void)hipStreamDestroy(x);
   }
 };
 
 namespace {
+
+// Frames each pool may admit per step of a round: admissions of a round are capped at this
+// times the round's steps (and the pool's free slots).
+constexpr int kAdmitPerStep = 128;
 
 // The worker keeps the device one round ahead of the host.  Round e = admissions chosen from
 // the free slots, one step of every pool holding frames, and a copy of every slot's phase.
@@ -136,43 +151,80 @@ namespace {
 // Host buffers come in pairs indexed by the round's parity, so the host never touches a
 // buffer the device may still be using.
 struct Events {
-  hipEvent_t round[2] = {nullptr, nullptr}, ext[2] = {nullptr, nullptr};
+  hipEvent_t round[2] = {nullptr, nullptr}, ext[2] = {nullptr, nullptr};   // on the out stream
   hipEvent_t copied[2] = {nullptr, nullptr};    // admissions' copies of round e (copy stream)
   hipEvent_t admitted[2] = {nullptr, nullptr};  // admission kernels of round e (compute stream)
+  hipEvent_t stepped[2] = {nullptr, nullptr};   // steps of round e (compute stream)
+  hipEvent_t extracted[2] = {nullptr, nullptr}; // extracts queued after round q (compute stream)
+  hipError_t create() {
+    for (hipEvent_t* e : all())
+      if (hipError_t r = hipEventCreateWithFlags(e, hipEventDisableTiming)) return r;
+    return hipSuccess;
+  }
+  std::vector<hipEvent_t*> all() {
+    std::vector<hipEvent_t*> v;
+    for (int i = 0; i < 2; ++i)
+      for (hipEvent_t* e : {&round[i], &ext[i], &copied[i], &admitted[i], &stepped[i], &extracted[i]}) v.push_back(e);
+    return v;
+  }
   ~Events() {
-    for (hipEvent_t e : {round[0], round[1], ext[0], ext[1], copied[0], copied[1], admitted[0], admitted[1]})
-      if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t* e : all())
+      if (*e) (void)hipEventDestroy(*e);
   }
 };
 
 struct Admit { Request* req; int64_t f0, n; };
 
-// Host side of the admissions of round e: queued frames into free slots (lowest first).
-// Returns whether any pool holds frames (and so takes a step in round e).
-bool choose_admissions(fecl_ldpc_server* sv, int64_t e, std::vector<std::vector<Admit>>* admits) {
+// Host side of the admissions of round e: queued frames into free slots.  A tile with a fresh
+// frame reads the channel values on its next check update besides its messages, and an
+// admission writes each frame's values across all of its tile's rows, so a round's admissions
+// go to few tiles: the (at most kFillTiles) tiles that hold frames and have the most free
+// slots, lowest slots first, then tiles holding no frame in index order.  Free slots of other
+// tiles wait for a later round.  Returns whether any pool holds frames (and so steps).
+constexpr int kFillTiles = 2;
+
+bool choose_admissions(fecl_ldpc_server* sv, int64_t e, int steps, std::vector<std::vector<Admit>>* admits) {
   bool any = false;
+  std::vector<int> order;
   std::lock_guard<std::mutex> lk(sv->mu);
   for (size_t pi = 0; pi < sv->pools.size(); ++pi) {
     Pool& P = *sv->pools[pi];
     (*admits)[pi].clear();
-    int n = 0;
+    const int64_t cap = std::min<int64_t>(P.nslots, (int64_t)kAdmitPerStep * steps);
+    int64_t n = 0;
     int32_t* slots = P.h_admit[e & 1].as<int32_t>();
-    while (!P.free_slots.empty() && !P.queue.empty() && n < P.admit_cap) {
-      Request* q = P.queue.front();
-      const int64_t k = std::min<int64_t>({(int64_t)P.free_slots.size(), q->frames - q->admitted,
-                                           (int64_t)(P.admit_cap - n)});
-      for (int64_t i = 0; i < k; ++i) {
-        const int32_t slot = P.free_slots.top();
-        P.free_slots.pop();
-        slots[n + i] = slot;
-        P.owner[slot] = {q, q->admitted + i};
-        P.admit_round[slot] = e;
+    if (!P.queue.empty()) {
+      const int T = (int)P.tile_free.size();
+      order.clear();
+      for (int t = 0; t < T; ++t)
+        if (P.tile_used[t] > 0 && !P.tile_free[t].empty()) order.push_back(t);
+      std::stable_sort(order.begin(), order.end(),
+                       [&](int x, int y) { return P.tile_free[x].size() > P.tile_free[y].size(); });
+      if ((int)order.size() > kFillTiles) order.resize(kFillTiles);
+      for (int t = 0; t < T; ++t)
+        if (P.tile_used[t] == 0) order.push_back(t);
+      for (int t : order) {
+        std::vector<int32_t>& fl = P.tile_free[t];
+        std::sort(fl.begin(), fl.end(), std::greater<int32_t>());   // lowest at the back
+        while (!fl.empty() && !P.queue.empty() && n < cap) {
+          Request* q = P.queue.front();
+          const int64_t k = std::min<int64_t>({(int64_t)fl.size(), q->frames - q->admitted, cap - n});
+          for (int64_t i = 0; i < k; ++i) {
+            const int32_t slot = fl.back();
+            fl.pop_back();
+            slots[n + i] = slot;
+            P.owner[slot] = {q, q->admitted + i};
+            P.admit_round[slot] = e;
+          }
+          P.tile_used[t] += (int32_t)k;
+          (*admits)[pi].push_back({q, q->admitted, k});
+          q->admitted += k;
+          sv->queued_frames -= k;
+          n += k;
+          if (q->admitted == q->frames) P.queue.pop_front();
+        }
+        if (P.queue.empty() || n >= cap) break;
       }
-      (*admits)[pi].push_back({q, q->admitted, k});
-      q->admitted += k;
-      sv->queued_frames -= k;
-      n += (int)k;
-      if (q->admitted == q->frames) P.queue.pop_front();
     }
     P.live += n;
     any |= P.live > 0;
@@ -180,72 +232,102 @@ bool choose_admissions(fecl_ldpc_server* sv, int64_t e, std::vector<std::vector<
   return any;
 }
 
-// Device side of round e: admission copies (on the copy stream, so they overlap the steps of
-// round e - 1 still running) and kernels, one step per pool, phase copies.
-int enqueue_round(fecl_ldpc_server* sv, Events* ev, int64_t e, const std::vector<std::vector<Admit>>& admits) {
+// Device side of round e: admission copies of host frames (on the copy stream, so they
+// overlap the steps of round e - 1 still running) and admission kernels, `steps` steps per
+// pool, phase copies.  Buffers of parity e & 1 were last used by round e - 2, which the host
+// has already seen complete, so they may be refilled or grown.
+int enqueue_round(fecl_ldpc_server* sv, Events* ev, int64_t e, int steps, const std::vector<std::vector<Admit>>& admits) {
   const int b = (int)(e & 1);
-  bool any_admit = false;
+  bool any_admit = false, any_copy = false;
   for (const auto& a : admits) any_admit |= !a.empty();
   if (any_admit) {
-    // staging buffers b were last read by the admission kernels of round e - 2
     if (e >= 2) HIP_TRY(hipStreamWaitEvent(sv->copy, ev->admitted[b], 0));
     for (size_t pi = 0; pi < sv->pools.size(); ++pi) {
       Pool& P = *sv->pools[pi];
       const int64_t N = P.h->code.N;
-      int64_t n = 0, staged = 0;
+      int64_t n = 0, host = 0, pageable = 0;
+      for (const Admit& a : admits[pi]) {
+        n += a.n;
+        if (!a.req->device) host += a.n;
+        if (!a.req->device && !a.req->pinned) pageable += a.n;
+      }
+      if (!n) continue;
+      // host frames are staged at their position among the round's admissions
+      if (host) {
+        if (P.d_stage[b].bytes < (size_t)(n * N * 4)) {
+          HIP_TRY(hipDeviceSynchronize());   // growing: nothing may still read the old buffer
+          HIP_TRY(P.d_stage[b].ensure((size_t)(n * N * 4)));
+        }
+        if (pageable) HIP_TRY(P.h_stage[b].ensure((size_t)(pageable * N * 4)));
+      }
+      int64_t off = 0, src = 0;
       for (const Admit& a : admits[pi]) {
         if (a.req->pinned) {   // page-locked request memory: copied straight to the device
-          HIP_TRY(hipMemcpyAsync(P.d_stage[b].as<float>() + n * N, a.req->llr + a.f0 * N, (size_t)(a.n * N) * 4,
+          HIP_TRY(hipMemcpyAsync(P.d_stage[b].as<float>() + off * N, a.req->llr + a.f0 * N, (size_t)(a.n * N) * 4,
                                  hipMemcpyHostToDevice, sv->copy));
-        } else {
-          std::memcpy(P.h_stage[b].as<float>() + staged * N, a.req->llr + a.f0 * N, (size_t)(a.n * N) * 4);
-          staged += a.n;
+          any_copy = true;
+        } else if (!a.req->device) {   // pageable: packed into pinned staging, then copied
+          std::memcpy(P.h_stage[b].as<float>() + src * N, a.req->llr + a.f0 * N, (size_t)(a.n * N) * 4);
+          HIP_TRY(hipMemcpyAsync(P.d_stage[b].as<float>() + off * N, P.h_stage[b].as<float>() + src * N,
+                                 (size_t)(a.n * N) * 4, hipMemcpyHostToDevice, sv->copy));
+          src += a.n;
+          any_copy = true;
         }
-        n += a.n;
-      }
-      if (!n) continue;
-      if (staged) {   // pageable frames were packed into the staging buffer in order
-        int64_t off = 0, src = 0;
-        for (const Admit& a : admits[pi]) {
-          if (!a.req->pinned) {
-            HIP_TRY(hipMemcpyAsync(P.d_stage[b].as<float>() + off * N, P.h_stage[b].as<float>() + src * N,
-                                   (size_t)(a.n * N) * 4, hipMemcpyHostToDevice, sv->copy));
-            src += a.n;
-          }
-          off += a.n;
-        }
+        off += a.n;
       }
       HIP_TRY(hipMemcpyAsync(P.d_admit[b].p, P.h_admit[b].p, (size_t)n * 4, hipMemcpyHostToDevice, sv->copy));
+      any_copy = true;
     }
-    HIP_TRY(hipEventRecord(ev->copied[b], sv->copy));
-    HIP_TRY(hipStreamWaitEvent(sv->st, ev->copied[b], 0));
+    if (any_copy) {
+      HIP_TRY(hipEventRecord(ev->copied[b], sv->copy));
+      HIP_TRY(hipStreamWaitEvent(sv->st, ev->copied[b], 0));
+    }
     for (size_t pi = 0; pi < sv->pools.size(); ++pi) {
       Pool& P = *sv->pools[pi];
-      int64_t n = 0;
-      for (const Admit& a : admits[pi]) n += a.n;
-      if (!n) continue;
       const uint8_t init = (uint8_t)((P.iterations == 1 ? PH_FINAL : PH_RUN) | PH_FRESH);
-      HIP_TRY(kernels_admit(P.lay, P.h->dc, P.s, P.d_stage[b].as<float>(), P.d_admit[b].as<int32_t>(), (int)n, init,
-                            sv->st));
+      // one launch per run of staged host frames, one per device-resident chunk
+      int64_t off = 0, run0 = -1;
+      auto flush = [&](int64_t end) -> int {
+        if (run0 >= 0 && end > run0)
+          HIP_TRY(kernels_admit(P.lay, P.h->dc, P.s, P.d_stage[b].as<float>() + run0 * P.h->code.N,
+                                P.d_admit[b].as<int32_t>() + run0, (int)(end - run0), init, sv->st));
+        run0 = -1;
+        return FECL_OK;
+      };
+      for (const Admit& a : admits[pi]) {
+        if (a.req->device) {
+          if (int rc = flush(off)) return rc;
+          HIP_TRY(kernels_admit(P.lay, P.h->dc, P.s, a.req->llr + a.f0 * P.h->code.N, P.d_admit[b].as<int32_t>() + off,
+                                (int)a.n, init, sv->st));
+        } else if (run0 < 0) {
+          run0 = off;
+        }
+        off += a.n;
+      }
+      if (int rc = flush(off)) return rc;
     }
     HIP_TRY(hipEventRecord(ev->admitted[b], sv->st));
   }
-  for (auto& pp : sv->pools) {
-    Pool& P = *pp;
-    if (!P.live) continue;
-    HIP_TRY(kernels_check(P.lay, P.h->dc, P.s, P.opts, P.h->code.max_row_degree, sv->st));
-    HIP_TRY(kernels_variable(P.lay, P.h->dc, P.s, P.out, P.h->code.max_var_degree, P.opts.fused_syn,
-                             (P.opts.nt_rows & 2) != 0, sv->st));
-    HIP_TRY(kernels_syndrome_control(P.lay, P.h->dc, P.s, P.iterations, P.opts.fused_syn, -1,
-                                     P.h->code.max_row_degree, sv->st));
+  for (int j = 0; j < steps; ++j) {
+    for (auto& pp : sv->pools) {
+      Pool& P = *pp;
+      if (!P.live) continue;
+      HIP_TRY(kernels_check(P.lay, P.h->dc, P.s, P.opts, P.h->code.max_row_degree, sv->st));
+      HIP_TRY(kernels_variable(P.lay, P.h->dc, P.s, P.out, P.h->code.max_var_degree, P.opts.fused_syn,
+                               (P.opts.nt_rows & 2) != 0, sv->st));
+      HIP_TRY(kernels_syndrome_control(P.lay, P.h->dc, P.s, P.iterations, P.opts.fused_syn, -1,
+                                       P.h->code.max_row_degree, sv->st));
+    }
   }
+  HIP_TRY(hipEventRecord(ev->stepped[b], sv->st));
+  HIP_TRY(hipStreamWaitEvent(sv->out, ev->stepped[b], 0));
   for (auto& pp : sv->pools) {   // every pool, so that each round's phases are complete
     Pool& P = *pp;
-    HIP_TRY(hipMemcpyAsync(P.h_phase[b].p, P.s.phase, (size_t)P.nslots, hipMemcpyDeviceToHost, sv->st));
+    HIP_TRY(hipMemcpyAsync(P.h_phase[b].p, P.s.phase, (size_t)P.nslots, hipMemcpyDeviceToHost, sv->out));
   }
-  HIP_TRY(hipEventRecord(ev->round[b], sv->st));
+  HIP_TRY(hipEventRecord(ev->round[b], sv->out));
   std::lock_guard<std::mutex> lk(sv->mu);
-  ++sv->steps;
+  sv->steps += steps;
   return FECL_OK;
 }
 
@@ -267,25 +349,36 @@ int retire_round(fecl_ldpc_server* sv, Events* ev, int64_t q, bool* queued) {
         list[P.ext_owner[b].size()] = (int32_t)slot;
         P.ext_owner[b].push_back(P.owner[slot]);
         P.owner[slot] = {nullptr, 0};
-        P.free_slots.push((int32_t)slot);
+        const int t = (int)(slot / P.lay.tw());
+        P.tile_free[t].push_back((int32_t)slot);
+        --P.tile_used[t];
         --P.live;
       }
       any |= !P.ext_owner[b].empty();
     }
   }
   if (!any) return FECL_OK;
+  // the extract kernels read the slot lists in place from page-locked memory; their results
+  // (buffers of parity b, free since the host handed out round q - 2's) go back on the out
+  // stream while the next round steps
+  for (auto& pp : sv->pools) {
+    Pool& P = *pp;
+    const int n = (int)P.ext_owner[b].size();
+    if (!n) continue;
+    HIP_TRY(kernels_extract(P.lay, P.h->dc, P.s, P.out, static_cast<const int32_t*>(P.h_ext_slots[b].dev), n,
+                            P.d_ext_bits[b].as<uint8_t>(), P.d_ext_iters[b].as<int32_t>(), sv->st));
+  }
+  HIP_TRY(hipEventRecord(ev->extracted[b], sv->st));
+  HIP_TRY(hipStreamWaitEvent(sv->out, ev->extracted[b], 0));
   for (auto& pp : sv->pools) {
     Pool& P = *pp;
     const int n = (int)P.ext_owner[b].size();
     if (!n) continue;
     const int64_t K = P.h->code.K;
-    HIP_TRY(hipMemcpyAsync(P.d_ext_slots.p, P.h_ext_slots[b].p, (size_t)n * 4, hipMemcpyHostToDevice, sv->st));
-    HIP_TRY(kernels_extract(P.lay, P.h->dc, P.s, P.out, P.d_ext_slots.as<int32_t>(), n, P.d_ext_bits.as<uint8_t>(),
-                            P.d_ext_iters.as<int32_t>(), sv->st));
-    HIP_TRY(hipMemcpyAsync(P.h_ext_bits[b].p, P.d_ext_bits.p, (size_t)n * K, hipMemcpyDeviceToHost, sv->st));
-    HIP_TRY(hipMemcpyAsync(P.h_ext_iters[b].p, P.d_ext_iters.p, (size_t)n * 4, hipMemcpyDeviceToHost, sv->st));
+    HIP_TRY(hipMemcpyAsync(P.h_ext_bits[b].p, P.d_ext_bits[b].p, (size_t)n * K, hipMemcpyDeviceToHost, sv->out));
+    HIP_TRY(hipMemcpyAsync(P.h_ext_iters[b].p, P.d_ext_iters[b].p, (size_t)n * 4, hipMemcpyDeviceToHost, sv->out));
   }
-  HIP_TRY(hipEventRecord(ev->ext[b], sv->st));
+  HIP_TRY(hipEventRecord(ev->ext[b], sv->out));
   *queued = true;
   return FECL_OK;
 }
@@ -320,12 +413,7 @@ void hand_out(fecl_ldpc_server* sv, int64_t q) {
 
 int server_run(fecl_ldpc_server* sv) {
   Events ev;
-  for (int i = 0; i < 2; ++i) {
-    HIP_TRY(hipEventCreateWithFlags(&ev.round[i], hipEventDisableTiming));
-    HIP_TRY(hipEventCreateWithFlags(&ev.ext[i], hipEventDisableTiming));
-    HIP_TRY(hipEventCreateWithFlags(&ev.copied[i], hipEventDisableTiming));
-    HIP_TRY(hipEventCreateWithFlags(&ev.admitted[i], hipEventDisableTiming));
-  }
+  HIP_TRY(ev.create());
   std::vector<std::vector<Admit>> admits(sv->pools.size());
   int64_t enqueued = 0, processed = 0;   // rounds enqueued / rounds whose phases were read
   int64_t ext_round = -1;                // round whose extract results are still to be handed out
@@ -334,9 +422,14 @@ int server_run(fecl_ldpc_server* sv) {
       std::lock_guard<std::mutex> lk(sv->mu);
       if (sv->stop) return FECL_OK;
     }
-    const bool live = choose_admissions(sv, enqueued, &admits);
+    int steps = 1;
+    {
+      std::lock_guard<std::mutex> lk(sv->mu);
+      steps = sv->steps_per_round;
+    }
+    const bool live = choose_admissions(sv, enqueued, steps, &admits);
     if (live) {
-      if (int rc = enqueue_round(sv, &ev, enqueued, admits)) return rc;
+      if (int rc = enqueue_round(sv, &ev, enqueued, steps, admits)) return rc;
       ++enqueued;
     }
     // read every completed round but the newest (which keeps the device busy meanwhile)
@@ -428,18 +521,18 @@ int fecl_ldpc_server_create(fecl_ldpc* const* handles, int32_t count, const int6
     HIP_TRY(hipMemset(P->s.phase, PH_FREE, (size_t)P->nslots));
     HIP_TRY(hipMemset(P->s.tile_flags, 0, (size_t)tiles * 4));
     HIP_TRY(hipMemset(P->s.fail, 0, (size_t)tiles * P->lay.fpl() * 8));
-    P->admit_cap = (int)std::min<int64_t>(P->nslots, 128);
     const Code& c = h->code;
+    // staging for host frames grows with the rounds' admissions; slot lists fit every slot
     for (int b = 0; b < 2; ++b) {
-      HIP_TRY(P->d_stage[b].ensure((size_t)P->admit_cap * c.N * 4));
-      HIP_TRY(P->d_admit[b].ensure((size_t)P->admit_cap * 4));
+      HIP_TRY(P->d_stage[b].ensure((size_t)std::min<int64_t>(P->nslots, kAdmitPerStep) * c.N * 4));
+      HIP_TRY(P->d_admit[b].ensure((size_t)P->nslots * 4));
     }
-    HIP_TRY(P->d_ext_slots.ensure((size_t)P->nslots * 4));
-    HIP_TRY(P->d_ext_bits.ensure((size_t)P->nslots * c.K));
-    HIP_TRY(P->d_ext_iters.ensure((size_t)P->nslots * 4));
     for (int b = 0; b < 2; ++b) {
-      HIP_TRY(P->h_stage[b].ensure((size_t)P->admit_cap * c.N * 4));
-      HIP_TRY(P->h_admit[b].ensure((size_t)P->admit_cap * 4));
+      HIP_TRY(P->d_ext_bits[b].ensure((size_t)P->nslots * c.K));
+      HIP_TRY(P->d_ext_iters[b].ensure((size_t)P->nslots * 4));
+    }
+    for (int b = 0; b < 2; ++b) {
+      HIP_TRY(P->h_admit[b].ensure((size_t)P->nslots * 4));
       HIP_TRY(P->h_ext_slots[b].ensure((size_t)P->nslots * 4));
       HIP_TRY(P->h_ext_bits[b].ensure((size_t)P->nslots * c.K));
       HIP_TRY(P->h_ext_iters[b].ensure((size_t)P->nslots * 4));
@@ -447,12 +540,15 @@ int fecl_ldpc_server_create(fecl_ldpc* const* handles, int32_t count, const int6
     }
     P->owner.assign((size_t)P->nslots, {nullptr, 0});
     P->admit_round.assign((size_t)P->nslots, 0);
-    for (int64_t s = 0; s < P->nslots; ++s) P->free_slots.push((int32_t)s);
+    P->tile_free.assign((size_t)tiles, {});
+    P->tile_used.assign((size_t)tiles, 0);
+    for (int64_t s = 0; s < P->nslots; ++s) P->tile_free[(size_t)(s / tw)].push_back((int32_t)s);
     sv->pools.push_back(std::move(P));
   }
   HIP_TRY(hipDeviceSynchronize());
   HIP_TRY(hipStreamCreateWithFlags(&sv->st, hipStreamNonBlocking));
   HIP_TRY(hipStreamCreateWithFlags(&sv->copy, hipStreamNonBlocking));
+  HIP_TRY(hipStreamCreateWithFlags(&sv->out, hipStreamNonBlocking));
   fecl_ldpc_server* raw = sv.get();
   sv->worker = std::thread(server_loop, raw);
   *out = sv.release();
@@ -477,8 +573,12 @@ int fecl_ldpc_server_submit(fecl_ldpc_server* sv, int32_t code, const float* llr
   r->iters = iterations;
   {
     hipPointerAttribute_t attr{};
-    r->pinned = hipPointerGetAttributes(&attr, llr) == hipSuccess && attr.type == hipMemoryTypeHost;
+    const bool known = hipPointerGetAttributes(&attr, llr) == hipSuccess;
     (void)hipGetLastError();   // a pageable pointer is not an error here
+    r->pinned = known && attr.type == hipMemoryTypeHost;
+    r->device = known && attr.type == hipMemoryTypeDevice;
+    if (r->device && attr.device != sv->device)
+      return fail(FECL_INVALID_ARGUMENT, "LLRs in device memory of another device");
   }
   r->t_submit = Clock::now();
   {
@@ -510,6 +610,13 @@ int fecl_ldpc_server_wait(fecl_ldpc_server* sv, uint64_t ticket, double timeout_
     return FECL_OK;
   }
   if (sv->error) return fail(sv->error, sv->error_msg);
+  return FECL_OK;
+}
+
+int fecl_ldpc_server_set_round_steps(fecl_ldpc_server* sv, int32_t steps) {
+  if (!sv || steps < 1 || steps > kMaxRoundSteps) return fail(FECL_INVALID_ARGUMENT, "invalid arguments");
+  std::lock_guard<std::mutex> lk(sv->mu);
+  sv->steps_per_round = steps;
   return FECL_OK;
 }
 

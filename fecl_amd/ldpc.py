@@ -727,15 +727,34 @@ class Server:
         check(lib().fecl_ldpc_server_slots(self._s, int(code), ctypes.byref(v)))
         return v.value
 
+    def setRoundSteps(self, steps):
+        """Steps each pool takes per round between the host's admissions and retirements
+        (1..64, default 1): more steps cost fewer host round trips, but a finished frame waits
+        up to steps - 1 more steps for its retirement and admissions come less often."""
+        check(lib().fecl_ldpc_server_set_round_steps(self._s, int(steps)))
+
     def submit(self, code, llr):
-        """Queue frames (host float32 [frames * N]) of code index `code`; returns a ticket."""
+        """Queue frames of code index `code`: float32 [frames * N], a host array or a
+        contiguous CUDA tensor on the server's device (read in place, no host copy; kept alive
+        until wait() returns it).  Returns a ticket."""
         d = self._decoders[code]
-        x = np.ascontiguousarray(llr, dtype=np.float32).ravel()
-        frames = d._frames(x.size)
+        if _is_cuda_tensor(llr):
+            import torch
+            if llr.dtype != torch.float32 or not llr.is_contiguous():
+                raise ValueError("device LLRs must be a contiguous float32 tensor")
+            x = llr
+            frames = d._frames(x.numel())
+            # the server reads the frames on its own stream: they must be complete now
+            torch.cuda.current_stream(x.device).synchronize()
+            ptr = ctypes.c_void_p(x.data_ptr())
+        else:
+            x = np.ascontiguousarray(llr, dtype=np.float32).ravel()
+            frames = d._frames(x.size)
+            ptr = _p(x)
         msg = np.zeros(frames * d.msgSize(), np.uint8)
         it = np.zeros(frames, np.int32)
         t = ctypes.c_uint64()
-        check(lib().fecl_ldpc_server_submit(self._s, int(code), _p(x), frames, _p(msg), _p(it), ctypes.byref(t)))
+        check(lib().fecl_ldpc_server_submit(self._s, int(code), ptr, frames, _p(msg), _p(it), ctypes.byref(t)))
         self._pending[t.value] = (x, msg, it)
         return t.value
 

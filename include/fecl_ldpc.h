@@ -276,14 +276,17 @@ int fecl_ldpc_set_graph_persistent(fecl_ldpc* h, int32_t enable);
 /* Continuous-batching decode server (no counterpart in the reference).  Each of the `count`
  * codes gets a pool of device frame slots: slots[i] frames, or, where slots is NULL or
  * slots[i] <= 0, a share of memory_bytes (<= 0: half of the device's free memory) sized by one
- * slot's footprint.  A worker thread repeats: admit queued frames into free slots, run one
- * check/variable/syndrome step on every pool holding live frames, retire finished frames.
- * Requests arriving while others run join at the next step; each frame's result equals a
- * batch decode of it (frames share no state).  Handles must outlive the server; each pool
- * decodes with the options (and layout) its handle had when the server was created.
- *   submit: host LLRs [frames][N] (FP32) of code `code`; msg [frames][K] (and iterations
- *           [frames], nullable) receive the results; all three stay valid until wait()
- *           reports done.
+ * slot's footprint.  A worker thread repeats rounds: admit queued frames into free slots, run
+ * check/variable/syndrome steps (fecl_ldpc_server_set_round_steps, default 1) on every
+ * pool holding live frames, retire finished frames.  Requests arriving while others run join
+ * at the next round; each frame's result equals a batch decode of it (frames share no state).
+ * Handles must outlive the server; each pool decodes with the options (and layout) its handle
+ * had when the server was created.
+ *   submit: LLRs [frames][N] (FP32) of code `code`, in host memory (pageable or page-locked)
+ *           or in device memory of the server's device (read in place; the caller's writes
+ *           must be complete when submit is called); msg [frames][K] (and iterations
+ *           [frames], nullable) receive the results in host memory; all three stay valid
+ *           until wait() reports done.
  *   wait:   blocks up to timeout_s (< 0: forever); done = 1 and latency_s = completion -
  *           submission time once finished (the ticket is then released). */
 typedef struct fecl_ldpc_server fecl_ldpc_server;
@@ -296,6 +299,10 @@ int fecl_ldpc_server_wait(fecl_ldpc_server* s, uint64_t ticket, double timeout_s
                           double* latency_s);
 int fecl_ldpc_server_stats(fecl_ldpc_server* s, int64_t* frames_done, int64_t* steps,
                            int64_t* queued_frames);
+/* Steps per round (1..64; default 1), from the next round on.  More steps cost fewer host
+ * round trips per step, but a finished frame waits up to steps - 1 more steps for its
+ * retirement and admissions come less often. */
+int fecl_ldpc_server_set_round_steps(fecl_ldpc_server* s, int32_t steps);
 int fecl_ldpc_server_destroy(fecl_ldpc_server* s);
 
 /* Ldpc::puncturing (src/Ldpc.h:94, src/detail/Ldpc.cpp:333-353): the index sequence of FeCl's

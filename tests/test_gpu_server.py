@@ -1,5 +1,6 @@
-"""Continuous-batching server: requests of mixed codes and sizes, submitted while others run,
-each decoded exactly as it is alone (frames share no state), plus admission by memory."""
+"""Continuous-batching server: requests of mixed codes and sizes, from host or device memory,
+submitted while others run, each decoded exactly as it is alone (frames share no state), with
+one or several steps per round; plus admission by memory and the round-steps bounds."""
 import threading
 import time
 
@@ -23,35 +24,38 @@ def decoders(fmt=F.MsgFormat.F16, iters=50):
     return out
 
 
-@pytest.mark.parametrize("pinned", [False, True])
-def test_server_results_same_alone_and_in_the_mix(pinned):
-    """pinned: the LLRs live in page-locked host memory, which the server copies to the device
-    directly instead of through its staging buffer."""
+@pytest.mark.parametrize("inputs,steps", [("pageable", 1), ("pinned", 4), ("device", 4), ("mixed", 1),
+                                          ("mixed", 4), ("mixed", 7)])
+def test_server_results_same_alone_and_in_the_mix(inputs, steps):
+    """inputs: where the requests' LLRs live: pageable host memory (staged by the server),
+    page-locked host memory (copied to the device directly), device memory (read in place by
+    the admission kernels), or a random mix of the three within each round.  steps: steps per
+    round between the host's admissions and retirements."""
     import torch
     decs = decoders()
-    rng = np.random.default_rng(5)
+    rng = np.random.default_rng(5 + steps)
     pool = {i: dvbs2_llr(200, CODES[i][2], seed=50 + i, rate=CODES[i][1], n=CODES[i][0]).astype(np.float32)
             for i in range(len(CODES))}
-    keep = []
-    if pinned:
-        for i in pool:
-            t = torch.from_numpy(pool[i]).pin_memory()
-            keep.append(t)
-            pool[i] = t.numpy()
+    pinned = {i: torch.from_numpy(pool[i]).pin_memory() for i in pool}
+    dev = {i: torch.from_numpy(pool[i]).cuda() for i in pool}
+    torch.cuda.synchronize()
     reqs = []
     for _ in range(40):
         i = int(rng.integers(0, len(CODES)))
         f = int(rng.integers(1, 40))
         o = int(rng.integers(0, 200 - f))
-        reqs.append((i, pool[i][o:o + f]))
+        kind = inputs if inputs != "mixed" else ("pageable", "pinned", "device")[int(rng.integers(0, 3))]
+        src = {"pageable": pool[i][o:o + f], "pinned": pinned[i].numpy()[o:o + f], "device": dev[i][o:o + f]}[kind]
+        reqs.append((i, pool[i][o:o + f], src))
     # small pools so that requests wait for slots and join mid-flight
     tw = [d.layout()[1] for d in decs]
     sv = F.Server(decs, slots=[tw[0], tw[1], 2 * tw[2]])
+    sv.setRoundSteps(steps)
     results = [None] * len(reqs)
 
     def submitter():
-        for k, (i, x) in enumerate(reqs):
-            results[k] = sv.submit(i, x)
+        for k, (i, _, src) in enumerate(reqs):
+            results[k] = sv.submit(i, src)
             time.sleep(0.002)
 
     th = threading.Thread(target=submitter)
@@ -61,14 +65,14 @@ def test_server_results_same_alone_and_in_the_mix(pinned):
     st = sv.stats()
     sv.close()
     assert all(g is not None for g in got), "server did not finish"
-    assert st["frames_done"] == sum(x.shape[0] for _, x in reqs)
-    for (i, x), (msg, it, lat) in zip(reqs, got):
+    assert st["frames_done"] == sum(x.shape[0] for _, x, _ in reqs)
+    for (i, x, _), (msg, it, lat) in zip(reqs, got):
         alone, ait = decs[i].decode(x.astype(np.float64), iterations=True)
         assert np.array_equal(msg, alone) and np.array_equal(it, ait)
         assert lat > 0
     # and against the oracle for one request per code
     for i in range(len(CODES)):
-        k = next(k for k, (j, _) in enumerate(reqs) if j == i)
+        k = next(k for k, (j, _, _) in enumerate(reqs) if j == i)
         x = reqs[k][1]
         n, rate, _ = CODES[i]
         ref = O.decode(O.dvbs2(n, *rate), x.astype(np.float64), O.APPROXIMATE, 50, prec=32, msg_fmt=O.MSG_F16,
@@ -88,3 +92,14 @@ def test_server_admission_by_memory_budget():
         total += slots
     sv.close()
     assert total > 0
+
+
+def test_server_round_steps_bounds():
+    decs = decoders()
+    sv = F.Server(decs, slots=[d.layout()[1] for d in decs])
+    for bad in (0, 65, -1):
+        with pytest.raises(ValueError):
+            sv.setRoundSteps(bad)
+    sv.setRoundSteps(64)
+    sv.setRoundSteps(1)
+    sv.close()
