@@ -27,6 +27,7 @@
 #include "ldpc_code.h"
 #include "ldpc_engine.h"
 #include "ldpc_launch.h"
+#include "ldpc_tune_cache.h"
 
 namespace fecl {
 namespace engine {
@@ -1249,6 +1250,30 @@ int fecl_ldpc_autotune(fecl_ldpc* h, int64_t frames, int32_t probe_iterations, i
   if (frames < 1 || probe_iterations < 1) return fail(FECL_INVALID_ARGUMENT, "frames and probe_iterations must be >= 1");
   const Code& c = h->code;
   HIP_TRY(hipSetDevice(h->device));
+  // a pick kept for this GPU, code, algorithm, message format and batch size is pinned as is
+  tune::configure_from_env();
+  hipDeviceProp_t prop{};
+  HIP_TRY(hipGetDeviceProperties(&prop, h->device));
+  const std::string key = tune::make_key(prop.gcnArchName, prop.multiProcessorCount, c, h->algorithm, h->msg_format,
+                                         frames);
+  {
+    tune::Pick kept;
+    if (tune::lookup(key, &kept)) {
+      Layout lay;
+      lay.fmt = h->msg_format;
+      lay.words = kept.words;
+      lay.vpw = kept.vpw;
+      if (layout_fits(lay, h->algorithm, c.max_row_degree, c.max_var_degree) && vpw_fits(lay, c.max_var_degree)) {
+        h->words_pinned = kept.words;
+        h->vpw_pinned = kept.vpw;
+        choose_layout(h);
+        if (ms)
+          for (int i = 0; i < 6; ++i) ms[i] = kept.ms[i];
+        if (best_words) *best_words = h->words;
+        return FECL_OK;
+      }
+    }
+  }
   // Noise-only LLRs: no frame converges, so every probe iteration runs for every frame.
   std::vector<float> host((size_t)frames * c.N);
   uint64_t x = 0x9E3779B97F4A7C15ull;
@@ -1318,6 +1343,29 @@ int fecl_ldpc_autotune(fecl_ldpc* h, int64_t frames, int32_t probe_iterations, i
   h->vpw_pinned = best_v;
   choose_layout(h);
   if (best_words) *best_words = h->words;
+  tune::Pick pick;
+  pick.words = best_w;
+  pick.vpw = best_v;
+  if (ms)
+    for (int i = 0; i < 6; ++i) pick.ms[i] = ms[i];
+  std::string err;
+  if (!tune::store(key, pick, &err)) return fail(FECL_INVALID_ARGUMENT, err);
+  return FECL_OK;
+}
+
+int fecl_autotune_cache_config(int32_t mode, const char* path) {
+  std::string err;
+  if (!tune::configure(mode, path, &err)) return fail(FECL_INVALID_ARGUMENT, err);
+  return FECL_OK;
+}
+
+int fecl_autotune_cache_stats(int64_t* hits, int64_t* misses, int64_t* entries, int64_t* skipped_lines) {
+  tune::stats(hits, misses, entries, skipped_lines);
+  return FECL_OK;
+}
+
+int fecl_autotune_cache_clear(void) {
+  tune::clear();
   return FECL_OK;
 }
 

@@ -12,6 +12,7 @@ Host arrays (numpy float64, FeCl's type) go through the host entry points; torch
 tensors stay on the device and run on the current stream.
 """
 import ctypes
+import os
 import enum
 from fractions import Fraction
 
@@ -641,6 +642,26 @@ def puncturing(systSize, paritySize, options):
     check(lib().fecl_ldpc_puncturing(int(systSize), int(paritySize), _p(m) if m.size else None, m.size,
                                      _p(sm) if sm.size else None, sm.size, _p(seq), seq.size, ctypes.byref(n)))
     return Permutation(seq[:n.value], paritySize)
+
+
+def autotune_cache(mode=1, path=None):
+    """Where Ldpc.autotune keeps its picks (no counterpart in the reference): 0 = nowhere, 1 = this
+    process (the default), 2 = this process and the text file `path` (read now, appended with
+    every new pick).  Picks are keyed by GPU architecture, code, algorithm, message format and
+    batch size rounded up to a power of two."""
+    check(lib().fecl_autotune_cache_config(int(mode), None if path is None else os.fsencode(path)))
+
+
+def autotune_cache_stats():
+    """{"hits", "misses", "entries", "skipped_lines"} of the autotune cache."""
+    v = [ctypes.c_int64() for _ in range(4)]
+    check(lib().fecl_autotune_cache_stats(*[ctypes.byref(x) for x in v]))
+    return dict(zip(("hits", "misses", "entries", "skipped_lines"), (x.value for x in v)))
+
+
+def autotune_cache_clear():
+    """Forgets the picks held in this process (a cache file is left as it is)."""
+    check(lib().fecl_autotune_cache_clear())
 
 
 class BatchGraph:

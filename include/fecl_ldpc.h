@@ -243,6 +243,17 @@ int fecl_ldpc_set_compaction(fecl_ldpc* h, int32_t enable);
 int fecl_ldpc_set_cache_policy(fecl_ldpc* h, int32_t policy);
 int fecl_ldpc_autotune(fecl_ldpc* h, int64_t frames, int32_t probe_iterations, int32_t* best_words,
                        double* ms /*[6]*/);
+/* Autotune cache (no counterpart in the reference).  fecl_ldpc_autotune keeps each pick under
+ * (GPU architecture and compute-unit count, code: sizes and a hash of H, algorithm, message
+ * format, frames rounded up to a power of two); a later autotune with the same key pins the
+ * kept pick without probing (ms[] then holds the kept probe times).  mode 0 = off, 1 = this
+ * process (the default), 2 = this process and the text file `path`, read now and appended with
+ * every new pick.  Without a config call, the environment variable FECL_AUTOTUNE_CACHE names
+ * such a file at the first autotune.  stats: lookups that hit and missed, picks held, and file
+ * lines skipped as malformed.  clear forgets the picks held in this process (not the file). */
+int fecl_autotune_cache_config(int32_t mode, const char* path);
+int fecl_autotune_cache_stats(int64_t* hits, int64_t* misses, int64_t* entries, int64_t* skipped_lines);
+int fecl_autotune_cache_clear(void);
 
 /* Multi-code batch in one launch (no counterpart in the reference: FeCl decodes one code per
  * call).  Captures, for each i < count, a full decode of frames[i] frames of code handles[i]
