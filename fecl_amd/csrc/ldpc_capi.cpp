@@ -26,6 +26,7 @@
 #include "ldpc_archive.h"
 #include "ldpc_code.h"
 #include "ldpc_engine.h"
+#include "ldpc_host_convert.h"
 #include "ldpc_launch.h"
 #include "ldpc_tune_cache.h"
 
@@ -446,58 +447,6 @@ int check_handle(const fecl_ldpc* h) {
 // buffers, used by chunks of alternating parity.
 constexpr int64_t kHostChunkFrames = 512;
 constexpr int64_t kHostFirstChunkFrames = 256;   // smaller: its conversion overlaps nothing
-constexpr unsigned kConvertThreads = 8;
-
-// Runs body(a, b) over [0, n) split across up to kConvertThreads threads.
-// Host-buffer calls in flight in this process: FeCl's fan-out calls the hooks from
-// workGroupSize threads at once, and the conversion threads are shared out among them.
-std::atomic<int> g_host_calls{0};
-
-template <class Body>
-void parallel_ranges(size_t n, Body body) {
-  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-  const size_t share = std::max<size_t>(1, kConvertThreads / (size_t)std::max(1, g_host_calls.load()));
-  const size_t threads = std::max<size_t>(1, std::min<size_t>({share, hw, n / 65536 + 1}));
-  const size_t per = (n + threads - 1) / threads;
-  auto part = [&](size_t t) { body(std::min(n, per * t), std::min(n, per * (t + 1))); };
-  std::vector<std::thread> pool;
-  for (size_t t = 1; t < threads; ++t) pool.emplace_back(part, t);
-  part(0);
-  for (auto& th : pool) th.join();
-}
-
-// dst[i] = (float)src[i]: round to nearest even, as the device's double-to-float conversion.
-void to_float(const double* src, float* dst, size_t n) {
-  parallel_ranges(n, [=](size_t a, size_t b) {
-    for (size_t i = a; i < b; ++i) dst[i] = (float)src[i];
-  });
-}
-
-// A-priori inputs on the first K bits (soDecode): dst[f][v] = (float)((double)(float)llr[f][v]
-// + syst[f][v]) for v < K, (float)llr[f][v] otherwise.  FeCl's decoder instantiated with a
-// float LLR type stores the channel value as float, then adds the double a-priori value and
-// rounds once (src/detail/BpDecoder/BpDecoderImpl.cpp:75-80); rounding syst to float first
-// would round twice.
-void to_float_plus(const double* llr, const double* syst, float* dst, int64_t frames, int64_t N, int64_t K) {
-  parallel_ranges((size_t)frames * N, [=](size_t a, size_t b) {
-    for (size_t i = a; i < b; ++i) {
-      const size_t f = i / (size_t)N, v = i % (size_t)N;
-      const float x = (float)llr[i];
-      dst[i] = v < (size_t)K ? (float)((double)x + syst[f * (size_t)K + v]) : x;
-    }
-  });
-}
-
-void to_double(const float* src, double* dst, size_t n) {
-  parallel_ranges(n, [=](size_t a, size_t b) {
-    for (size_t i = a; i < b; ++i) dst[i] = (double)src[i];
-  });
-}
-
-void copy_bytes(const void* src, void* dst, size_t n) {
-  parallel_ranges(n, [=](size_t a, size_t b) { std::memcpy((char*)dst + a, (const char*)src + a, b - a); });
-}
-
 int host_decode(fecl_ldpc* h, const double* llr, const double* syst_in, const double* state_in, int64_t frames,
                 uint8_t* msg_bits, double* msg, double* syst, double* parity, double* state, int32_t* iterations) {
   if (int rc = check_handle(h)) return rc;

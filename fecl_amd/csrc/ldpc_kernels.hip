@@ -572,6 +572,13 @@ __global__ void fill_i32_kernel(int32_t* p, int64_t n, int32_t v) {
   if (i < n) p[i] = v;
 }
 
+// Several byte ranges copied in one launch (workgroup y copies range y, grid-stride in x).
+__global__ __launch_bounds__(kBlock) void copy_bytes_kernel(ByteCopies c) {
+  const ByteCopy& r = c.item[blockIdx.y];
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < r.n; i += (int64_t)gridDim.x * kBlock)
+    r.dst[i] = r.src[i];
+}
+
 // ------------------------------------------------------------------------------------------
 // Permutation::permute / dePermute of FP32 blocks (src/Permutation.h): a grid-stride loop over
 // (frame, output index); dePermute zeroes its output first.
@@ -890,6 +897,17 @@ static hipError_t launch_compact_in_place(const DevCode& c, const DevState& s, c
 hipError_t kernels_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t st) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(fill_i32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t kernels_copy_bytes(const ByteCopies& c, hipStream_t st) {
+  if (c.count <= 0) return hipSuccess;
+  if (c.count > kMaxByteCopies) return hipErrorInvalidValue;
+  int64_t longest = 0;
+  for (int i = 0; i < c.count; ++i) longest = std::max<int64_t>(longest, c.item[i].n);
+  const unsigned blocks = (unsigned)std::min<int64_t>(64, (longest + kBlock - 1) / kBlock);
+  if (blocks == 0) return hipSuccess;
+  hipLaunchKernelGGL(copy_bytes_kernel, dim3(blocks, (unsigned)c.count), dim3(kBlock), 0, st, c);
   return hipGetLastError();
 }
 

@@ -42,6 +42,19 @@ bool vpw_fits(const Layout& lay, int max_dv);
 inline int var_table_stride(int max_dv) { return max_dv <= 4 ? 4 : max_dv <= 8 ? 8 : max_dv <= 16 ? 16 : 32; }
 
 hipError_t kernels_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t st);
+// Byte ranges copied by one kernel launch, in stream order (the serving loop snapshots every
+// pool's frame phases into mapped page-locked memory with it).
+struct ByteCopy {
+  const uint8_t* src;
+  uint8_t* dst;
+  int64_t n;
+};
+constexpr int kMaxByteCopies = 32;
+struct ByteCopies {
+  ByteCopy item[kMaxByteCopies];
+  int32_t count;
+};
+hipError_t kernels_copy_bytes(const ByteCopies& c, hipStream_t st);
 // Permutation of FP32 blocks: out[f][i] = in[f][seq[i]] (permute), or out[f][seq[i]] = in[f][i]
 // with the other entries zeroed (inverse).
 hipError_t kernels_permute(const int64_t* seq, int64_t out_size, int64_t in_size, const float* in, float* out,

@@ -117,7 +117,7 @@ struct fecl_ldpc_server {
   int steps_per_round = kDefaultRoundSteps;   // read by the worker at each round (under mu)
   hipStream_t st = nullptr;      // admissions' kernels, steps, retirements' extracts
   hipStream_t copy = nullptr;    // admissions' host-to-device copies, overlapping the steps
-  hipStream_t out = nullptr;     // phases and results to the host, overlapping the steps
+  hipStream_t out = nullptr;     // results to the host, overlapping the steps
   int device = 0;
   std::thread worker;
   ~fecl_ldpc_server() {
@@ -151,10 +151,10 @@ constexpr int kAdmitPerStep = 128;
 // Host buffers come in pairs indexed by the round's parity, so the host never touches a
 // buffer the device may still be using.
 struct Events {
-  hipEvent_t round[2] = {nullptr, nullptr}, ext[2] = {nullptr, nullptr};   // on the out stream
+  hipEvent_t round[2] = {nullptr, nullptr};     // round e's phases in host memory (compute stream)
+  hipEvent_t ext[2] = {nullptr, nullptr};       // results of the extract queued after round q (out stream)
   hipEvent_t copied[2] = {nullptr, nullptr};    // admissions' copies of round e (copy stream)
   hipEvent_t admitted[2] = {nullptr, nullptr};  // admission kernels of round e (compute stream)
-  hipEvent_t stepped[2] = {nullptr, nullptr};   // steps of round e (compute stream)
   hipEvent_t extracted[2] = {nullptr, nullptr}; // extracts queued after round q (compute stream)
   hipError_t create() {
     for (hipEvent_t* e : all())
@@ -164,7 +164,7 @@ struct Events {
   std::vector<hipEvent_t*> all() {
     std::vector<hipEvent_t*> v;
     for (int i = 0; i < 2; ++i)
-      for (hipEvent_t* e : {&round[i], &ext[i], &copied[i], &admitted[i], &stepped[i], &extracted[i]}) v.push_back(e);
+      for (hipEvent_t* e : {&round[i], &ext[i], &copied[i], &admitted[i], &extracted[i]}) v.push_back(e);
     return v;
   }
   ~Events() {
@@ -234,7 +234,7 @@ bool choose_admissions(fecl_ldpc_server* sv, int64_t e, int steps, std::vector<s
 
 // Device side of round e: admission copies of host frames (on the copy stream, so they
 // overlap the steps of round e - 1 still running) and admission kernels, `steps` steps per
-// pool, phase copies.  Buffers of parity e & 1 were last used by round e - 2, which the host
+// pool, the phase snapshot.  Buffers of parity e & 1 were last used by round e - 2, which the host
 // has already seen complete, so they may be refilled or grown.
 int enqueue_round(fecl_ldpc_server* sv, Events* ev, int64_t e, int steps, const std::vector<std::vector<Admit>>& admits) {
   const int b = (int)(e & 1);
@@ -319,13 +319,16 @@ int enqueue_round(fecl_ldpc_server* sv, Events* ev, int64_t e, int steps, const 
                                        P.h->code.max_row_degree, sv->st));
     }
   }
-  HIP_TRY(hipEventRecord(ev->stepped[b], sv->st));
-  HIP_TRY(hipStreamWaitEvent(sv->out, ev->stepped[b], 0));
-  for (auto& pp : sv->pools) {   // every pool, so that each round's phases are complete
+  // every pool's phases as this round leaves them, written by one kernel in stream order into
+  // mapped page-locked memory: the next round's admissions and steps, queued behind it, cannot
+  // change them while they are copied (a copy on another stream would race with them)
+  ByteCopies snap{};
+  for (auto& pp : sv->pools) {
     Pool& P = *pp;
-    HIP_TRY(hipMemcpyAsync(P.h_phase[b].p, P.s.phase, (size_t)P.nslots, hipMemcpyDeviceToHost, sv->out));
+    snap.item[snap.count++] = ByteCopy{P.s.phase, static_cast<uint8_t*>(P.h_phase[b].dev), P.nslots};
   }
-  HIP_TRY(hipEventRecord(ev->round[b], sv->out));
+  HIP_TRY(kernels_copy_bytes(snap, sv->st));
+  HIP_TRY(hipEventRecord(ev->round[b], sv->st));
   std::lock_guard<std::mutex> lk(sv->mu);
   sv->steps += steps;
   return FECL_OK;
@@ -487,6 +490,7 @@ extern "C" {
 int fecl_ldpc_server_create(fecl_ldpc* const* handles, int32_t count, const int64_t* slots, int64_t memory_bytes,
                             fecl_ldpc_server** out) {
   if (!out || !handles || count < 1) return fail(FECL_INVALID_ARGUMENT, "invalid arguments");
+  if (count > kMaxByteCopies) return fail(FECL_INVALID_ARGUMENT, "at most 32 codes per server");
   *out = nullptr;
   std::unique_ptr<fecl_ldpc_server> sv(new fecl_ldpc_server());
   sv->device = handles[0]->device;

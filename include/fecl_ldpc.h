@@ -285,7 +285,7 @@ int fecl_ldpc_graph_destroy(fecl_ldpc_graph* g);
 int fecl_ldpc_set_graph_persistent(fecl_ldpc* h, int32_t enable);
 
 /* Continuous-batching decode server (no counterpart in the reference).  Each of the `count`
- * codes gets a pool of device frame slots: slots[i] frames, or, where slots is NULL or
+ * (at most 32) codes gets a pool of device frame slots: slots[i] frames, or, where slots is NULL or
  * slots[i] <= 0, a share of memory_bytes (<= 0: half of the device's free memory) sized by one
  * slot's footprint.  A worker thread repeats rounds: admit queued frames into free slots, run
  * check/variable/syndrome steps (fecl_ldpc_server_set_round_steps, default 1) on every
