@@ -14,6 +14,7 @@ stream), and the CPU baseline (stock FeCl built from the reference sources, or t
 restatement when that build is absent) timed on a bounded sample of the same frames.
 """
 import argparse
+import gc
 import json
 import os
 import subprocess
@@ -166,11 +167,13 @@ def run_uneven(args):
             d.decode(x)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    eager_out = []
     for _ in range(args.steps):
-        for d, x in zip(decs, llrs):
-            d.decode(x)
+        eager_out = [d.decode(x, iterations=True) for d, x in zip(decs, llrs)]
     torch.cuda.synchronize()
     eager = time.perf_counter() - t1
+    if not all(torch.equal(m, e[0]) and torch.equal(it, e[1]) for m, it, e in zip(g.msg, g.iterations, eager_out)):
+        raise RuntimeError("the graph's results differ from eager decodes")
     # the persistent form of the graph (one kernel for all codes; one lane layout, so the
     # decoders keep the automatic one), checked against the per-step graph's outputs
     pdecs = []
@@ -207,6 +210,7 @@ def run_uneven(args):
             "ms_per_launch": round(total / args.steps * 1e3, 3),
             "eager_per_rate_calls_Gb_per_s": round(info_bits / eager / 1e9, 4),
             "persistent_graph_Gb_per_s": round(info_bits / persistent / 1e9, 4) if persistent else None,
+            "identical_results": "per-step graph, eager calls and persistent kernel: same bits and iteration counts",
             "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
                            "p99": round(float(np.percentile(lat, 99)) * 1e3, 3)},
             "dtype": "f16", "per_rate": per_rate}
@@ -245,10 +249,14 @@ class ServingSetup:
 
     def eager_frames_per_s(self, reps=3):
         """Each rate's pool decoded by one eager decode() call from device memory: frames per
-        second over the seven calls (the serving mix has equal frames per rate)."""
+        second over the seven calls (the serving mix has equal frames per rate).  The bits and
+        iteration counts of that decode are kept: every served request must equal its slice."""
         import torch
+        self.eager_bits, self.eager_iters = [], []
         for d, x in zip(self.decs, self.dev):
-            d.decode(x)
+            bits, it = d.decode(x, iterations=True)
+            self.eager_bits.append(bits.view(self.POOL, d.msgSize()).cpu().numpy())
+            self.eager_iters.append(it.cpu().numpy())
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(reps):
@@ -286,18 +294,27 @@ def run_serving(F, setup, args, duration_s, offered_fps, seed, device_inputs=Tru
         tickets.append((ri, o, f, sv.submit(ri, src)))
     t_sub_end = time.perf_counter()
     offered_frames = sum(f for _, _, f, _ in tickets)
-    lats, errs, frames, bits = [], 0, 0, 0
+    lats, frames, bits, results = [], 0, 0, []
     for ri, o, f, t in tickets:
         r = sv.wait(t, timeout=600.0)
         if r is None:
             raise RuntimeError("serving request timed out")
         msg, it, lat = r
         lats.append(lat)
-        K = setup.decs[ri].msgSize()
-        errs += int(np.any(msg.reshape(f, K) != setup.infos[ri][o:o + f], axis=1).sum())
+        results.append((ri, o, f, msg, it))
         frames += f
-        bits += f * K
+        bits += f * setup.decs[ri].msgSize()
     t_end = time.perf_counter()
+    # checks, outside the timed region: FER against the sent bits, and every request against
+    # the same frames decoded alone (one eager batch of the rate's pool)
+    errs, differ = 0, 0
+    for ri, o, f, msg, it in results:
+        K = setup.decs[ri].msgSize()
+        m = msg.reshape(f, K)
+        errs += int(np.any(m != setup.infos[ri][o:o + f], axis=1).sum())
+        if not (np.array_equal(m, setup.eager_bits[ri][o:o + f]) and
+                np.array_equal(np.asarray(it), setup.eager_iters[ri][o:o + f])):
+            differ += 1
     st = sv.stats()
     sv.close()
     wall = t_end - t_start
@@ -307,7 +324,8 @@ def run_serving(F, setup, args, duration_s, offered_fps, seed, device_inputs=Tru
             "throughput_frames_per_s": round(frames / wall, 1), "info_Gb_per_s": round(bits / wall / 1e9, 4),
             "latency_ms": {"p50": round(float(np.percentile(lats, 50)) * 1e3, 2),
                            "p99": round(float(np.percentile(lats, 99)) * 1e3, 2), "unit": "per request"},
-            "fer": round(errs / max(1, frames), 5), "server_steps": st["steps"], "slots_per_rate": slots[0],
+            "fer": round(errs / max(1, frames), 5), "requests_differing_from_eager_decode": differ,
+            "server_steps": st["steps"], "slots_per_rate": slots[0],
             "steps_per_round": args.serve_round_steps}
 
 
@@ -813,6 +831,7 @@ def main():
     ap.add_argument("--serve-round-steps", type=int, default=1, help="server steps per round")
     ap.add_argument("--offered-fps", type=float, default=200000.0)
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-uneven", action="store_true", help="skip the seven-rate uneven batch leg")
     ap.add_argument("--no-waterfall", action="store_true", help="skip the FP16/BF16 vs FP64 waterfall FER leg")
     ap.add_argument("--no-forms", action="store_true", help="skip the Linear / FP16 / BF16 lines")
     ap.add_argument("--ebn0", type=str, default="",
@@ -1058,16 +1077,26 @@ def main():
                                   "cpu": _cpu_model(), "host_cpus": cores_info}
         if not args.no_waterfall:
             result["waterfall_fer_vs_fp64"] = run_waterfall(F, H, device, threads, args.iterations, args.seed)
-    if not args.no_serving:
-        # the serving pools start from a device heap without the batch bench's buffers (decoder
-        # workspaces, input batches, torch's cached blocks)
+    if not args.no_serving or not args.no_uneven:
+        # the uneven batch and the serving pools start from a device heap without the batch
+        # bench's buffers (decoder workspaces, input batches, torch's cached blocks)
         del dec, batches, out, kept
-        import gc
         gc.collect()
         torch.cuda.synchronize()
         torch.cuda.empty_cache()
+    if not args.no_serving:
         print(f"[bench] serving loop: {args.serve_seconds} s of arrivals", file=sys.stderr, flush=True)
         result["serving"] = serving_report(F, args, device, args.serve_seconds)
+        gc.collect()
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
+    # after serving: its slot pools are sensitive to a device heap that earlier legs fragmented
+    if not args.no_uneven:
+        print("[bench] uneven seven-rate batch", file=sys.stderr, flush=True)
+        ua = argparse.Namespace(**vars(args))
+        ua.steps, ua.warmup = 3, 1
+        result["uneven"] = run_uneven(ua)
+        print(f"[bench] uneven: {result['uneven']}", file=sys.stderr, flush=True)
     print(json.dumps(result), flush=True)
 
 
