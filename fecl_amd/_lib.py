@@ -51,7 +51,7 @@ SIGNATURES = {
     "fecl_ldpc_get_vars_per_wave": [P, P],
     "fecl_ldpc_autotune": [P, I64, I32, P, P],
     "fecl_autotune_cache_config": [I32, ctypes.c_char_p],
-    "fecl_autotune_cache_stats": [P, P, P, P],
+    "fecl_autotune_cache_stats": [P, P, P, P, P],
     "fecl_autotune_cache_clear": [],
     "fecl_ldpc_graph_create": [P, P, I32, P, P, P, P],
     "fecl_ldpc_graph_launch": [P, P],

@@ -1297,8 +1297,9 @@ int fecl_ldpc_autotune(fecl_ldpc* h, int64_t frames, int32_t probe_iterations, i
   pick.vpw = best_v;
   if (ms)
     for (int i = 0; i < 6; ++i) pick.ms[i] = ms[i];
+  // a cache file that cannot be appended does not undo the pick (stats count the failure)
   std::string err;
-  if (!tune::store(key, pick, &err)) return fail(FECL_INVALID_ARGUMENT, err);
+  (void)tune::store(key, pick, &err);
   return FECL_OK;
 }
 
@@ -1308,8 +1309,9 @@ int fecl_autotune_cache_config(int32_t mode, const char* path) {
   return FECL_OK;
 }
 
-int fecl_autotune_cache_stats(int64_t* hits, int64_t* misses, int64_t* entries, int64_t* skipped_lines) {
-  tune::stats(hits, misses, entries, skipped_lines);
+int fecl_autotune_cache_stats(int64_t* hits, int64_t* misses, int64_t* entries, int64_t* skipped_lines,
+                              int64_t* write_failures) {
+  tune::stats(hits, misses, entries, skipped_lines, write_failures);
   return FECL_OK;
 }
 

@@ -22,7 +22,7 @@ struct Cache {
   bool configured = false;   // configure() ran (the environment is then ignored)
   std::string path;
   std::unordered_map<std::string, Pick> map;
-  int64_t hits = 0, misses = 0, skipped = 0;
+  int64_t hits = 0, misses = 0, skipped = 0, write_failures = 0;
 };
 
 Cache& cache() {
@@ -126,9 +126,10 @@ bool configure(int mode, const char* path, std::string* err) {
       c.path.clear();
       return false;
     }
-    std::ifstream probe(c.path);
-    if (!probe) {   // create it with its header so that later appends need no check
-      std::ofstream out(c.path);
+    // a missing or empty file gets its header now, so that later appends need no check
+    std::ifstream probe(c.path, std::ios::ate);
+    if (!probe || probe.tellg() == 0) {
+      std::ofstream out(c.path, std::ios::trunc);
       if (!out || !(out << kHeader << '\n')) {
         if (err) *err = "autotune cache " + c.path + ": cannot create the file";
         c.mode = 1;
@@ -177,6 +178,7 @@ bool store(const std::string& key, const Pick& p, std::string* err) {
   if (c.mode == 2) {
     std::ofstream out(c.path, std::ios::app);
     if (!out || !(out << format_line(key, p))) {
+      ++c.write_failures;
       if (err) *err = "autotune cache " + c.path + ": cannot append";
       return false;
     }
@@ -184,20 +186,21 @@ bool store(const std::string& key, const Pick& p, std::string* err) {
   return true;
 }
 
-void stats(int64_t* hits, int64_t* misses, int64_t* entries, int64_t* skipped_lines) {
+void stats(int64_t* hits, int64_t* misses, int64_t* entries, int64_t* skipped_lines, int64_t* write_failures) {
   Cache& c = cache();
   std::lock_guard<std::mutex> lk(c.mu);
   if (hits) *hits = c.hits;
   if (misses) *misses = c.misses;
   if (entries) *entries = (int64_t)c.map.size();
   if (skipped_lines) *skipped_lines = c.skipped;
+  if (write_failures) *write_failures = c.write_failures;
 }
 
 void clear() {
   Cache& c = cache();
   std::lock_guard<std::mutex> lk(c.mu);
   c.map.clear();
-  c.hits = c.misses = c.skipped = 0;
+  c.hits = c.misses = c.skipped = c.write_failures = 0;
 }
 
 }  // namespace tune

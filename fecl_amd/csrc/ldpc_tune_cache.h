@@ -35,9 +35,10 @@ bool configure(int mode, const char* path, std::string* err);
 // Applies FECL_AUTOTUNE_CACHE once, unless configure() ran first.
 void configure_from_env();
 bool lookup(const std::string& key, Pick* out);
-// Keeps the pick (replacing an older one of the same key) and appends it to the file.
+// Keeps the pick (replacing an older one of the same key) and appends it to the file; returns
+// false (with *err) when the file cannot be appended (the pick is kept in the process anyway).
 bool store(const std::string& key, const Pick& p, std::string* err);
-void stats(int64_t* hits, int64_t* misses, int64_t* entries, int64_t* skipped_lines);
+void stats(int64_t* hits, int64_t* misses, int64_t* entries, int64_t* skipped_lines, int64_t* write_failures);
 void clear();
 
 }  // namespace tune

@@ -653,10 +653,10 @@ def autotune_cache(mode=1, path=None):
 
 
 def autotune_cache_stats():
-    """{"hits", "misses", "entries", "skipped_lines"} of the autotune cache."""
-    v = [ctypes.c_int64() for _ in range(4)]
+    """{"hits", "misses", "entries", "skipped_lines", "write_failures"} of the autotune cache."""
+    v = [ctypes.c_int64() for _ in range(5)]
     check(lib().fecl_autotune_cache_stats(*[ctypes.byref(x) for x in v]))
-    return dict(zip(("hits", "misses", "entries", "skipped_lines"), (x.value for x in v)))
+    return dict(zip(("hits", "misses", "entries", "skipped_lines", "write_failures"), (x.value for x in v)))
 
 
 def autotune_cache_clear():

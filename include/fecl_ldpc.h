@@ -249,10 +249,13 @@ int fecl_ldpc_autotune(fecl_ldpc* h, int64_t frames, int32_t probe_iterations, i
  * kept pick without probing (ms[] then holds the kept probe times).  mode 0 = off, 1 = this
  * process (the default), 2 = this process and the text file `path`, read now and appended with
  * every new pick.  Without a config call, the environment variable FECL_AUTOTUNE_CACHE names
- * such a file at the first autotune.  stats: lookups that hit and missed, picks held, and file
- * lines skipped as malformed.  clear forgets the picks held in this process (not the file). */
+ * such a file at the first autotune.  stats: lookups that hit and missed, picks held, file
+ * lines skipped as malformed, and appends that failed (a pick that cannot be written to the
+ * file is still pinned and kept in the process).  clear forgets the picks held in this process
+ * (not the file).  Any argument of stats may be NULL. */
 int fecl_autotune_cache_config(int32_t mode, const char* path);
-int fecl_autotune_cache_stats(int64_t* hits, int64_t* misses, int64_t* entries, int64_t* skipped_lines);
+int fecl_autotune_cache_stats(int64_t* hits, int64_t* misses, int64_t* entries, int64_t* skipped_lines,
+                              int64_t* write_failures);
 int fecl_autotune_cache_clear(void);
 
 /* Multi-code batch in one launch (no counterpart in the reference: FeCl decodes one code per

@@ -181,19 +181,36 @@ static void test_tune_cache() {
   }
   for (auto& x : th) x.join();
   EXPECT(mismatches == 0, "invalid picks read back");
-  int64_t hits = 0, misses = 0, entries = 0, skipped = 0;
-  tune::stats(&hits, &misses, &entries, &skipped);
-  EXPECT(hits == 1600 && entries > 0 && skipped == 0, "hits %lld entries %lld skipped %lld", (long long)hits,
-         (long long)entries, (long long)skipped);
+  int64_t hits = 0, misses = 0, entries = 0, skipped = 0, wfail = 0;
+  tune::stats(&hits, &misses, &entries, &skipped, &wfail);
+  EXPECT(hits == 1600 && entries > 0 && skipped == 0 && wfail == 0, "hits %lld entries %lld skipped %lld",
+         (long long)hits, (long long)entries, (long long)skipped);
   // a fresh view of the file holds the same keys
   tune::clear();
   EXPECT(tune::configure(2, path, &err), "reload: %s", err.c_str());
   int64_t entries2 = 0;
-  tune::stats(nullptr, nullptr, &entries2, &skipped);
+  tune::stats(nullptr, nullptr, &entries2, &skipped, nullptr);
   EXPECT(entries2 == entries && skipped == 0, "reloaded %lld of %lld entries", (long long)entries2,
          (long long)entries);
   EXPECT(tune::configure(1, nullptr, &err), "back to process-only");
   std::remove(path);
+  // a file that can no longer be appended: the pick stays in the process, the failure counts
+  char dir[] = "/tmp/fecl_tune_dirXXXXXX";
+  EXPECT(mkdtemp(dir) != nullptr, "mkdtemp");
+  const std::string in_dir = std::string(dir) + "/cache.txt";
+  EXPECT(tune::configure(2, in_dir.c_str(), &err), "configure in a directory: %s", err.c_str());
+  std::remove(in_dir.c_str());
+  rmdir(dir);
+  tune::Pick p;
+  p.words = 2;
+  p.vpw = 1;
+  const std::string k = tune::make_key("gfx950", 256, c, 0, 0, 64);
+  EXPECT(!tune::store(k, p, &err), "append into a removed directory reported");
+  tune::Pick q;
+  int64_t wf = 0;
+  tune::stats(nullptr, nullptr, nullptr, nullptr, &wf);
+  EXPECT(tune::lookup(k, &q) && q.words == 2 && wf == 1, "kept in process, failure counted (%lld)", (long long)wf);
+  EXPECT(tune::configure(1, nullptr, &err), "back to process-only");
   EXPECT(tune::frames_bucket(1) == 1 && tune::frames_bucket(1000) == 1024 && tune::frames_bucket(1024) == 1024,
          "frame buckets");
 }

@@ -24,6 +24,15 @@ def test_file_is_created_with_its_header(tmp_path):
     assert p.read_text() == HEADER
 
 
+def test_an_empty_file_gets_its_header(tmp_path):
+    p = tmp_path / "tune.txt"
+    p.write_text("")
+    F.autotune_cache(2, str(p))
+    assert p.read_text() == HEADER
+    st = F.autotune_cache_stats()
+    assert st["entries"] == 0 and st["write_failures"] == 0
+
+
 def test_file_entries_load_and_malformed_lines_are_skipped(tmp_path):
     p = tmp_path / "tune.txt"
     p.write_text(HEADER
