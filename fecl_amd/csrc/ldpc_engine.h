@@ -208,7 +208,12 @@ Layout layout_of(const fecl_ldpc* h);
 DevOpts make_opts(const fecl_ldpc* h);
 // Sizes ws for `frames` frames of h's layout and returns the device views.
 int prepare(fecl_ldpc* h, Workspace* ws, const Layout& lay, int64_t frames, bool soft, DevState* sp, DevOut* op);
-constexpr int64_t kMaxPassFrames = 8192;
+// Frames per pass unless fecl_ldpc_set_pass_frames says otherwise.  A pass runs every iteration
+// of its frames; the variable update starts on the tiles the check update wrote last, which the
+// 256 MB Infinity Cache still holds, and with four 256-frame tiles a quarter of the rows come
+// from there (4096-frame batches: 0.929 -> 0.959 Gb/s FP32, 1.742 -> 1.766 FP16 against one
+// pass; 2048-frame passes 0.932 and 1.730).
+constexpr int64_t kMaxPassFrames = 1024;
 // A compaction is enqueued only while at least this many steps may remain.
 constexpr int kCompactMinSteps = 4;
 // Captured graphs: a device-decided compaction after every this many steps (4, 6 and 8 measured

@@ -386,7 +386,7 @@ class Ldpc:
         check(lib().fecl_ldpc_set_cache_policy(self._h, int(policy)))
 
     def setPassFrames(self, frames):
-        """Decode batches in passes of at most `frames` frames (0 = 8192); results are the same."""
+        """Decode batches in passes of at most `frames` frames (0 = 1024); results are the same."""
         check(lib().fecl_ldpc_set_pass_frames(self._h, int(frames)))
 
     def setCompaction(self, enable):

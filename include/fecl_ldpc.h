@@ -223,7 +223,7 @@ int fecl_ldpc_get_layout(const fecl_ldpc* h, int32_t* words, int32_t* frames_per
  * A batch is decoded in consecutive passes of at most this many frames (rounded up to whole
  * tiles), each pass running all of its iterations before the next starts, so that a pass's
  * messages and channel values can stay in the device's last-level cache between
- * iterations.  0 (default) = 8192 frames per pass. */
+ * iterations.  0 (default) = 1024 frames per pass. */
 int fecl_ldpc_set_pass_frames(fecl_ldpc* h, int64_t frames);
 /* Compaction (no counterpart in the reference; results are identical either way).  A tile
  * of frames works until its last frame finishes, so once frames converge most tiles carry a
