@@ -839,6 +839,8 @@ def main():
     ap.add_argument("--compaction", choices=["on", "off"], default="on",
                     help="compaction of live frames into fewer tiles (results are the same)")
     ap.add_argument("--pass-frames", type=int, default=0, help="frames per decode pass (0 = library default)")
+    ap.add_argument("--cache-policy", type=int, default=-1,
+                    help="message-row cache hints: -1 automatic, else bit 0 check / bit 1 variable non-temporal")
     ap.add_argument("--vpw", type=int, default=0, help="variables per wavefront (0 automatic; with --words >= 0)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -893,6 +895,7 @@ def main():
                 "probe_ms": {f"w{k[0]}v{k[1]}": round(v, 3) for k, v in times.items()}}
         print(f"[bench] autotune: {tune}", file=sys.stderr, flush=True)
     dec.setPassFrames(args.pass_frames)
+    dec.setCachePolicy(args.cache_policy)
     dec.setCompaction(args.compaction == "on")
     words, frames_per_tile = dec.layout()
     points = tuple(float(x) for x in args.ebn0.split(",")) if args.ebn0 else EBN0
@@ -980,7 +983,7 @@ def main():
                    "lane_words": words, "frames_per_tile": frames_per_tile, "vars_per_wave": dec.varsPerWave(),
                    "compaction": args.compaction,
                    "autotune": tune,
-                   "pass_frames": args.pass_frames},
+                   "pass_frames": args.pass_frames, "cache_policy": args.cache_policy},
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 3),
                        "p99": round(float(np.percentile(lat, 99)) * 1e3, 3), "unit": "per batch decode"},
         "fer": round(errs / max(1, nfr), 5),
