@@ -1093,7 +1093,6 @@ def main():
         gc.collect()
         torch.cuda.synchronize()
         torch.cuda.empty_cache()
-    # after serving: its slot pools are sensitive to a device heap that earlier legs fragmented
     if not args.no_uneven:
         print("[bench] uneven seven-rate batch", file=sys.stderr, flush=True)
         ua = argparse.Namespace(**vars(args))
