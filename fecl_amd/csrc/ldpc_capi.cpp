@@ -242,7 +242,6 @@ int64_t pass_tiles(const fecl_ldpc* h, const Layout& lay, int64_t frames) {
 int prepare_graph_compaction(fecl_ldpc* h, Workspace* ws, int64_t frames) {
   const Layout lay = layout_of(h);
   const int64_t tiles = pass_tiles(h, lay, frames);
-  if (int rc = size_area(h->code, lay, area_of(ws, 1), tiles, false)) return rc;
   for (DevBuf* b : {&ws->map[0], &ws->map[1], &ws->src_of}) HIP_TRY(b->ensure((size_t)tiles * lay.tw() * 4));
   HIP_TRY(ws->ctl.ensure(sizeof(CompactCtl)));
   return FECL_OK;
@@ -313,19 +312,12 @@ int run_batch(fecl_ldpc* h, Workspace* ws, hipStream_t st, const BatchIO& io, in
   int cur_area = 0;
   const int32_t* cur_map = nullptr;
   // Captured graphs: no host sees the live counts, so after every kGraphCompactEvery-th step
-  // the device decides and compacts in place (live frames through scratch into the region's
-  // first slots); a checkpoint that declines costs a few launches that return at once.
+  // the device decides and compacts in place (live frames into the region's first slots, in
+  // one pass); a checkpoint that declines costs three launches that return at once.
   const size_t slot_bytes = (size_t)s.tiles * TW * 4;
   const bool graph_compact = h->compaction && capture && !o.fused_syn && s.tiles >= 2 &&
-                             ws->spare.msg.bytes >= (size_t)s.tiles * c.E * lay.rw() * 4 &&
-                             ws->spare.L.bytes >= (size_t)s.tiles * c.N * TW * 4 &&
-                             ws->spare.phase.bytes >= (size_t)s.tiles * TW && ws->spare.iter.bytes >= slot_bytes &&
-                             ws->spare.flags.bytes >= (size_t)s.tiles * 8 && ws->map[0].bytes >= slot_bytes &&
-                             ws->map[1].bytes >= slot_bytes && ws->src_of.bytes >= slot_bytes &&
-                             ws->ctl.bytes >= sizeof(CompactCtl);
-  DevState scratch{};
-  DevOut scratch_out{};
-  if (graph_compact) view_area(area_of(ws, 1), s.tiles, s.active, false, &scratch, &scratch_out);
+                             ws->map[0].bytes >= slot_bytes && ws->map[1].bytes >= slot_bytes &&
+                             ws->src_of.bytes >= slot_bytes && ws->ctl.bytes >= sizeof(CompactCtl);
   int checkpoint = 0;
 
   HIP_TRY(kernels_fill_i32(s.active, h->iterations, 0, st));
@@ -351,7 +343,7 @@ int run_batch(fecl_ldpc* h, Workspace* ws, hipStream_t st, const BatchIO& io, in
         int32_t* src_of = ws->src_of.as<int32_t>();
         HIP_TRY(kernels_compact_plan(lay, cur, cur_map, next_map, src_of, ctl, h->iterations - 1 - step, st));
         HIP_TRY(kernels_results(lay, h->dc, cur, cur_out, frames, frame0, io.res, io.dbl_out, cur_map, ctl, st));
-        HIP_TRY(kernels_compact_in_place(lay, h->dc, cur, scratch, src_of, ctl, st));
+        HIP_TRY(kernels_compact_in_place(lay, h->dc, cur, src_of, ctl, st));
         cur_map = next_map;
         ++checkpoint;
       }

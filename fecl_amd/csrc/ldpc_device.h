@@ -791,27 +791,20 @@ __device__ static inline bool control_tile(const DevState& s, int iterations, in
 // a row sits in word p / PW, bits 16 * (p % PW) up) from their source slots; empty slots get 0.
 // A workgroup covers kCompactEdges edges of one destination tile.
 constexpr int kCompactEdges = 16;
-// With ctl: nothing unless ctl->go, destination tiles past the live frames are skipped, and a
-// null src_of means slot j's source is slot j (the copy back from scratch).
-__device__ static inline int compact_source(const int32_t* src_of, const CompactCtl* ctl, int64_t j) {
-  if (ctl && j >= ctl->live) return -1;
-  return src_of ? src_of[j] : (int)j;
-}
 
 // Edges [e_begin, e_begin + kCompactEdges) of destination tile tb (every thread of the
 // workgroup calls it).
 template <class F>
 __device__ static inline void compact_msg_block(const DevCode& c, const uint32_t* src, uint32_t* dst,
-                                                const int32_t* src_of, const CompactCtl* ctl, int tb, int e_begin) {
+                                                const int32_t* src_of, int tb, int e_begin) {
   constexpr int TW = F::TW, RW = F::RW, PW = F::PW;
-  if (ctl && (!ctl->go || (int64_t)tb * TW >= ctl->live)) return;
   for (int i = threadIdx.x; i < kCompactEdges * RW; i += kBlock) {
     const int e = e_begin + i / RW, w = i % RW;
     if (e >= c.E) break;
     uint32_t word = 0;
 #pragma unroll
     for (int h = 0; h < PW; ++h) {
-      const int from = compact_source(src_of, ctl, (int64_t)tb * TW + w * PW + h);
+      const int from = src_of[(int64_t)tb * TW + w * PW + h];
       if (from >= 0) {
         const int ta = from / TW, pa = from % TW;
         const uint32_t sw = src[((int64_t)ta * c.E + e) * RW + pa / PW];
@@ -827,12 +820,11 @@ __device__ static inline void compact_msg_block(const DevCode& c, const uint32_t
 // tile tb.
 template <int TW>
 __device__ static inline void compact_L_block(const DevCode& c, const float* src, float* dst, const int32_t* src_of,
-                                              const CompactCtl* ctl, int tb, int v_begin) {
-  if (ctl && (!ctl->go || (int64_t)tb * TW >= ctl->live)) return;
+                                              int tb, int v_begin) {
   for (int i = threadIdx.x; i < kCompactEdges * TW; i += kBlock) {
     const int v = v_begin + i / TW, p = i % TW;
     if (v >= c.N) break;
-    const int from = compact_source(src_of, ctl, (int64_t)tb * TW + p);
+    const int from = src_of[(int64_t)tb * TW + p];
     float x = 0.0f;
     if (from >= 0) x = src[((int64_t)(from / TW) * c.N + v) * TW + from % TW];
     dst[((int64_t)tb * c.N + v) * TW + p] = x;
@@ -841,8 +833,8 @@ __device__ static inline void compact_L_block(const DevCode& c, const float* src
 
 // Phase and iteration count of destination slot j (an empty slot: PH_FREE).
 __device__ static inline void compact_slot(const uint8_t* src_phase, const int32_t* src_iter, uint8_t* dst_phase,
-                                           int32_t* dst_iter, const int32_t* src_of, const CompactCtl* ctl, int j) {
-  const int from = compact_source(src_of, ctl, j);
+                                           int32_t* dst_iter, const int32_t* src_of, int j) {
+  const int from = src_of[j];
   dst_phase[j] = from >= 0 ? src_phase[from] : (uint8_t)PH_FREE;
   dst_iter[j] = from >= 0 ? src_iter[from] : 0;
 }

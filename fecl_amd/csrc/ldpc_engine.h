@@ -82,8 +82,8 @@ struct Workspace {
   // Compaction: the batch's first region of tiles uses the buffers above; later regions
   // alternate between `spare` (three quarters as many tiles) and those buffers.  map[a]: slot -> frame
   // of the region in area a (area 0's first region needs none); src_of: the plan's scratch.
-  // Captured graphs compact in place instead: `spare` (as many tiles) is the scratch, map[]
-  // alternates per checkpoint, and ctl holds the device's decision (CompactCtl).
+  // Captured graphs compact in place instead: map[] alternates per checkpoint, and ctl holds
+  // the device's decision (CompactCtl).
   Area spare;
   DevBuf map[2], src_of, ctl;
   DevBuf stage_in[2], stage_state[2], stage_out[2];   // host path, double-buffered
@@ -216,9 +216,9 @@ int prepare(fecl_ldpc* h, Workspace* ws, const Layout& lay, int64_t frames, bool
 constexpr int64_t kMaxPassFrames = 1024;
 // A compaction is enqueued only while at least this many steps may remain.
 constexpr int kCompactMinSteps = 4;
-// Captured graphs: a device-decided compaction after every this many steps (4, 6 and 8 measured
-// on the uneven seven-rate batch: 3.06, 3.13 and 3.10 Gb/s).
-constexpr int kGraphCompactEvery = 6;
+// Captured graphs: a device-decided compaction after every this many steps (3, 4 and 6 measured
+// on the uneven seven-rate batch: 3.37, 3.40 and 3.37 Gb/s).
+constexpr int kGraphCompactEvery = 4;
 // Sizes ws's in-place compaction buffers for a captured decode of `frames` frames (before
 // capture: nothing may allocate while a graph is recorded).
 int prepare_graph_compaction(fecl_ldpc* h, Workspace* ws, int64_t frames);

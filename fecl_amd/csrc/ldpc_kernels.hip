@@ -471,9 +471,9 @@ __global__ void iters_out_kernel(const int32_t* iter, int64_t n, int64_t frames,
 // get -1.  No write goes past dst_slots.
 // With ctl (a captured graph, where no host sees the counts): the kernel decides.  It moves
 // the frames only when they fit in fewer tiles than hold live frames now, and the tile-steps
-// freed over the `remaining` steps outweigh moving each live frame twice (into scratch and
-// back, 1.5 frame-iterations a move).  Otherwise map_out[j] = map_in[j] for every slot and
-// src_of is left alone.  ctl->go and ctl->live record the outcome.
+// freed over the `remaining` steps outweigh moving each live frame once (counted as 1.5
+// frame-iterations, as the eager decode counts it).  Otherwise map_out[j] = map_in[j] for
+// every slot and src_of is left alone.  ctl->go and ctl->live record the outcome.
 constexpr int kPlanThreads = 1024;
 __global__ __launch_bounds__(kPlanThreads) void compact_plan_kernel(const uint8_t* phase, const int32_t* map_in,
                                                                     int32_t slots, int32_t* src_of, int32_t* map_out,
@@ -514,7 +514,7 @@ __global__ __launch_bounds__(kPlanThreads) void compact_plan_kernel(const uint8_
   const int total = sums[kPlanThreads - 1];
   if (ctl) {
     const int64_t need = (total + tw - 1) / tw, freed = (int64_t)tiles_live - need;
-    const bool go = freed > 0 && 2 * freed * remaining * tw > 6 * (int64_t)total;
+    const bool go = freed > 0 && 2 * freed * remaining * tw > 3 * (int64_t)total;
     if (tid == 0) {
       ctl->go = go ? 1 : 0;
       ctl->live = total;
@@ -543,28 +543,114 @@ __global__ __launch_bounds__(kPlanThreads) void compact_plan_kernel(const uint8_
 
 template <class F>
 __global__ __launch_bounds__(kBlock) void compact_msg_kernel(DevCode c, const uint32_t* src, uint32_t* dst,
-                                                             const int32_t* src_of, const CompactCtl* ctl) {
-  compact_msg_block<F>(c, src, dst, src_of, ctl, blockIdx.y, blockIdx.x * kCompactEdges);
+                                                             const int32_t* src_of) {
+  compact_msg_block<F>(c, src, dst, src_of, blockIdx.y, blockIdx.x * kCompactEdges);
 }
 
 template <int TW>
 __global__ __launch_bounds__(kBlock) void compact_L_kernel(DevCode c, const float* src, float* dst,
-                                                           const int32_t* src_of, const CompactCtl* ctl) {
-  compact_L_block<TW>(c, src, dst, src_of, ctl, blockIdx.y, blockIdx.x * kCompactEdges);
+                                                           const int32_t* src_of) {
+  compact_L_block<TW>(c, src, dst, src_of, blockIdx.y, blockIdx.x * kCompactEdges);
 }
 
 __global__ void compact_slots_kernel(const uint8_t* src_phase, const int32_t* src_iter, uint8_t* dst_phase,
-                                     int32_t* dst_iter, const int32_t* src_of, int32_t dst_slots,
-                                     const CompactCtl* ctl) {
+                                     int32_t* dst_iter, const int32_t* src_of, int32_t dst_slots) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= dst_slots || (ctl && !ctl->go)) return;
-  compact_slot(src_phase, src_iter, dst_phase, dst_iter, src_of, ctl, j);
+  if (j >= dst_slots) return;
+  compact_slot(src_phase, src_iter, dst_phase, dst_iter, src_of, j);
 }
 
 template <int FPL>
-__global__ __launch_bounds__(kWave) void retile_kernel(DevState s, const CompactCtl* ctl) {
-  if (ctl && !ctl->go) return;
+__global__ __launch_bounds__(kWave) void retile_kernel(DevState s) {
   (void)retile_tile<FPL>(s, blockIdx.x, threadIdx.x);
+}
+
+// In place, in one launch (captured graphs).  Destination slot j < ctl->live takes source slot
+// src_of[j], and src_of[j] >= j because live slots keep their order, so every source of
+// destination tile tb lies in tile tb or later.  A workgroup owns a set of rows (message edges
+// or channel variables) in every tile and walks the destination tiles upward: it gathers a
+// tile's values into registers, meets a barrier, then stores them.  Its stores before tile tb
+// went to tiles below tb, where no source of tb's slots lies, so each value is read before any
+// store reaches it.  Workgroups own disjoint rows.  The last workgroup moves the phases and
+// iteration counts the same way, kBlock slots at a time, then sets the tile flags.
+constexpr int kInPlaceElems = 4;   // values a thread carries per destination tile
+
+template <class F>
+__global__ __launch_bounds__(kBlock) void compact_in_place_kernel(DevCode c, DevState s, const int32_t* src_of,
+                                                                  const CompactCtl* ctl, int32_t msg_blocks,
+                                                                  int32_t L_blocks) {
+  constexpr int TW = F::TW, RW = F::RW, PW = F::PW, FPL = F::FPL;
+  constexpr int kEdges = kInPlaceElems * kBlock / RW, kVars = kInPlaceElems * kBlock / TW;
+  if (!ctl->go) return;
+  const int64_t live = ctl->live;
+  const int dtiles = (int)((live + TW - 1) / TW);
+  const int b = blockIdx.x;
+  if (b < msg_blocks) {
+    const int e0 = b * kEdges;
+    for (int tb = 0; tb < dtiles; ++tb) {
+      uint32_t word[kInPlaceElems];
+#pragma unroll
+      for (int k = 0; k < kInPlaceElems; ++k) {
+        const int i = threadIdx.x + k * kBlock, e = e0 + i / RW, w = i % RW;
+        word[k] = 0;
+        if (e >= c.E) continue;
+#pragma unroll
+        for (int h = 0; h < PW; ++h) {
+          const int64_t j = (int64_t)tb * TW + w * PW + h;
+          if (j >= live) continue;
+          const int from = src_of[j];
+          const uint32_t sw = s.msg[((int64_t)(from / TW) * c.E + e) * RW + (from % TW) / PW];
+          word[k] |= (PW == 1 ? sw : (sw >> (16 * (from % PW))) & 0xffffu) << (16 * h);
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < kInPlaceElems; ++k) {
+        const int i = threadIdx.x + k * kBlock, e = e0 + i / RW, w = i % RW;
+        if (e < c.E) s.msg[((int64_t)tb * c.E + e) * RW + w] = word[k];
+      }
+    }
+  } else if (b < msg_blocks + L_blocks) {
+    const int v0 = (b - msg_blocks) * kVars;
+    for (int tb = 0; tb < dtiles; ++tb) {
+      float x[kInPlaceElems];
+#pragma unroll
+      for (int k = 0; k < kInPlaceElems; ++k) {
+        const int i = threadIdx.x + k * kBlock, v = v0 + i / TW, p = i % TW;
+        const int64_t j = (int64_t)tb * TW + p;
+        x[k] = 0.0f;
+        if (v < c.N && j < live) {
+          const int from = src_of[j];
+          x[k] = s.L[((int64_t)(from / TW) * c.N + v) * TW + from % TW];
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < kInPlaceElems; ++k) {
+        const int i = threadIdx.x + k * kBlock, v = v0 + i / TW, p = i % TW;
+        if (v < c.N) s.L[((int64_t)tb * c.N + v) * TW + p] = x[k];
+      }
+    }
+  } else {
+    const int64_t slots = (int64_t)s.tiles * TW;
+    for (int64_t j0 = 0; j0 < slots; j0 += kBlock) {
+      const int64_t j = j0 + threadIdx.x;
+      uint8_t ph = PH_FREE;
+      int32_t it = 0;
+      if (j < live) {
+        const int from = src_of[j];
+        ph = s.phase[from];
+        it = s.iter[from];
+      }
+      __syncthreads();
+      if (j < slots) {
+        s.phase[j] = ph;
+        s.iter[j] = it;
+      }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x / kWave; t < s.tiles; t += kBlock / kWave) (void)retile_tile<FPL>(s, t, threadIdx.x % kWave);
+  }
 }
 
 __global__ void fill_i32_kernel(int32_t* p, int64_t n, int32_t v) {
@@ -856,20 +942,7 @@ static hipError_t launch_results(const DevCode& c, const DevState& s, const DevO
   return hipGetLastError();
 }
 
-// Moves the frames of slots src_of[j] of `from` into slots j of `to` (ctl: see compact_source).
-template <class F>
-static void launch_move(const DevCode& c, const DevState& from, const DevState& to, const int32_t* src_of,
-                        const CompactCtl* ctl, hipStream_t st) {
-  constexpr int TW = F::TW;
-  const int32_t dst_slots = to.tiles * TW;
-  hipLaunchKernelGGL((compact_msg_kernel<F>), dim3((c.E + kCompactEdges - 1) / kCompactEdges, to.tiles), dim3(kBlock),
-                     0, st, c, from.msg, to.msg, src_of, ctl);
-  hipLaunchKernelGGL((compact_L_kernel<TW>), dim3((c.N + kCompactEdges - 1) / kCompactEdges, to.tiles), dim3(kBlock),
-                     0, st, c, from.L, to.L, src_of, ctl);
-  hipLaunchKernelGGL(compact_slots_kernel, dim3((dst_slots + 255) / 256), dim3(256), 0, st, from.phase, from.iter,
-                     to.phase, to.iter, src_of, dst_slots, ctl);
-}
-
+// Moves the frames of slots src_of[j] of `from` into slots j of `to` (src_of[j] = -1: empty).
 template <class F>
 static hipError_t launch_compact(const DevCode& c, const DevState& from, const DevState& to, const int32_t* map_in,
                                  int32_t* map_out, int32_t* src_of, hipStream_t st) {
@@ -877,18 +950,24 @@ static hipError_t launch_compact(const DevCode& c, const DevState& from, const D
   const int32_t slots = from.tiles * TW, dst_slots = to.tiles * TW;
   hipLaunchKernelGGL(compact_plan_kernel, dim3(1), dim3(kPlanThreads), 0, st, from.phase, map_in, slots, src_of,
                      map_out, dst_slots, (CompactCtl*)nullptr, TW, 0);
-  launch_move<F>(c, from, to, src_of, nullptr, st);
-  hipLaunchKernelGGL((retile_kernel<FPL>), dim3(to.tiles), dim3(kWave), 0, st, to, (const CompactCtl*)nullptr);
+  hipLaunchKernelGGL((compact_msg_kernel<F>), dim3((c.E + kCompactEdges - 1) / kCompactEdges, to.tiles), dim3(kBlock),
+                     0, st, c, from.msg, to.msg, (const int32_t*)src_of);
+  hipLaunchKernelGGL((compact_L_kernel<TW>), dim3((c.N + kCompactEdges - 1) / kCompactEdges, to.tiles), dim3(kBlock),
+                     0, st, c, from.L, to.L, (const int32_t*)src_of);
+  hipLaunchKernelGGL(compact_slots_kernel, dim3((dst_slots + 255) / 256), dim3(256), 0, st, from.phase, from.iter,
+                     to.phase, to.iter, (const int32_t*)src_of, dst_slots);
+  hipLaunchKernelGGL((retile_kernel<FPL>), dim3(to.tiles), dim3(kWave), 0, st, to);
   return hipGetLastError();
 }
 
 template <class F>
-static hipError_t launch_compact_in_place(const DevCode& c, const DevState& s, const DevState& scratch,
-                                          const int32_t* src_of, const CompactCtl* ctl, hipStream_t st) {
-  constexpr int FPL = F::FPL;
-  launch_move<F>(c, s, scratch, src_of, ctl, st);    // live frames, in slot order, into scratch
-  launch_move<F>(c, scratch, s, nullptr, ctl, st);   // back into the region's first slots
-  hipLaunchKernelGGL((retile_kernel<FPL>), dim3(s.tiles), dim3(kWave), 0, st, s, ctl);
+static hipError_t launch_compact_in_place(const DevCode& c, const DevState& s, const int32_t* src_of,
+                                          const CompactCtl* ctl, hipStream_t st) {
+  constexpr int kEdges = kInPlaceElems * kBlock / F::RW, kVars = kInPlaceElems * kBlock / F::TW;
+  static_assert(kEdges >= 1 && kVars >= 1, "a workgroup owns at least one row");
+  const int32_t msg_blocks = (c.E + kEdges - 1) / kEdges, L_blocks = (c.N + kVars - 1) / kVars;
+  hipLaunchKernelGGL((compact_in_place_kernel<F>), dim3(msg_blocks + L_blocks + 1), dim3(kBlock), 0, st, c, s, src_of,
+                     ctl, msg_blocks, L_blocks);
   return hipGetLastError();
 }
 
@@ -1024,10 +1103,10 @@ hipError_t kernels_compact_plan(const Layout& lay, const DevState& s, const int3
   return hipGetLastError();
 }
 
-hipError_t kernels_compact_in_place(const Layout& lay, const DevCode& c, const DevState& s, const DevState& scratch,
-                                    const int32_t* src_of, const CompactCtl* ctl, hipStream_t st) {
-  if (s.tiles <= 0 || scratch.tiles < s.tiles) return hipErrorInvalidValue;
-  FECL_DISPATCH(lay, launch_compact_in_place<F>(c, s, scratch, src_of, ctl, st));
+hipError_t kernels_compact_in_place(const Layout& lay, const DevCode& c, const DevState& s, const int32_t* src_of,
+                                    const CompactCtl* ctl, hipStream_t st) {
+  if (s.tiles <= 0 || !ctl || !src_of) return hipErrorInvalidValue;
+  FECL_DISPATCH(lay, launch_compact_in_place<F>(c, s, src_of, ctl, st));
 }
 
 }  // namespace fecl

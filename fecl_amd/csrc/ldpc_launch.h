@@ -95,11 +95,11 @@ hipError_t kernels_compact(const Layout& lay, const DevCode& c, const DevState& 
 // ctl (go, live) from the phases, the steps left (`remaining`) and the tiles holding live
 // frames, and writes map_out (a copy of map_in when it declines) and src_of (s.tiles * TW
 // entries each).  Between the plan and the move the caller hands the region's finished
-// frames' results over with the same ctl.  The move gathers the live frames into `scratch`
-// (as many tiles as s) and copies them back into s's first slots, then resets s's tile flags.
+// frames' results over with the same ctl.  The move (one launch, nothing unless ctl->go)
+// takes the live frames into s's first slots in one pass and resets s's tile flags.
 hipError_t kernels_compact_plan(const Layout& lay, const DevState& s, const int32_t* map_in, int32_t* map_out,
                                 int32_t* src_of, CompactCtl* ctl, int32_t remaining, hipStream_t st);
-hipError_t kernels_compact_in_place(const Layout& lay, const DevCode& c, const DevState& s, const DevState& scratch,
-                                    const int32_t* src_of, const CompactCtl* ctl, hipStream_t st);
+hipError_t kernels_compact_in_place(const Layout& lay, const DevCode& c, const DevState& s, const int32_t* src_of,
+                                    const CompactCtl* ctl, hipStream_t st);
 
 }  // namespace fecl

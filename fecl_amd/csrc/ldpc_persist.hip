@@ -284,14 +284,14 @@ __device__ static inline void run_move(const PCode* P, int a, int t, int chunk) 
   const int l_chunks = (P->c.N + kMoveSpan - 1) / kMoveSpan;
   if (chunk < msg_chunks) {
     for (int e = chunk * kMoveSpan; e < (chunk + 1) * kMoveSpan && e < P->c.E; e += kCompactEdges)
-      compact_msg_block<F>(P->c, from.msg, to.msg, P->src_of, nullptr, t, e);
+      compact_msg_block<F>(P->c, from.msg, to.msg, P->src_of, t, e);
   } else if (chunk < msg_chunks + l_chunks) {
     const int vb = (chunk - msg_chunks) * kMoveSpan;
     for (int v = vb; v < vb + kMoveSpan && v < P->c.N; v += kCompactEdges)
-      compact_L_block<TW>(P->c, from.L, to.L, P->src_of, nullptr, t, v);
+      compact_L_block<TW>(P->c, from.L, to.L, P->src_of, t, v);
   } else {
     for (int j = t * TW + threadIdx.x; j < (t + 1) * TW; j += kBlock)
-      compact_slot(from.phase, from.iter, to.phase, to.iter, P->src_of, nullptr, j);
+      compact_slot(from.phase, from.iter, to.phase, to.iter, P->src_of, j);
   }
 }
 

@@ -1,6 +1,7 @@
 // Autotune cache (see ldpc_tune_cache.h).
 #include "ldpc_tune_cache.h"
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -91,7 +92,8 @@ uint64_t code_hash(const Code& c) {
 }
 
 int64_t frames_bucket(int64_t frames) {
-  int64_t b = 1;
+  if (frames <= kFineBucketFrames) return std::max<int64_t>(1, (frames + 63) / 64) * 64;
+  int64_t b = kFineBucketFrames;
   while (b < frames && b < (int64_t(1) << 40)) b <<= 1;
   return b;
 }

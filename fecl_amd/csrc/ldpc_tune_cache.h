@@ -1,6 +1,6 @@
 // Autotune cache: the lane layout fecl_ldpc_autotune picked, kept per (GPU architecture and
-// compute-unit count, code, algorithm, message format, batch size rounded up to a power of two)
-// so that later autotunes of the same key pin it without probing.  Pure C++ (no device code),
+// compute-unit count, code, algorithm, message format, batch-size bucket) so that later
+// autotunes of the same key pin it without probing.  Pure C++ (no device code),
 // so the CPU tests and the sanitizer builds exercise it directly.
 //
 // The optional file holds one header line and one line per pick:
@@ -24,7 +24,10 @@ struct Pick {
 
 // FNV-1a over the code's sizes and CSR arrays (the decoder runs on this exact H).
 uint64_t code_hash(const Code& c);
-// The batch-size bucket of a frame count: the next power of two, at least 1.
+// The batch-size bucket of a frame count: up to kFineBucketFrames, the next multiple of 64 (the
+// smallest tile), because the layouts' padding to whole tiles decides the pick there (601
+// frames fill five 128-frame tiles but three 256-frame ones); above it, the next power of two.
+constexpr int64_t kFineBucketFrames = 8192;
 int64_t frames_bucket(int64_t frames);
 std::string make_key(const std::string& arch, int cus, const Code& c, int algorithm, int msg_format, int64_t frames);
 

@@ -648,7 +648,7 @@ def autotune_cache(mode=1, path=None):
     """Where Ldpc.autotune keeps its picks (no counterpart in the reference): 0 = nowhere, 1 = this
     process (the default), 2 = this process and the text file `path` (read now, appended with
     every new pick).  Picks are keyed by GPU architecture, code, algorithm, message format and
-    batch size rounded up to a power of two."""
+    batch size (rounded up to a multiple of 64 up to 8192 frames, to a power of two above)."""
     check(lib().fecl_autotune_cache_config(int(mode), None if path is None else os.fsencode(path)))
 
 

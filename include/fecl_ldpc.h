@@ -230,11 +230,10 @@ int fecl_ldpc_set_pass_frames(fecl_ldpc* h, int64_t frames);
  * few live frames.  With compaction on (default), an eager decode moves the live frames into
  * three quarters as many tiles or fewer whenever they fit and enough iterations remain, and
  * later iterations run on those tiles only; the batch's workspace grows by up to 75 %.
- * Captured decodes (fecl_ldpc_graph_create) compact in place instead: after every sixth
+ * Captured decodes (fecl_ldpc_graph_create) compact in place instead: after every fourth
  * step, while at least four remain, a device kernel decides whether the live frames fit in
- * fewer tiles and moves them into the region's first tiles through a scratch area; such a
- * graph holds one more area of the pass's tiles (message rows, channel values, per-slot
- * state) plus three slot-sized maps.  The fused syndrome never compacts. */
+ * fewer tiles and moves them, in one pass, into the region's first tiles; such a graph holds
+ * three more slot-sized maps.  The fused syndrome never compacts. */
 int fecl_ldpc_set_compaction(fecl_ldpc* h, int32_t enable);
 /* Cache hints for the message rows (no counterpart in the reference; results are identical
  * for any value): bit 0 = non-temporal loads/stores in the check kernel, bit 1 = in the
@@ -245,8 +244,9 @@ int fecl_ldpc_autotune(fecl_ldpc* h, int64_t frames, int32_t probe_iterations, i
                        double* ms /*[6]*/);
 /* Autotune cache (no counterpart in the reference).  fecl_ldpc_autotune keeps each pick under
  * (GPU architecture and compute-unit count, code: sizes and a hash of H, algorithm, message
- * format, frames rounded up to a power of two); a later autotune with the same key pins the
- * kept pick without probing (ms[] then holds the kept probe times).  mode 0 = off, 1 = this
+ * format, frames rounded up to a multiple of 64 up to 8192 frames, to a power of two above);
+ * a later autotune with the same key pins the kept pick without probing (ms[] then holds the
+ * kept probe times).  mode 0 = off, 1 = this
  * process (the default), 2 = this process and the text file `path`, read now and appended with
  * every new pick.  Without a config call, the environment variable FECL_AUTOTUNE_CACHE names
  * such a file at the first autotune.  stats: lookups that hit and missed, picks held, file

@@ -211,7 +211,9 @@ static void test_tune_cache() {
   tune::stats(nullptr, nullptr, nullptr, nullptr, &wf);
   EXPECT(tune::lookup(k, &q) && q.words == 2 && wf == 1, "kept in process, failure counted (%lld)", (long long)wf);
   EXPECT(tune::configure(1, nullptr, &err), "back to process-only");
-  EXPECT(tune::frames_bucket(1) == 1 && tune::frames_bucket(1000) == 1024 && tune::frames_bucket(1024) == 1024,
+  EXPECT(tune::frames_bucket(1) == 64 && tune::frames_bucket(200) == 256 && tune::frames_bucket(601) == 640 &&
+             tune::frames_bucket(1024) == 1024 && tune::frames_bucket(8192) == 8192 &&
+             tune::frames_bucket(8193) == 16384 && tune::frames_bucket(20000) == 32768,
          "frame buckets");
 }
 

@@ -132,26 +132,26 @@ def test_device_path_dvbs2_compaction_on_and_off():
 
 
 def graph_compactions(iters, tw, iterations):
-    """In-place compactions a captured decode makes: after every sixth step (while 4 steps
+    """In-place compactions a captured decode makes: after every fourth step (while 4 steps
     remain) the device counts the frames still live (c > s+1 updates) and the tiles holding
-    them, and moves them into the first slots when that frees a tile and the freed tile-steps
-    outweigh two moves of 1.5 frame-iterations per live frame."""
+    them, and moves them into the first slots (in place, one move) when that frees a tile and
+    the freed tile-steps outweigh 1.5 frame-iterations per live frame."""
     slots = list(range(len(iters)))
     n = 0
-    for step in range(5, iterations, 6):
+    for step in range(3, iterations, 4):
         if step + 4 >= iterations:
             break
         live = [f for f in slots if f >= 0 and iters[f] > step + 1]
         tiles_live = len({j // tw for j, f in enumerate(slots) if f >= 0 and iters[f] > step + 1})
         freed = tiles_live - (len(live) + tw - 1) // tw
-        if freed > 0 and 2 * freed * (iterations - 1 - step) * tw > 6 * len(live):
+        if freed > 0 and 2 * freed * (iterations - 1 - step) * tw > 3 * len(live):
             slots, n = live + [-1] * (len(slots) - len(live)), n + 1
     return n
 
 
 def test_graph_compacts_on_the_device():
     """One hipGraph over two batches that converge at staggered iterations (FP32 and FP16
-    messages, 64 and 128 frames per tile): the device decides at every sixth step and compacts
+    messages, 64 and 128 frames per tile): the device decides at every fourth step and compacts
     in place.  Replayed with new inputs copied in eagerly between launches, every replay's bits
     and iteration counts equal the oracle's and the eager decode's."""
     import torch
@@ -181,6 +181,41 @@ def test_graph_compacts_on_the_device():
             torch.cuda.synchronize()
             assert np.array_equal(bits.cpu().numpy().reshape(frames, oc.K), got), (rep, i)
             assert np.array_equal(it.cpu().numpy(), refs[i]["iters"]), (rep, i)
+
+
+@pytest.mark.parametrize("fmt,ofmt,words,frames", [(F.MsgFormat.F16, O.MSG_F16, 4, 4096),
+                                                   (F.MsgFormat.BF16, O.MSG_BF16, 2, 2048),
+                                                   (F.MsgFormat.F32, O.MSG_F32, 2, 1536)])
+def test_graph_in_place_compaction_wide_tiles(fmt, ofmt, words, frames):
+    """In-place compaction with 512-, 256- and 128-frame tiles (2 and 4 words per lane), over
+    several 1024-frame passes: a workgroup of the move then owns fewer edges or variables, and a
+    destination tile's sources more often lie in that same tile.  Two replays: bits and
+    iteration counts equal the oracle's and the eager decode's."""
+    import torch
+    oc, H = gallager()
+    dec = gpu_decoder(H, F.Approximate, ITERS, fmt=fmt, words=words)
+    eager = gpu_decoder(H, F.Approximate, ITERS, fmt=fmt, words=words)
+    x = torch.zeros(frames * oc.N, dtype=torch.float32, device="cuda")
+    g = F.BatchGraph([dec], [x])
+    tw = dec.layout()[1]
+    pass_frames = 1024   # the default pass; each pass is a region of its own
+    for rep in range(2):
+        llr = staggered_llr(oc, frames, 120 + rep)
+        ref = oracle_run(oc, llr, F.Approximate, ITERS, fmt=ofmt, want=("hard", "iters"), threads=16)
+        n = sum(graph_compactions(ref["iters"][p:p + pass_frames], tw, ITERS) for p in range(0, frames, pass_frames))
+        print(f"fmt={fmt} words={words} tile={tw} replay {rep}: in-place compactions {n}", flush=True)
+        assert n >= 1
+        x.copy_(torch.from_numpy(llr.astype(np.float32).ravel()))
+        torch.cuda.synchronize()
+        g.launch()
+        torch.cuda.synchronize()
+        got = g.msg[0].cpu().numpy().reshape(frames, oc.K)
+        assert np.array_equal(got, ref["hard"]), rep
+        assert np.array_equal(g.iterations[0].cpu().numpy(), ref["iters"]), rep
+        bits, it = eager.decode(x, iterations=True)
+        torch.cuda.synchronize()
+        assert np.array_equal(bits.cpu().numpy().reshape(frames, oc.K), got), rep
+        assert np.array_equal(it.cpu().numpy(), ref["iters"]), rep
 
 
 @pytest.mark.parametrize("alg", [F.Linear, F.Exact])
