@@ -218,16 +218,18 @@ def test_graph_in_place_compaction_wide_tiles(fmt, ofmt, words, frames):
         assert np.array_equal(it.cpu().numpy(), ref["iters"]), rep
 
 
-@pytest.mark.parametrize("alg", [F.Linear, F.Exact])
-def test_graph_compaction_linear_exact_equal_eager(alg):
+@pytest.mark.parametrize("alg,fmt,words", [(F.Linear, F.MsgFormat.F32, 1), (F.Exact, F.MsgFormat.F32, 1),
+                                           (F.Linear, F.MsgFormat.F16, 2), (F.Exact, F.MsgFormat.BF16, 2)])
+def test_graph_compaction_linear_exact_equal_eager(alg, fmt, words):
     """Linear and Exact with scaling factor 0.75 in one hipGraph (in-place compaction on the
-    device) against eager decodes (compaction between regions on the host's counts): bits and
-    iteration counts equal on two replays."""
+    device) against eager decodes (compaction between regions on the host's counts), FP32 with
+    64-frame tiles and FP16/BF16 with 256-frame tiles: bits and iteration counts equal on two
+    replays."""
     import torch
     oc, H = gallager()
     frames = 1024
-    dec = gpu_decoder(H, alg, ITERS, sf=0.75, words=1)
-    eager = gpu_decoder(H, alg, ITERS, sf=0.75, words=1)
+    dec = gpu_decoder(H, alg, ITERS, sf=0.75, fmt=fmt, words=words)
+    eager = gpu_decoder(H, alg, ITERS, sf=0.75, fmt=fmt, words=words)
     x = torch.zeros(frames * oc.N, dtype=torch.float32, device="cuda")
     g = F.BatchGraph([dec], [x])
     for rep in range(2):
@@ -238,8 +240,8 @@ def test_graph_compaction_linear_exact_equal_eager(alg):
         torch.cuda.synchronize()
         bits, it = eager.decode(x, iterations=True)
         torch.cuda.synchronize()
-        n = graph_compactions(it.cpu().numpy(), 64, ITERS)
-        print(f"alg={alg} replay {rep}: in-place compactions {n}", flush=True)
+        n = graph_compactions(it.cpu().numpy(), dec.layout()[1], ITERS)
+        print(f"alg={alg} fmt={fmt} words={words} replay {rep}: in-place compactions {n}", flush=True)
         assert n >= 1
         assert np.array_equal(g.msg[0].cpu().numpy(), bits.cpu().numpy()), rep
         assert np.array_equal(g.iterations[0].cpu().numpy(), it.cpu().numpy()), rep
