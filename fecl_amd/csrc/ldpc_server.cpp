@@ -178,9 +178,13 @@ struct Admit { Request* req; int64_t f0, n; };
 // Host side of the admissions of round e: queued frames into free slots.  A tile with a fresh
 // frame reads the channel values on its next check update besides its messages, and an
 // admission writes each frame's values across all of its tile's rows, so a round's admissions
-// go to few tiles: the (at most kFillTiles) tiles that hold frames and have the most free
-// slots, lowest slots first, then tiles holding no frame in index order.  Free slots of other
-// tiles wait for a later round.  Returns whether any pool holds frames (and so steps).
+// go to few tiles (at most kFillTiles that hold frames, lowest slots first, then tiles holding
+// no frame in index order).  First comes the fullest tile holding frames that takes every
+// frame the round admits, then the tiles holding frames with the most free slots.  Below
+// saturation a round admits a few frames and they fill up nearly full tiles, so sparse tiles
+// drain and stop costing steps; a saturated pool's round admits more than any tile has free,
+// and its frames go to the tiles with the most room.  Free slots of other tiles wait for a
+// later round.  Returns whether any pool holds frames (and so steps).
 constexpr int kFillTiles = 2;
 
 bool choose_admissions(fecl_ldpc_server* sv, int64_t e, int steps, std::vector<std::vector<Admit>>* admits) {
@@ -200,6 +204,17 @@ bool choose_admissions(fecl_ldpc_server* sv, int64_t e, int steps, std::vector<s
         if (P.tile_used[t] > 0 && !P.tile_free[t].empty()) order.push_back(t);
       std::stable_sort(order.begin(), order.end(),
                        [&](int x, int y) { return P.tile_free[x].size() > P.tile_free[y].size(); });
+      // the fullest occupied tile that takes every frame of the round goes first
+      int64_t want = 0;
+      for (const Request* q : P.queue) {
+        want += q->frames - q->admitted;
+        if (want >= cap) break;
+      }
+      want = std::min(want, cap);
+      int best = -1;
+      for (int i = 0; i < (int)order.size(); ++i)
+        if ((int64_t)P.tile_free[order[i]].size() >= want) best = i;
+      if (best > 0) std::rotate(order.begin(), order.begin() + best, order.begin() + best + 1);
       if ((int)order.size() > kFillTiles) order.resize(kFillTiles);
       for (int t = 0; t < T; ++t)
         if (P.tile_used[t] == 0) order.push_back(t);
