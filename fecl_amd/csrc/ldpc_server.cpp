@@ -25,6 +25,7 @@
 #include <unordered_map>
 #include <vector>
 
+#include "ldpc_admission.h"
 #include "ldpc_engine.h"
 
 namespace fecl {
@@ -175,21 +176,15 @@ struct Events {
 
 struct Admit { Request* req; int64_t f0, n; };
 
-// Host side of the admissions of round e: queued frames into free slots.  A tile with a fresh
-// frame reads the channel values on its next check update besides its messages, and an
-// admission writes each frame's values across all of its tile's rows, so a round's admissions
-// go to few tiles (at most kFillTiles that hold frames, lowest slots first, then tiles holding
-// no frame in index order).  First comes the fullest tile holding frames that takes every
-// frame the round admits, then the tiles holding frames with the most free slots.  Below
-// saturation a round admits a few frames and they fill up nearly full tiles, so sparse tiles
-// drain and stop costing steps; a saturated pool's round admits more than any tile has free,
-// and its frames go to the tiles with the most room.  Free slots of other tiles wait for a
-// later round.  Returns whether any pool holds frames (and so steps).
+// Host side of the admissions of round e: queued frames into free slots of the tiles that
+// admission_order (ldpc_admission.h) lists, lowest slots first; free slots of other tiles wait
+// for a later round.  Returns whether any pool holds frames (and so steps).
 constexpr int kFillTiles = 2;
 
 bool choose_admissions(fecl_ldpc_server* sv, int64_t e, int steps, std::vector<std::vector<Admit>>* admits) {
   bool any = false;
   std::vector<int> order;
+  std::vector<int64_t> free_count, used_count;
   std::lock_guard<std::mutex> lk(sv->mu);
   for (size_t pi = 0; pi < sv->pools.size(); ++pi) {
     Pool& P = *sv->pools[pi];
@@ -199,25 +194,18 @@ bool choose_admissions(fecl_ldpc_server* sv, int64_t e, int steps, std::vector<s
     int32_t* slots = P.h_admit[e & 1].as<int32_t>();
     if (!P.queue.empty()) {
       const int T = (int)P.tile_free.size();
-      order.clear();
-      for (int t = 0; t < T; ++t)
-        if (P.tile_used[t] > 0 && !P.tile_free[t].empty()) order.push_back(t);
-      std::stable_sort(order.begin(), order.end(),
-                       [&](int x, int y) { return P.tile_free[x].size() > P.tile_free[y].size(); });
-      // the fullest occupied tile that takes every frame of the round goes first
-      int64_t want = 0;
+      int64_t want = 0;   // frames the round admits, if the tiles have room
       for (const Request* q : P.queue) {
         want += q->frames - q->admitted;
         if (want >= cap) break;
       }
-      want = std::min(want, cap);
-      int best = -1;
-      for (int i = 0; i < (int)order.size(); ++i)
-        if ((int64_t)P.tile_free[order[i]].size() >= want) best = i;
-      if (best > 0) std::rotate(order.begin(), order.begin() + best, order.begin() + best + 1);
-      if ((int)order.size() > kFillTiles) order.resize(kFillTiles);
-      for (int t = 0; t < T; ++t)
-        if (P.tile_used[t] == 0) order.push_back(t);
+      free_count.resize(T);
+      used_count.resize(T);
+      for (int t = 0; t < T; ++t) {
+        free_count[t] = (int64_t)P.tile_free[t].size();
+        used_count[t] = P.tile_used[t];
+      }
+      admission_order(free_count, used_count, std::min(want, cap), kFillTiles, &order);
       for (int t : order) {
         std::vector<int32_t>& fl = P.tile_free[t];
         std::sort(fl.begin(), fl.end(), std::greater<int32_t>());   // lowest at the back

@@ -5,6 +5,7 @@
 //   archive reader     round trips, every truncation, random byte damage
 //   autotune cache     lookups and stores from many threads, with and without the file
 //   host conversions   FP64 -> FP32 -> FP64 helpers run by concurrent callers sharing threads
+//   admission order    which tiles of a serving pool take a round's frames
 #include <algorithm>
 #include <atomic>
 #include <cmath>
@@ -16,6 +17,7 @@
 #include <unistd.h>
 #include <vector>
 
+#include "ldpc_admission.h"
 #include "ldpc_archive.h"
 #include "ldpc_code.h"
 #include "ldpc_host_convert.h"
@@ -259,11 +261,75 @@ static void test_host_convert() {
   EXPECT(wrong == 0, "%d conversion mismatches", wrong.load());
 }
 
+static std::vector<int> order_of(const std::vector<int64_t>& free_slots, const std::vector<int64_t>& used,
+                                 int64_t want, int max_occupied) {
+  std::vector<int> o;
+  admission_order(free_slots, used, want, max_occupied, &o);
+  return o;
+}
+
+static void test_admission_order() {
+  using V = std::vector<int>;
+  // a nearly full tile (4 free), a sparse one (54 free), two empty ones
+  const std::vector<int64_t> f1 = {4, 54, 64, 64}, u1 = {60, 10, 0, 0};
+  EXPECT(order_of(f1, u1, 3, 2) == V({0, 1, 2, 3}), "a small round goes to the fullest tile that takes it");
+  EXPECT(order_of(f1, u1, 4, 2) == V({0, 1, 2, 3}), "exactly the free slots of the fullest tile");
+  EXPECT(order_of(f1, u1, 10, 2) == V({1, 0, 2, 3}), "only the sparse tile takes 10 frames");
+  EXPECT(order_of(f1, u1, 128, 2) == V({1, 0, 2, 3}), "no tile takes the round: most room first");
+  // at most max_occupied tiles that hold frames, the fullest fitting one first
+  const std::vector<int64_t> f2 = {10, 20, 30, 5, 64}, u2 = {54, 44, 34, 59, 0};
+  EXPECT(order_of(f2, u2, 25, 2) == V({2, 1, 4}), "25 frames: only tile 2 has room");
+  EXPECT(order_of(f2, u2, 8, 2) == V({0, 2, 4}), "8 frames: tile 0 is the fullest with room");
+  EXPECT(order_of(f2, u2, 8, 4) == V({0, 2, 1, 3, 4}), "8 frames, four occupied tiles allowed");
+  // full tiles take nothing; a pool with no frame fills empty tiles in index order
+  EXPECT(order_of({0, 5}, {64, 59}, 1, 2) == V({1}), "full tile skipped");
+  EXPECT(order_of({64, 64}, {0, 0}, 200, 2) == V({0, 1}), "empty pool");
+  // equal free counts keep index order among themselves
+  EXPECT(order_of({8, 8, 3}, {56, 56, 61}, 5, 2) == V({1, 0}), "ties: the later of the tiles with room");
+  // random pools: at most max_occupied occupied tiles, then every empty tile in index order; the
+  // first occupied tile has room for the round if any tile has
+  std::mt19937_64 rng(7);
+  for (int iter = 0; iter < 2000; ++iter) {
+    const int T = 1 + (int)(rng() % 12), tw = 64;
+    std::vector<int64_t> fr(T), us(T);
+    for (int t = 0; t < T; ++t) {
+      us[t] = (int64_t)(rng() % 3 == 0 ? 0 : rng() % (tw + 1));
+      fr[t] = tw - us[t];
+    }
+    const int64_t want = 1 + (int64_t)(rng() % 100);
+    const int cap = 1 + (int)(rng() % 3);
+    const std::vector<int> o = order_of(fr, us, want, cap);
+    int occupied = 0, empties = 0;
+    bool ok = true;
+    for (size_t i = 0; i < o.size(); ++i) {
+      if (us[o[i]] > 0) {
+        ok &= fr[o[i]] > 0 && empties == 0;
+        ++occupied;
+      } else {
+        ok &= (i == 0 || us[o[i - 1]] > 0 || o[i - 1] < o[i]);
+        ++empties;
+      }
+    }
+    int64_t fits = -1, most = -1;
+    for (int t = 0; t < T; ++t) {
+      if (us[t] == 0 || fr[t] == 0) continue;
+      if (fr[t] >= want && (fits < 0 || fr[t] < fits)) fits = fr[t];
+      most = std::max(most, fr[t]);
+    }
+    const int total_empty = (int)std::count(us.begin(), us.end(), (int64_t)0);
+    ok &= occupied <= cap && empties == total_empty;
+    if (occupied > 0) ok &= fr[o[0]] == (fits >= 0 ? fits : most);
+    EXPECT(ok, "random pool %d (T=%d want=%lld cap=%d)", iter, T, (long long)want, cap);
+    if (!ok) break;
+  }
+}
+
 int main() {
   test_codes();
   test_archive();
   test_tune_cache();
   test_host_convert();
+  test_admission_order();
   std::printf("%d failures\n", g_failures);
   return g_failures ? 1 : 0;
 }
