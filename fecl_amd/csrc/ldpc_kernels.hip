@@ -319,14 +319,18 @@ __global__ __launch_bounds__(kBlock) void state_in_kernel(DevCode c, DevState s,
 // Results of a region of tiles go to the frames its slots hold: slot j holds frame map[j]
 // (-1: none), or frame j when map is null (the batch's first region).
 
-// Hard decisions [tile][K][FPL] -> msg [F][K] bytes (0/1); blockIdx.y is the slot.
+// Launches whose grid.y counts frames or slots split into launches of at most this many rows
+// (the device's reported grid limit in y is 65536).
+constexpr int64_t kMaxGridY = 65535;
+
+// Hard decisions [tile][K][FPL] -> msg [F][K] bytes (0/1); slot j0 + blockIdx.y.
 template <int FPL>
 __global__ __launch_bounds__(kBlock) void unpack_hard_kernel(const uint64_t* hard, int K, int64_t frames,
                                                              uint8_t* msg, int64_t frame0, const int32_t* map,
-                                                             const CompactCtl* ctl) {
+                                                             const CompactCtl* ctl, int64_t j0) {
   constexpr int TW = kWave * FPL;
   if (ctl && !ctl->go) return;
-  const int64_t j = blockIdx.y;
+  const int64_t j = j0 + blockIdx.y;
   const int64_t f = map ? (int64_t)map[j] : j;
   const int v = blockIdx.x * kBlock + threadIdx.x;
   if (v >= K || f < 0 || f >= frames) return;
@@ -917,9 +921,11 @@ static hipError_t launch_results(const DevCode& c, const DevState& s, const DevO
   constexpr int FPL = F::FPL, TW = F::TW;
   const int64_t n = map ? (int64_t)s.tiles * TW : frames;   // slots of the region, or frames
   if (n <= 0) return hipSuccess;
-  if (r.msg_bits)
-    hipLaunchKernelGGL((unpack_hard_kernel<FPL>), dim3((c.K + kBlock - 1) / kBlock, (unsigned)n), dim3(kBlock), 0, st,
-                       out.hard, c.K, frames, r.msg_bits, frame0, map, ctl);
+  if (r.msg_bits) {
+    for (int64_t j0 = 0; j0 < n; j0 += kMaxGridY)
+      hipLaunchKernelGGL((unpack_hard_kernel<FPL>), dim3((c.K + kBlock - 1) / kBlock, (unsigned)std::min(kMaxGridY, n - j0)),
+                         dim3(kBlock), 0, st, out.hard, c.K, frames, r.msg_bits, frame0, map, ctl, j0);
+  }
   if (r.iterations)
     hipLaunchKernelGGL(iters_out_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, s.iter, n, frames,
                        r.iterations, frame0, map, ctl);
@@ -1071,13 +1077,19 @@ hipError_t kernels_admit(const Layout& lay, const DevCode& c, const DevState& s,
 hipError_t kernels_extract(const Layout& lay, const DevCode& c, const DevState& s, const DevOut& out,
                            const int32_t* slots, int n, uint8_t* bits, int32_t* iters, hipStream_t st) {
   if (n <= 0) return hipSuccess;
-  dim3 grid((unsigned)std::min<int64_t>((c.K + kBlock - 1) / kBlock, 64), (unsigned)n);
-  switch (lay.fpl()) {
-    case 1: hipLaunchKernelGGL((extract_kernel<1>), grid, dim3(kBlock), 0, st, out.hard, s.iter, c.K, slots, n, bits, iters); break;
-    case 2: hipLaunchKernelGGL((extract_kernel<2>), grid, dim3(kBlock), 0, st, out.hard, s.iter, c.K, slots, n, bits, iters); break;
-    case 4: hipLaunchKernelGGL((extract_kernel<4>), grid, dim3(kBlock), 0, st, out.hard, s.iter, c.K, slots, n, bits, iters); break;
-    case 8: hipLaunchKernelGGL((extract_kernel<8>), grid, dim3(kBlock), 0, st, out.hard, s.iter, c.K, slots, n, bits, iters); break;
-    default: return hipErrorInvalidValue;
+  if (lay.fpl() != 1 && lay.fpl() != 2 && lay.fpl() != 4 && lay.fpl() != 8) return hipErrorInvalidValue;
+  for (int i0 = 0; i0 < n; i0 += (int)kMaxGridY) {   // slots i0 .. of the list, at most kMaxGridY a launch
+    const int m = (int)std::min<int64_t>(kMaxGridY, n - i0);
+    const int32_t* sl = slots + i0;
+    uint8_t* b = bits + (int64_t)i0 * c.K;
+    int32_t* it = iters + i0;
+    dim3 grid((unsigned)std::min<int64_t>((c.K + kBlock - 1) / kBlock, 64), (unsigned)m);
+    switch (lay.fpl()) {
+      case 1: hipLaunchKernelGGL((extract_kernel<1>), grid, dim3(kBlock), 0, st, out.hard, s.iter, c.K, sl, m, b, it); break;
+      case 2: hipLaunchKernelGGL((extract_kernel<2>), grid, dim3(kBlock), 0, st, out.hard, s.iter, c.K, sl, m, b, it); break;
+      case 4: hipLaunchKernelGGL((extract_kernel<4>), grid, dim3(kBlock), 0, st, out.hard, s.iter, c.K, sl, m, b, it); break;
+      default: hipLaunchKernelGGL((extract_kernel<8>), grid, dim3(kBlock), 0, st, out.hard, s.iter, c.K, sl, m, b, it); break;
+    }
   }
   return hipGetLastError();
 }

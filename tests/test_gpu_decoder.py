@@ -596,3 +596,27 @@ def test_large_batch_across_the_pass_boundary():
     assert all(np.array_equal(hb[f], hb[f % 64]) for f in (64, 4095, 4096, 8191, 8192))
     ref = oracle_run(oc, llr[:8].astype(np.float64), F.Approximate, 50, fmt=O.MSG_F16, want=("hard", "iters"))
     assert np.array_equal(hb[:8], ref["hard"]) and np.array_equal(it[:8].cpu().numpy(), ref["iters"])
+
+
+def test_one_pass_beyond_the_grid_y_limit():
+    """70000 frames of the Gallager code in one pass: the hard-bit unpack, whose launch grid
+    counts frames in y, is split at 65535 rows (the device reports 65536 as its grid limit in
+    y).  Bits and iteration counts equal the same batch in the default 1024-frame passes, and
+    the oracle on frames either side of the split."""
+    import torch
+    oc, H = gallager()
+    frames, distinct = 70000, 4096
+    base = gallager_llr(oc.N, oc.K, distinct, 2.5, 31).astype(np.float32)
+    idx = np.arange(frames) % distinct
+    x = torch.from_numpy(base).cuda()[torch.from_numpy(idx).cuda()].contiguous().view(-1)
+    one = gpu_decoder(H, F.Approximate, 20)
+    one.setPassFrames(frames)
+    passes = gpu_decoder(H, F.Approximate, 20)
+    b1, i1 = one.decode(x, iterations=True)
+    b2, i2 = passes.decode(x, iterations=True)
+    torch.cuda.synchronize()
+    assert torch.equal(b1, b2) and torch.equal(i1, i2)
+    sample = np.array([0, 1, 65534, 65535, 65536, 69999])
+    ref = oracle_run(oc, base[idx[sample]].astype(np.float64), F.Approximate, 20, want=("hard", "iters"))
+    assert np.array_equal(b1.view(frames, oc.K).cpu().numpy()[sample], ref["hard"])
+    assert np.array_equal(i1.cpu().numpy()[sample], ref["iters"])
