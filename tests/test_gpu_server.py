@@ -81,6 +81,39 @@ def test_server_results_same_alone_and_in_the_mix(inputs, steps):
         assert np.array_equal(got[k][1], ref["iters"])
 
 
+@pytest.mark.parametrize("fmt,words", [(F.MsgFormat.F16, 1), (F.MsgFormat.F32, 2)])
+def test_server_pinned_layouts_over_several_tiles(fmt, words):
+    """Pools in a pinned lane layout (128-frame tiles), four tiles each, fed small requests from
+    device memory over time: rounds admit a few frames into partly filled tiles (the fullest
+    tile with room for the round first).  Every request equals its frames decoded alone."""
+    import torch
+    decs = decoders(fmt=fmt)
+    for d in decs:
+        d.setLayout(words)
+    tw = [d.layout()[1] for d in decs]
+    assert all(t == 128 for t in tw)
+    rng = np.random.default_rng(17 + words)
+    pool = {i: dvbs2_llr(300, CODES[i][2], seed=70 + i, rate=CODES[i][1], n=CODES[i][0]).astype(np.float32)
+            for i in range(len(CODES))}
+    dev = {i: torch.from_numpy(pool[i]).cuda() for i in pool}
+    torch.cuda.synchronize()
+    sv = F.Server(decs, slots=[4 * t for t in tw])
+    reqs, tickets = [], []
+    for _ in range(60):
+        i = int(rng.integers(0, len(CODES)))
+        f = int(rng.integers(1, 24))
+        o = int(rng.integers(0, 300 - f))
+        reqs.append((i, pool[i][o:o + f]))
+        tickets.append(sv.submit(i, dev[i][o:o + f]))
+        time.sleep(0.001)
+    got = [sv.wait(t, timeout=120.0) for t in tickets]
+    sv.close()
+    assert all(g is not None for g in got), "server did not finish"
+    for (i, x), (msg, it, _) in zip(reqs, got):
+        alone, ait = decs[i].decode(x.astype(np.float64), iterations=True)
+        assert np.array_equal(msg, alone) and np.array_equal(it, ait)
+
+
 def test_server_admission_by_memory_budget():
     decs = decoders()
     budget = 256 << 20
