@@ -278,11 +278,15 @@ int run_batch(fecl_ldpc* h, Workspace* ws, hipStream_t st, const BatchIO& io, in
   DevState s;
   DevOut out;
   if (int rc = prepare(h, ws, lay, frames, soft, &s, &out)) return rc;
-  if (!capture && ws->host_active_cap < h->iterations) {
-    if (ws->host_active) (void)hipHostFree(ws->host_active);
-    ws->host_active = nullptr;
-    HIP_TRY(hipHostMalloc((void**)&ws->host_active, (size_t)h->iterations * 4, hipHostMallocDefault));
-    ws->host_active_cap = h->iterations;
+  // one entry per tile per step: regions only shrink, so the first region's tiles bound them
+  const int64_t live_entries = (int64_t)h->iterations * s.tiles;
+  if (!capture && ws->host_live_cap < live_entries) {
+    if (ws->host_live) (void)hipHostFree(ws->host_live);
+    ws->host_live = ws->host_live_dev = nullptr;
+    ws->host_live_cap = 0;
+    HIP_TRY(hipHostMalloc((void**)&ws->host_live, (size_t)live_entries * 4, hipHostMallocMapped));
+    HIP_TRY(hipHostGetDevicePointer((void**)&ws->host_live_dev, ws->host_live, 0));
+    ws->host_live_cap = live_entries;
   }
   DevOpts o = base ? *base : make_opts(h);
   o.nt_rows = nt_rows_for(h, lay, s.tiles);
@@ -295,6 +299,10 @@ int run_batch(fecl_ldpc* h, Workspace* ws, hipStream_t st, const BatchIO& io, in
     }
   }
   int steps_run = 0;
+  // eager decodes: where each step's per-tile live counts sit in ws->host_live, and how many
+  std::vector<int64_t> step_off(capture ? 0 : (size_t)h->iterations);
+  std::vector<int32_t> step_tiles(capture ? 0 : (size_t)h->iterations);
+  int64_t live_off = 0;
   // Compaction (eager decodes): once the live frames two steps back fit in three quarters of
   // the current region's tiles or fewer, and the tile-steps this frees outweigh moving them,
   // they move into the other area and later steps run on that region only.  The region left
@@ -333,7 +341,14 @@ int run_batch(fecl_ldpc* h, Workspace* ws, hipStream_t st, const BatchIO& io, in
     if (prof) HIP_TRY(hipEventRecord(ws->pev[step][1], st));
     HIP_TRY(kernels_variable(lay, h->dc, cur, cur_out, c.max_var_degree, o.fused_syn, (o.nt_rows & 2) != 0, st));
     if (prof) HIP_TRY(hipEventRecord(ws->pev[step][2], st));
-    HIP_TRY(kernels_syndrome_control(lay, h->dc, cur, h->iterations, o.fused_syn, step, c.max_row_degree, st));
+    DevState cs = cur;
+    if (!capture) {   // this step's live frames per tile, for the host's count
+      step_off[step] = live_off;
+      step_tiles[step] = cur.tiles;
+      cs.live_out = ws->host_live_dev + live_off;
+      live_off += cur.tiles;
+    }
+    HIP_TRY(kernels_syndrome_control(lay, h->dc, cs, h->iterations, o.fused_syn, step, c.max_row_degree, st));
     if (prof) HIP_TRY(hipEventRecord(ws->pev[step][3], st));
     steps_run = step + 1;
     if (capture) {
@@ -349,13 +364,13 @@ int run_batch(fecl_ldpc* h, Workspace* ws, hipStream_t st, const BatchIO& io, in
       }
       continue;
     }
-    HIP_TRY(hipMemcpyAsync(&ws->host_active[step], &s.active[step], 4, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipEventRecord(ws->ev[step % 4], st));
     // Stop enqueuing once every frame of the batch has finished: the count of step-2 is read
     // while the last two steps run, so the device queue never drains on the host.
     if (step >= 2) {
       HIP_TRY(hipEventSynchronize(ws->ev[(step - 2) % 4]));
-      const int64_t live = ws->host_active[step - 2];
+      int64_t live = 0;
+      for (int32_t t = 0; t < step_tiles[step - 2]; ++t) live += ws->host_live[step_off[step - 2] + t];
       if (live == 0) break;
       // Live counts only fall, so the frames live now fit in `need` tiles.  Moving a frame
       // costs about one frame-iteration of memory traffic; it is counted 1.5 times.

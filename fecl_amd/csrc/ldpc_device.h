@@ -780,6 +780,7 @@ __device__ static inline bool control_tile(const DevState& s, int iterations, in
     s.tile_flags[t] = (blive ? TF_LIVE : 0u) | (brun ? TF_RUN : 0u) | (uni ? TF_UNIFORM : 0u);
     s.tile_iter[t] = it_ref;
     if (alive && step >= 0) atomicAdd(&s.active[step], alive);
+    if (s.live_out) s.live_out[t] = alive;
   }
   return blive != 0;
 }

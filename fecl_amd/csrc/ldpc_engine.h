@@ -88,8 +88,11 @@ struct Workspace {
   DevBuf map[2], src_of, ctl;
   DevBuf stage_in[2], stage_state[2], stage_out[2];   // host path, double-buffered
   HostBuf pin_in[2], pin_state[2], pin_out[2];          // host path: FP32, pinned
-  int32_t* host_active = nullptr;
-  int host_active_cap = 0;
+  // eager decodes: each step's live frames per tile, written by the control kernel (mapped
+  // page-locked memory; host_live_dev is its device address), steps after one another
+  int32_t* host_live = nullptr;
+  int32_t* host_live_dev = nullptr;
+  int64_t host_live_cap = 0;   // entries
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   hipEvent_t done = nullptr;   // recorded after the last use, waited on by the next user's stream
   std::vector<std::array<hipEvent_t, 4>> pev;   // per-step timing events (profiling only)
@@ -108,7 +111,7 @@ struct Workspace {
     for (HostBuf* b : {&pin_in[0], &pin_in[1], &pin_state[0], &pin_state[1],
                        &pin_out[0], &pin_out[1]})
       b->release();
-    if (host_active) (void)hipHostFree(host_active);
+    if (host_live) (void)hipHostFree(host_live);
     for (auto& e : ev)
       if (e) (void)hipEventDestroy(e);
     if (done) (void)hipEventDestroy(done);

@@ -58,6 +58,9 @@ struct DevState {
   int32_t* tile_iter;      // [tile] iteration count of a TF_UNIFORM tile
   int32_t* active;         // [iterations] frames still iterating after each step
   int32_t tiles;
+  // Eager decodes: [tile] frames still iterating in each tile after this step, written by the
+  // control kernel into mapped page-locked memory that the host reads (null: not written).
+  int32_t* live_out = nullptr;
 };
 
 struct DevOpts {
