@@ -281,6 +281,8 @@ int run_batch(fecl_ldpc* h, Workspace* ws, hipStream_t st, const BatchIO& io, in
   // one entry per tile per step: regions only shrink, so the first region's tiles bound them
   const int64_t live_entries = (int64_t)h->iterations * s.tiles;
   if (!capture && ws->host_live_cap < live_entries) {
+    // growing: control kernels still queued from an earlier decode may write the old buffer
+    if (ws->host_live) HIP_TRY(hipDeviceSynchronize());
     if (ws->host_live) (void)hipHostFree(ws->host_live);
     ws->host_live = ws->host_live_dev = nullptr;
     ws->host_live_cap = 0;
