@@ -114,6 +114,38 @@ def test_server_pinned_layouts_over_several_tiles(fmt, words):
         assert np.array_equal(msg, alone) and np.array_equal(it, ait)
 
 
+def test_server_linear_and_exact_pools():
+    """A server whose pools decode with Linear (FP32 messages) and Exact (BF16 messages), scaling
+    factor 0.75: requests from device memory over time, each equal to its frames decoded alone."""
+    import torch
+    specs = [(0, F.Linear, F.MsgFormat.F32), (1, F.Exact, F.MsgFormat.BF16)]
+    decs = []
+    for ci, alg, fmt in specs:
+        n, rate, _ = CODES[ci]
+        decs.append(F.Ldpc(F.DvbS2.matrix(n, rate), F.DecoderOptions().algorithm(alg).iterations(30)
+                           .scalingFactor(0.75).msgFormat(fmt)))
+    rng = np.random.default_rng(23)
+    pool = {k: dvbs2_llr(120, CODES[ci][2] - 0.4, seed=90 + k, rate=CODES[ci][1], n=CODES[ci][0]).astype(np.float32)
+            for k, (ci, _, _) in enumerate(specs)}
+    dev = {k: torch.from_numpy(pool[k]).cuda() for k in pool}
+    torch.cuda.synchronize()
+    sv = F.Server(decs, slots=[2 * d.layout()[1] for d in decs])
+    reqs, tickets = [], []
+    for _ in range(30):
+        k = int(rng.integers(0, len(specs)))
+        f = int(rng.integers(1, 30))
+        o = int(rng.integers(0, 120 - f))
+        reqs.append((k, pool[k][o:o + f]))
+        tickets.append(sv.submit(k, dev[k][o:o + f]))
+        time.sleep(0.001)
+    got = [sv.wait(t, timeout=120.0) for t in tickets]
+    sv.close()
+    assert all(g is not None for g in got), "server did not finish"
+    for (k, x), (msg, it, _) in zip(reqs, got):
+        alone, ait = decs[k].decode(x.astype(np.float64), iterations=True)
+        assert np.array_equal(msg, alone) and np.array_equal(it, ait)
+
+
 def test_server_admission_by_memory_budget():
     decs = decoders()
     budget = 256 << 20
