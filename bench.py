@@ -833,6 +833,7 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-uneven", action="store_true", help="skip the seven-rate uneven batch leg")
     ap.add_argument("--no-waterfall", action="store_true", help="skip the FP16/BF16 vs FP64 waterfall FER leg")
+    ap.add_argument("--no-plain", action="store_true", help="skip the reference's 16-frame Exact CPU case")
     ap.add_argument("--no-forms", action="store_true", help="skip the Linear / FP16 / BF16 lines")
     ap.add_argument("--ebn0", type=str, default="",
                     help="comma-separated Eb/N0 (dB) of the batches (default: the headline points)")
@@ -1088,6 +1089,10 @@ def main():
                                   "cpu": _cpu_model(), "host_cpus": cores_info}
         if not args.no_waterfall:
             result["waterfall_fer_vs_fp64"] = run_waterfall(F, H, device, threads, args.iterations, args.seed)
+        if not args.no_plain:
+            # the reference's own CPU case (16 Exact frames of FP64 LLRs) beside stock FeCl
+            print("[bench] the reference's CPU case", file=sys.stderr, flush=True)
+            result["reference_cpu_case"] = run_plain(args, device)
     if not args.no_serving or not args.no_uneven:
         # the uneven batch and the serving pools start from a device heap without the batch
         # bench's buffers (decoder workspaces, input batches, torch's cached blocks)
