@@ -247,6 +247,33 @@ def test_graph_compaction_linear_exact_equal_eager(alg, fmt, words):
         assert np.array_equal(g.iterations[0].cpu().numpy(), it.cpu().numpy()), rep
 
 
+def test_graph_with_fused_syndrome():
+    """A hipGraph over a decoder whose syndrome is folded into the check kernel (that form does
+    not compact): on two replays, bits and iteration counts equal the oracle's and the eager
+    decode's."""
+    import torch
+    oc, H = gallager()
+    frames = 512
+    dec = gpu_decoder(H, F.Approximate, ITERS, words=1, fused=True)
+    eager = gpu_decoder(H, F.Approximate, ITERS, words=1, fused=True)
+    x = torch.zeros(frames * oc.N, dtype=torch.float32, device="cuda")
+    g = F.BatchGraph([dec], [x])
+    for rep in range(2):
+        llr = staggered_llr(oc, frames, 140 + rep)
+        ref = oracle_run(oc, llr, F.Approximate, ITERS, want=("hard", "iters"), threads=16)
+        x.copy_(torch.from_numpy(llr.astype(np.float32).ravel()))
+        torch.cuda.synchronize()
+        g.launch()
+        torch.cuda.synchronize()
+        got = g.msg[0].cpu().numpy().reshape(frames, oc.K)
+        assert np.array_equal(got, ref["hard"]), rep
+        assert np.array_equal(g.iterations[0].cpu().numpy(), ref["iters"]), rep
+        bits, it = eager.decode(x, iterations=True)
+        torch.cuda.synchronize()
+        assert np.array_equal(bits.cpu().numpy().reshape(frames, oc.K), got), rep
+        assert np.array_equal(it.cpu().numpy(), ref["iters"]), rep
+
+
 def test_short_frame_compaction_matches_the_oracle():
     """DVB-S2 n=16200 r1/2, 512 frames mixing converging and non-converging Eb/N0 over 8 tiles
     of 64: soDecode outputs bit-identical to the oracle through the compactions."""
