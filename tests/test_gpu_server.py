@@ -81,17 +81,19 @@ def test_server_results_same_alone_and_in_the_mix(inputs, steps):
         assert np.array_equal(got[k][1], ref["iters"])
 
 
-@pytest.mark.parametrize("fmt,words", [(F.MsgFormat.F16, 1), (F.MsgFormat.F32, 2)])
-def test_server_pinned_layouts_over_several_tiles(fmt, words):
-    """Pools in a pinned lane layout (128-frame tiles), four tiles each, fed small requests from
-    device memory over time: rounds admit a few frames into partly filled tiles (the fullest
-    tile with room for the round first).  Every request equals its frames decoded alone."""
+@pytest.mark.parametrize("fmt,words,tile", [(F.MsgFormat.F16, 1, 128), (F.MsgFormat.F32, 2, 128),
+                                            (F.MsgFormat.F32, 4, 256)])
+def test_server_pinned_layouts_over_several_tiles(fmt, words, tile):
+    """Pools in a pinned lane layout (128- or 256-frame tiles), four tiles each, fed small
+    requests from device memory over time: rounds admit a few frames into partly filled tiles
+    (the fullest tile with room for the round first).  Every request equals its frames decoded
+    alone."""
     import torch
     decs = decoders(fmt=fmt)
     for d in decs:
         d.setLayout(words)
     tw = [d.layout()[1] for d in decs]
-    assert all(t == 128 for t in tw)
+    assert all(t == tile for t in tw)
     rng = np.random.default_rng(17 + words)
     pool = {i: dvbs2_llr(300, CODES[i][2], seed=70 + i, rate=CODES[i][1], n=CODES[i][0]).astype(np.float32)
             for i in range(len(CODES))}
