@@ -155,6 +155,29 @@ __device__ static inline float apply_sf(const DevOpts& o, int row, int dcol, flo
 }
 
 // ------------------------------------------------------------------------------------------
+// Lane groups of a tile (TF_GROUP_SHIFT): the 64-bit mask of lanes holding a live frame folded
+// to one bit per kLaneGroup lanes.
+__device__ static inline uint32_t lane_groups(uint64_t lanes) {
+  static_assert(kLaneGroup == 4 && TF_GROUP_SHIFT + kWave / kLaneGroup <= 32, "16 groups of 4 lanes");
+  lanes |= lanes >> 1;
+  lanes |= lanes >> 2;   // bit 4g: any of lanes 4g .. 4g+3
+  uint32_t g = 0;
+#pragma unroll
+  for (int i = 0; i < kWave / kLaneGroup; ++i) g |= (uint32_t)((lanes >> (kLaneGroup * i)) & 1u) << i;
+  return g;
+}
+// The lane whose words this lane's row loads fetch: itself if its group may hold a live frame,
+// else the first lane of the first such group, whose lines the wave reads anyway.  A lane of
+// another group holds no frame that takes an update, so it stores nothing, and what it
+// computes from those words is never kept; its loads then add no memory traffic.  This is
+// where tiles that hold few frames (small batches, the last frames of a batch after a
+// compaction, a serving pool's partly filled tiles) save their row traffic.
+__device__ static inline int lane_for_loads(uint32_t tflags, int lane) {
+  const uint32_t g = tflags >> TF_GROUP_SHIFT;
+  const int first = kLaneGroup * __builtin_ctz(g | (1u << (kWave / kLaneGroup - 1)));
+  return ((g >> (lane / kLaneGroup)) & 1u) ? lane : first;
+}
+
 // Per-lane view of the frames a check-node wavefront serves.
 template <int FPL>
 struct CnLanes {
@@ -261,8 +284,10 @@ __device__ static inline void cn_minsum_row(const DevCode& c, const DevState& s,
   const int d = uniform(c.row_ptr[r + 1]) - e0;
   const uint32_t tflags = uniform((int)s.tile_flags[t]);
   if (!(tflags & TF_LIVE) || d < 0) return;
-  uint32_t* mrow = s.msg + ((int64_t)t * c.E + e0) * RW + lane * F::W;
-  const float* Lt = s.L + (int64_t)t * c.N * TW + lane * FPL;
+  const int ll = lane_for_loads(tflags, lane);
+  uint32_t* mrow = s.msg + ((int64_t)t * c.E + e0) * RW + lane * F::W;   // stores
+  const uint32_t* mld = mrow + (ll - lane) * F::W;                        // row loads
+  const float* Lt = s.L + (int64_t)t * c.N * TW + ll * FPL;
   uint8_t ph[FPL];
   int it[FPL];
   lane_state<FPL>(s, t, tflags, lane, ph, it);
@@ -270,7 +295,7 @@ __device__ static inline void cn_minsum_row(const DevCode& c, const DevState& s,
   if constexpr (DMAX > 0) {
 #pragma unroll
     for (int j = 0; j < DMAX; ++j)
-      if (j < d) F::load(mrow + (int64_t)j * RW, xs[j]);
+      if (j < d) F::load(mld + (int64_t)j * RW, xs[j]);
   }
   CnLanes<FPL> ln;
   if (!cn_decide<FPL>(o, ph, it, ln)) return;
@@ -311,7 +336,7 @@ __device__ static inline void cn_minsum_row(const DevCode& c, const DevState& s,
   } else {
     for (int j = 0; j < d; ++j) {
       float x[FPL];
-      F::load(mrow + (int64_t)j * RW, x);
+      F::load(mld + (int64_t)j * RW, x);
       fresh_x(j, x);
       accumulate(j, x);
     }
@@ -344,7 +369,7 @@ __device__ static inline void cn_minsum_row(const DevCode& c, const DevState& s,
   } else {
     for (int j = 0; j < d; ++j) {
       float x[FPL];
-      F::load(mrow + (int64_t)j * RW, x);
+      F::load(mld + (int64_t)j * RW, x);
       fresh_x(j, x);
       emit(j, x);
     }
@@ -405,15 +430,17 @@ __device__ static inline void cn_minsum_raw_row(const DevCode& c, const DevState
   const int d = uniform(c.row_ptr[r + 1]) - e0;
   const uint32_t tflags = uniform((int)s.tile_flags[t]);
   if (!(tflags & TF_LIVE) || d < 0) return;
-  uint32_t* mrow = s.msg + ((int64_t)t * c.E + e0) * RW + lane * W;
-  const float* Lt = s.L + (int64_t)t * c.N * TW + lane * FPL;
+  const int ll = lane_for_loads(tflags, lane);
+  uint32_t* mrow = s.msg + ((int64_t)t * c.E + e0) * RW + lane * W;   // stores
+  const uint32_t* mld = mrow + (ll - lane) * W;                        // row loads
+  const float* Lt = s.L + (int64_t)t * c.N * TW + ll * FPL;
   uint8_t ph[FPL];
   int it[FPL];
   lane_state<FPL>(s, t, tflags, lane, ph, it);
   uint32_t w[DMAX][W];
 #pragma unroll
   for (int j = 0; j < DMAX; ++j)
-    if (j < d) load_words<W, NT>(mrow + (int64_t)j * RW, w[j]);
+    if (j < d) load_words<W, NT>(mld + (int64_t)j * RW, w[j]);
   CnLanes<FPL> ln;
   if (!cn_decide<FPL>(o, ph, it, ln)) return;
   if (ln.wave_fresh) {   // frames whose v2c is still L[col(e)], rounded to the storage format
@@ -543,10 +570,11 @@ __device__ static inline void vn_vars(const DevCode& c, const DevState& s, const
     for (int j = 0; j < VMAX; ++j) eid[i][j] = c.var_tab[(v0 + i) * VMAX + j];
   const uint32_t tflags = uniform((int)s.tile_flags[t]);
   if (!(tflags & (init_pass ? TF_VNINIT : TF_LIVE)) || eid[0][0] < -1) return;
+  const int ll = lane_for_loads(tflags, lane);
   float Lv[VPW][FPL];
 #pragma unroll
   for (int i = 0; i < VPW; ++i)
-    if (v0 + i < c.N) load_floats<FPL>(s.L + ((int64_t)t * c.N + v0 + i) * TW + lane * FPL, Lv[i]);
+    if (v0 + i < c.N) load_floats<FPL>(s.L + ((int64_t)t * c.N + v0 + i) * TW + ll * FPL, Lv[i]);
   int dv[VPW];
 #pragma unroll
   for (int i = 0; i < VPW; ++i) {
@@ -560,13 +588,14 @@ __device__ static inline void vn_vars(const DevCode& c, const DevState& s, const
   uint64_t fw[FPL];
 #pragma unroll
   for (int k = 0; k < FPL; ++k) fw[k] = s.fail[t * FPL + k];
-  uint32_t* mt = s.msg + (int64_t)t * c.E * RW + lane * W;
+  uint32_t* mt = s.msg + (int64_t)t * c.E * RW + lane * W;   // stores
+  const uint32_t* mtl = mt + (ll - lane) * W;                 // row loads
   uint32_t wv[VPW][VMAX][W];
 #pragma unroll
   for (int i = 0; i < VPW; ++i)
 #pragma unroll
     for (int j = 0; j < VMAX; ++j)
-      if (j < dv[i]) load_words<W, NT>(mt + (int64_t)eid[i][j] * RW, wv[i][j]);
+      if (j < dv[i]) load_words<W, NT>(mtl + (int64_t)eid[i][j] * RW, wv[i][j]);
   bool run[FPL], fin[FPL];
   bool lane_run = false, lane_fin = false;
 #pragma unroll
@@ -777,7 +806,8 @@ __device__ static inline bool control_tile(const DevState& s, int iterations, in
   if (lane == 0) {
 #pragma unroll
     for (int k = 0; k < FPL; ++k) s.fail[t * FPL + k] = 0;
-    s.tile_flags[t] = (blive ? TF_LIVE : 0u) | (brun ? TF_RUN : 0u) | (uni ? TF_UNIFORM : 0u);
+    s.tile_flags[t] = (blive ? TF_LIVE : 0u) | (brun ? TF_RUN : 0u) | (uni ? TF_UNIFORM : 0u) |
+                      (lane_groups(blive) << TF_GROUP_SHIFT);
     s.tile_iter[t] = it_ref;
     if (alive && step >= 0) atomicAdd(&s.active[step], alive);
     if (s.live_out) s.live_out[t] = alive;
@@ -865,7 +895,8 @@ __device__ static inline bool retile_tile(const DevState& s, int t, int lane) {
   if (lane == 0) {
 #pragma unroll
     for (int k = 0; k < FPL; ++k) s.fail[t * FPL + k] = 0;
-    s.tile_flags[t] = (blive ? TF_LIVE : 0u) | (brun ? TF_RUN : 0u) | (uni ? TF_UNIFORM : 0u);
+    s.tile_flags[t] = (blive ? TF_LIVE : 0u) | (brun ? TF_RUN : 0u) | (uni ? TF_UNIFORM : 0u) |
+                      (lane_groups(blive) << TF_GROUP_SHIFT);
     s.tile_iter[t] = it_ref;
   }
   return blive != 0;

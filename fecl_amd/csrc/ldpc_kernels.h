@@ -30,9 +30,14 @@ enum : uint8_t {
 // Tile flags: any frame RUN or FINAL (check and variable kernels work), any frame RUN
 // (syndrome works), any frame waiting for its state-seeded variable pass, and "uniform": every
 // frame of the tile is RUN with no pending flag and the same iteration count (tile_iter), so
-// kernels need not read per-frame state.  Only the control kernel sets TF_UNIFORM; every other
-// writer of a tile's flags clears it.
+// kernels need not read per-frame state.  The control kernel and the retile after a compaction
+// set TF_UNIFORM from the phases; every other writer of a tile's flags clears it.
+// Bits TF_GROUP_SHIFT.. hold the tile's lane groups: bit g is set when lanes
+// kLaneGroup*g .. kLaneGroup*g+kLaneGroup-1 may hold a frame that is RUN or FINAL or waits for
+// its state-seeded pass.  A set bit may be stale (a superset is safe); a clear bit is exact.
+// Check and variable kernels skip the row loads of the other lanes (lane_for_loads).
 enum : uint32_t { TF_LIVE = 1, TF_RUN = 2, TF_VNINIT = 4, TF_UNIFORM = 8 };
+constexpr int TF_GROUP_SHIFT = 16, kLaneGroup = 4;
 
 struct DevCode {
   const int32_t* row_ptr;

@@ -70,15 +70,17 @@ __global__ __launch_bounds__(kBlock) void cn_box_kernel(DevCode c, DevState s, D
   const int d = uniform(c.row_ptr[r + 1]) - e0;
   const uint32_t tflags = uniform((int)s.tile_flags[t]);
   if (!(tflags & TF_LIVE) || d < 0) return;
-  uint32_t* mrow = s.msg + ((int64_t)t * c.E + e0) * RW + lane * F::W;
-  const float* Lt = s.L + (int64_t)t * c.N * TW + lane * FPL;
+  const int ll = lane_for_loads(tflags, lane);
+  uint32_t* mrow = s.msg + ((int64_t)t * c.E + e0) * RW + lane * F::W;   // stores
+  const uint32_t* mld = mrow + (ll - lane) * F::W;                        // row loads
+  const float* Lt = s.L + (int64_t)t * c.N * TW + ll * FPL;
   uint8_t ph[FPL];
   int it[FPL];
   lane_state<FPL>(s, t, tflags, lane, ph, it);
   float x[DMAX][FPL];
 #pragma unroll
   for (int j = 0; j < DMAX; ++j)
-    if (j < d) F::load(mrow + (int64_t)j * RW, x[j]);
+    if (j < d) F::load(mld + (int64_t)j * RW, x[j]);
   CnLanes<FPL> ln;
   if (!cn_decide<FPL>(o, ph, it, ln)) return;
   if (o.fused_syn && ln.wave_seen) cn_syndrome<FPL>(c, s, t, e0, d, lane);
@@ -281,6 +283,10 @@ __global__ __launch_bounds__(kBlock) void ingest_kernel(DevCode c, DevState s, c
         const uint8_t base = init_phase & PH_MASK;
         fl = (base == PH_RUN || base == PH_FINAL ? TF_LIVE : 0u) | (base == PH_RUN ? TF_RUN : 0u) |
              ((init_phase & PH_VNINIT) ? TF_VNINIT : 0u);
+        // slots 0 .. n-1 sit in lanes 0 .. ceil(n / fpl) - 1
+        const int64_t n = live < TW ? live : TW;
+        const int groups = (int)((n + (int64_t)fpl * kLaneGroup - 1) / ((int64_t)fpl * kLaneGroup));
+        fl |= ((1u << groups) - 1u) << TF_GROUP_SHIFT;
       }
       s.tile_flags[t] = fl;
     }
@@ -431,8 +437,10 @@ __global__ __launch_bounds__(kBlock) void admit_kernel(DevCode c, DevState s, co
     s.phase[p] = init_phase;
     s.iter[p] = 0;
     const uint8_t base = init_phase & PH_MASK;
+    const int group = (p % TW) / (TW / kWave) / kLaneGroup;   // slot p sits in lane (p % TW) / FPL
     atomicAnd(&s.tile_flags[t], ~TF_UNIFORM);
-    atomicOr(&s.tile_flags[t], (base == PH_RUN || base == PH_FINAL ? TF_LIVE : 0u) | (base == PH_RUN ? TF_RUN : 0u));
+    atomicOr(&s.tile_flags[t], (base == PH_RUN || base == PH_FINAL ? TF_LIVE : 0u) | (base == PH_RUN ? TF_RUN : 0u) |
+                                   (1u << (TF_GROUP_SHIFT + group)));
   }
 }
 
