@@ -328,22 +328,30 @@ __global__ __launch_bounds__(kBlock) void state_in_kernel(DevCode c, DevState s,
 // Launches whose grid.y counts frames or slots split into launches of at most this many rows
 // (the device's reported grid limit in y is 65536).
 constexpr int64_t kMaxGridY = 65535;
+constexpr int64_t kUnpackBlocks = 8192;
 
-// Hard decisions [tile][K][FPL] -> msg [F][K] bytes (0/1); slot j0 + blockIdx.y.
+// Hard decisions [tile][K][FPL] -> msg [F][K] bytes (0/1); slot j0 + blockIdx.y, bits
+// blockIdx.x * kBlock + threadIdx.x in steps of gridDim.x * kBlock.  A slot whose frame is still
+// decoding (RUN or FINAL) is skipped: it moves into the next region and is handed over by a
+// later call, after its final update (every frame is done by the last call of a decode).
 template <int FPL>
-__global__ __launch_bounds__(kBlock) void unpack_hard_kernel(const uint64_t* hard, int K, int64_t frames,
-                                                             uint8_t* msg, int64_t frame0, const int32_t* map,
-                                                             const CompactCtl* ctl, int64_t j0) {
+__global__ __launch_bounds__(kBlock) void unpack_hard_kernel(const uint64_t* hard, const uint8_t* phase, int K,
+                                                             int64_t frames, uint8_t* msg, int64_t frame0,
+                                                             const int32_t* map, const CompactCtl* ctl, int64_t j0) {
   constexpr int TW = kWave * FPL;
   if (ctl && !ctl->go) return;
   const int64_t j = j0 + blockIdx.y;
   const int64_t f = map ? (int64_t)map[j] : j;
-  const int v = blockIdx.x * kBlock + threadIdx.x;
-  if (v >= K || f < 0 || f >= frames) return;
+  if (f < 0 || f >= frames) return;
+  const int base = phase[j] & PH_MASK;
+  if (base == PH_RUN || base == PH_FINAL) return;
   const int64_t t = j / TW;
   const int p = (int)(j % TW);
-  const uint64_t w = hard[(t * K + v) * FPL + (p % FPL)];
-  msg[(frame0 + f) * K + v] = (uint8_t)((w >> (p / FPL)) & 1ull);
+  const uint64_t* hw = hard + t * K * FPL + (p % FPL);
+  uint8_t* out = msg + (frame0 + f) * K;
+#pragma unroll 4
+  for (int v = blockIdx.x * kBlock + threadIdx.x; v < K; v += gridDim.x * kBlock)
+    out[v] = (uint8_t)((hw[(int64_t)v * FPL] >> (p / FPL)) & 1ull);
 }
 
 // [tile][X][TW] float -> [F][count] (Out = float or double), x < count, through LDS.
@@ -930,9 +938,13 @@ static hipError_t launch_results(const DevCode& c, const DevState& s, const DevO
   const int64_t n = map ? (int64_t)s.tiles * TW : frames;   // slots of the region, or frames
   if (n <= 0) return hipSuccess;
   if (r.msg_bits) {
+    // about kUnpackBlocks workgroups in all (at least one per slot), each striding over the bits:
+    // a hand-over that finds few finished frames, or none, dispatches few empty workgroups
+    const int64_t kb = (c.K + kBlock - 1) / kBlock;
+    const unsigned gx = (unsigned)std::max<int64_t>(1, std::min<int64_t>(kb, kUnpackBlocks / n));
     for (int64_t j0 = 0; j0 < n; j0 += kMaxGridY)
-      hipLaunchKernelGGL((unpack_hard_kernel<FPL>), dim3((c.K + kBlock - 1) / kBlock, (unsigned)std::min(kMaxGridY, n - j0)),
-                         dim3(kBlock), 0, st, out.hard, c.K, frames, r.msg_bits, frame0, map, ctl, j0);
+      hipLaunchKernelGGL((unpack_hard_kernel<FPL>), dim3(gx, (unsigned)std::min(kMaxGridY, n - j0)), dim3(kBlock), 0, st,
+                         out.hard, s.phase, c.K, frames, r.msg_bits, frame0, map, ctl, j0);
   }
   if (r.iterations)
     hipLaunchKernelGGL(iters_out_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, s.iter, n, frames,
