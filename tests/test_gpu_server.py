@@ -148,6 +148,35 @@ def test_server_linear_and_exact_pools():
         assert np.array_equal(msg, alone) and np.array_equal(it, ait)
 
 
+def test_server_admission_into_a_finished_lane_group():
+    """A tile's row loads skip the 4-lane groups that hold no live frame, so an admission must
+    mark its slot's group live.  One 64-frame FP32 tile (one frame per lane, groups of 4
+    slots): request A (4 frames at 4 dB) finishes within a few steps while request B (4 frames
+    at -1 dB, which never converge, 1000 iterations) keeps decoding; request C is then admitted
+    into free slots of groups that hold no live frame.  C finishes while B still decodes, and
+    every request equals its frames decoded alone."""
+    d = F.Ldpc(F.DvbS2.matrix(64800, (1, 2)), F.DecoderOptions().algorithm(F.Approximate).iterations(1000))
+    d.setLayout(1)
+    assert d.layout()[1] == 64
+    a = dvbs2_llr(4, 4.0, seed=201).astype(np.float32)
+    b = dvbs2_llr(4, -1.0, seed=202).astype(np.float32)
+    c = dvbs2_llr(4, 1.8, seed=203).astype(np.float32)
+    sv = F.Server([d], slots=[64])
+    ta, tb = sv.submit(0, a), sv.submit(0, b)
+    ga = sv.wait(ta, timeout=60.0)
+    tc = sv.submit(0, c)
+    gc = sv.wait(tc, timeout=60.0)
+    done_with_c = sv.stats()["frames_done"]
+    gb = sv.wait(tb, timeout=120.0)
+    sv.close()
+    assert ga is not None and gb is not None and gc is not None, "server did not finish"
+    assert done_with_c == 8, "B finished before C: the case this test sets up did not arise"
+    for x, (msg, it, _) in ((a, ga), (b, gb), (c, gc)):
+        alone, ait = d.decode(x.astype(np.float64), iterations=True)
+        assert np.array_equal(msg, alone) and np.array_equal(it, ait)
+    assert int(gb[1].min()) == 1000 and int(ga[1].max()) < 10
+
+
 def test_server_admission_by_memory_budget():
     decs = decoders()
     budget = 256 << 20
