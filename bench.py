@@ -700,7 +700,7 @@ def time_form(F, H, alg, fmt, iterations, batches, frames, K, steps, device):
 
 def host_threads():
     """Threads for the host legs: the CPUs this process may run on (sched_getaffinity), capped
-    by the cgroup CPU quota when one is set and at 16 (a one-GPU box's CPU share)."""
+    by the cgroup CPU quota when one is set and at 16 (the CPU share of one GPU on the test machines)."""
     aff = len(os.sched_getaffinity(0))
     quota = None
     try:

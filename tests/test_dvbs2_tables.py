@@ -7,7 +7,7 @@ the matrix the library builds (information bit 360 g joins exactly the checks of
 bit 360 g + i joins (a + i q) mod M for every address a of the row), so a change to the
 library's tables or to its construction fails one of these checks.  The edge counts and degree
 structure of the seven normal rates the batching and serving workloads use are also checked
-against the figures the survey took from stock FeCl's tables."""
+against figures counted from stock FeCl's tables."""
 import hashlib
 import json
 import os
