@@ -330,14 +330,16 @@ __global__ __launch_bounds__(kBlock) void state_in_kernel(DevCode c, DevState s,
 constexpr int64_t kMaxGridY = 65535;
 constexpr int64_t kUnpackBlocks = 8192;
 
-// Hard decisions [tile][K][FPL] -> msg [F][K] bytes (0/1); slot j0 + blockIdx.y, bits
-// blockIdx.x * kBlock + threadIdx.x in steps of gridDim.x * kBlock.  A slot whose frame is still
-// decoding (RUN or FINAL) is skipped: it moves into the next region and is handed over by a
-// later call, after its final update (every frame is done by the last call of a decode).
+// Hard decisions [tile][K][FPL] -> msg [F][K] bytes (0/1), and with `iters` the iteration
+// count; slot j0 + blockIdx.y, bits blockIdx.x * kBlock + threadIdx.x in steps of
+// gridDim.x * kBlock.  A slot whose frame is still decoding (RUN or FINAL) is skipped: it moves
+// into the next region and is handed over by a later call, after its final update (every frame
+// is done by the last call of a decode).
 template <int FPL>
-__global__ __launch_bounds__(kBlock) void unpack_hard_kernel(const uint64_t* hard, const uint8_t* phase, int K,
-                                                             int64_t frames, uint8_t* msg, int64_t frame0,
-                                                             const int32_t* map, const CompactCtl* ctl, int64_t j0) {
+__global__ __launch_bounds__(kBlock) void unpack_hard_kernel(const uint64_t* hard, const uint8_t* phase,
+                                                             const int32_t* iter, int K, int64_t frames, uint8_t* msg,
+                                                             int32_t* iters, int64_t frame0, const int32_t* map,
+                                                             const CompactCtl* ctl, int64_t j0) {
   constexpr int TW = kWave * FPL;
   if (ctl && !ctl->go) return;
   const int64_t j = j0 + blockIdx.y;
@@ -345,6 +347,7 @@ __global__ __launch_bounds__(kBlock) void unpack_hard_kernel(const uint64_t* har
   if (f < 0 || f >= frames) return;
   const int base = phase[j] & PH_MASK;
   if (base == PH_RUN || base == PH_FINAL) return;
+  if (iters && blockIdx.x == 0 && threadIdx.x == 0) iters[frame0 + f] = iter[j];
   const int64_t t = j / TW;
   const int p = (int)(j % TW);
   const uint64_t* hw = hard + t * K * FPL + (p % FPL);
@@ -944,11 +947,11 @@ static hipError_t launch_results(const DevCode& c, const DevState& s, const DevO
     const unsigned gx = (unsigned)std::max<int64_t>(1, std::min<int64_t>(kb, kUnpackBlocks / n));
     for (int64_t j0 = 0; j0 < n; j0 += kMaxGridY)
       hipLaunchKernelGGL((unpack_hard_kernel<FPL>), dim3(gx, (unsigned)std::min(kMaxGridY, n - j0)), dim3(kBlock), 0, st,
-                         out.hard, s.phase, c.K, frames, r.msg_bits, frame0, map, ctl, j0);
-  }
-  if (r.iterations)
+                         out.hard, s.phase, s.iter, c.K, frames, r.msg_bits, r.iterations, frame0, map, ctl, j0);
+  } else if (r.iterations) {
     hipLaunchKernelGGL(iters_out_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, s.iter, n, frames,
                        r.iterations, frame0, map, ctl);
+  }
   auto to_frames = [&](const float* in, int64_t X, int64_t count, void* dst) {
     dim3 grid((unsigned)((count + 63) / 64), s.tiles);
     if (dbl) hipLaunchKernelGGL((tiles_to_frames_kernel<double, TW>), grid, dim3(kBlock), 0, st, in, X, count,
